@@ -1,0 +1,343 @@
+/*
+ * vertexsurge.h — C ABI of the MI355X (gfx950) variable-length expansion path.
+ *
+ * This is the drop-in boundary for VertexSurge's VExpand and Mintersect
+ * operators.  The reference runs both operators on the host over
+ * memory-mapped graph columns reached through a global `Graph*`
+ * (src/include/operator/physical_operator.hh:58, src/objects/storage/
+ * graph_store.cpp:70).  Here the CSR is uploaded to HBM once into a graph
+ * handle, and the operators' `run_internal` bodies call the entry points
+ * below.  Plain pointers and sizes only: no C++ or torch types cross the ABI,
+ * and no exception leaves it.  Every entry point returns a vs_status; on
+ * failure `vs_last_error()` returns a message for the calling thread.
+ *
+ * Bit-matrix conventions (src/include/common/common.hh:219-282, MASK64):
+ *   bit (r, c) = "vertex c is reachable in the set of source row r";
+ *   within a 64-bit word the bit position is r % 64, least significant first.
+ *
+ * Handles are not thread-safe, matching the reference's single-caller
+ * operators (physical_operator.cpp:62-101).
+ */
+#ifndef VERTEXSURGE_H_
+#define VERTEXSURGE_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+typedef enum {
+  VS_OK = 0,
+  VS_ERR_INVALID = 1,     /* bad argument (replaces the reference's assert/exit) */
+  VS_ERR_NOMEM = 2,       /* device or host allocation failed */
+  VS_ERR_HIP = 3,         /* a HIP runtime call failed */
+  VS_ERR_CAPACITY = 4,    /* caller buffer too small; size reported back */
+  VS_ERR_UNSUPPORTED = 5, /* option not implemented */
+  VS_ERR_NODEVICE = 6     /* no GPU visible */
+} vs_status;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *vs_last_error(void);
+/* Library version string. */
+const char *vs_version(void);
+
+/* ------------------------------------------------------------- enums */
+/* ExpandPathType, src/include/operator/expand.hh:25-29. */
+typedef enum {
+  VS_PATH_ANY = 0,           /* walks of length in [k_min, k_max] */
+  VS_PATH_SHORTEST = 1,      /* BFS distance in [k_min, k_max] */
+  VS_PATH_FIND_SHORTEST = 2  /* per-row distance to a given target */
+} vs_path_type;
+
+/* Direction, src/include/common/common.hh:154-159 (Undecide is resolved by
+ * the caller from the edge schema before reaching the ABI). */
+typedef enum {
+  VS_DIR_FORWARD = 0,  /* follow CSR edges u -> adj[u] */
+  VS_DIR_BACKWARD = 1, /* follow reversed edges */
+  VS_DIR_BOTH = 2      /* union of both (== FORWARD on a symmetric graph) */
+} vs_direction;
+
+/* Per-hop expansion scheduling.  The reference's ExpandStrategy::Dense runs
+ * every edge every hop (expand.cpp:263-311); Adaptive switches on a density
+ * ratio (expand.cpp:346-414).  On the GPU every choice below yields the same
+ * bits; only the time differs. */
+typedef enum {
+  VS_SCHED_HEURISTIC = 0, /* push/pull by frontier-edge density */
+  VS_SCHED_PUSH = 1,      /* always sparse scatter */
+  VS_SCHED_PULL = 2,      /* always dense gather */
+  VS_SCHED_LEARNED = 3    /* MLP attached to the graph; heuristic fallback */
+} vs_sched;
+
+/* vs_vexpand_params.flags */
+#define VS_VX_TIMING 0x1u /* record per-hop / per-launch HIP event timings */
+
+/* Result export layouts. */
+typedef enum {
+  VS_LAYOUT_NATIVE = 0,   /* [tile][vertex][tile_words] uint64 words */
+  VS_LAYOUT_DVSM = 1,     /* reference DenseVarStackMatrix, 1024-row stacks
+                             (src/include/storage/bitmap.hh:329-425) */
+  VS_LAYOUT_ROWMAJOR = 2  /* row r = ceil(V/64) words, bit c%64 of word c/64 */
+} vs_layout;
+
+/* --------------------------------------------------------- host CSR data */
+/* A host-side CSR (uint64 offsets[V+1], uint32 adj[E], each list sorted
+ * ascending, no duplicates) produced by the synthetic generators below.
+ * Mirrors the *_csr_src_dst / *_index columns of the reference
+ * (src/include/meta/graph_loader.hh:163-182). */
+typedef struct vs_csr_s *vs_csr_t;
+
+/* Symmetrized R-MAT: 2^scale vertices, edge_factor * 2^scale generated pairs,
+ * quadrant probabilities a, b, c (d = 1-a-b-c); self-loops and duplicates
+ * removed.  Counter-based: the same (scale, ef, a, b, c, seed) gives the same
+ * CSR on every machine and thread count. */
+vs_status vs_rmat_csr(uint32_t scale, uint32_t edge_factor, double a, double b,
+                      double c, uint64_t seed, vs_csr_t *out);
+/* Symmetrized Erdos-Renyi G(V, M): M uniform pairs, same cleanup. */
+vs_status vs_er_csr(uint32_t num_vertices, uint64_t num_pairs, uint64_t seed,
+                    vs_csr_t *out);
+/* CSR from an explicit edge list (src[i] -> dst[i]); optionally symmetrized;
+ * self-loops kept unless drop_self_loops; duplicates removed. */
+vs_status vs_csr_from_edges(uint32_t num_vertices, const uint32_t *src,
+                            const uint32_t *dst, uint64_t n, int symmetrize,
+                            int drop_self_loops, vs_csr_t *out);
+vs_status vs_csr_info(vs_csr_t csr, uint32_t *num_vertices, uint64_t *num_edges,
+                      const uint64_t **offsets, const uint32_t **adj);
+vs_status vs_csr_free(vs_csr_t csr);
+
+/* S distinct vertices drawn uniformly from [0, V), sorted ascending
+ * (the contract of uniform_random_numbers, src/include/utils/utils.hh:74-99;
+ * the draw sequence is this library's own counter-based one). */
+vs_status vs_sample_sources(uint32_t num_vertices, uint32_t S, uint64_t seed,
+                            uint32_t *out);
+/* T random 2-hop paths a-b-c: a uniform over non-isolated vertices, b uniform
+ * in N(a), c uniform in N(b).  out holds 3*T vertex ids, row-major. */
+vs_status vs_sample_paths3(const uint64_t *offsets, const uint32_t *adj,
+                           uint32_t num_vertices, uint64_t T, uint64_t seed,
+                           uint32_t *out);
+
+/* ----------------------------------------------------------------- device */
+vs_status vs_device_count(int *count);
+vs_status vs_set_device(int device);
+vs_status vs_device_synchronize(void);
+/* Raw device buffers for callers that keep inputs resident in HBM. */
+vs_status vs_dev_alloc(size_t bytes, void **dptr);
+vs_status vs_dev_free(void *dptr);
+vs_status vs_memcpy_h2d(void *dst, const void *src, size_t bytes);
+vs_status vs_memcpy_d2h(void *dst, const void *src, size_t bytes);
+
+/* ------------------------------------------------------------------ graph */
+/* Replaces the global PhysicalOperator::graph_store and its
+ * get_edges_refs/get_csr_edge_refs views (graph_store.cpp:75-189): the CSR is
+ * copied to HBM once and every later call reads it there. */
+typedef struct vs_graph_s *vs_graph_t;
+
+#define VS_GRAPH_SYMMETRIC 0x1u /* adj is symmetric: reverse CSR == CSR */
+
+vs_status vs_graph_create(const uint64_t *offsets, const uint32_t *adj,
+                          uint32_t num_vertices, uint64_t num_edges,
+                          uint32_t flags, vs_graph_t *out);
+vs_status vs_graph_destroy(vs_graph_t g);
+vs_status vs_graph_info(vs_graph_t g, uint32_t *num_vertices,
+                        uint64_t *num_edges, uint64_t *device_bytes);
+/* Release cached device workspace (the per-graph buffer pool). */
+vs_status vs_graph_trim(vs_graph_t g);
+
+/* Push/pull tuning attached to a graph (written by vs_autotune_vexpand or
+ * by the caller). */
+typedef struct {
+  uint32_t tile_words;  /* storage tile width in 64-bit words: 1..64, pow2 */
+  uint32_t push_alpha;  /* push while frontier-edges * alpha < E */
+  uint32_t pull_beta;   /* unused when 0; pull->push when popcount*beta < rows*V */
+  uint32_t reserved;
+} vs_tuning;
+vs_status vs_graph_set_tuning(vs_graph_t g, const vs_tuning *t);
+vs_status vs_graph_get_tuning(vs_graph_t g, vs_tuning *t);
+
+/* ---------------------------------------------------------------- VExpand */
+/* Parameters of VExpand (big_operators.hh:101-129, vexpand.cpp:7-101). */
+typedef struct {
+  uint32_t k_min;
+  uint32_t k_max;
+  int32_t path_type;     /* vs_path_type: ANY or SHORTEST */
+  int32_t direction;     /* vs_direction */
+  int32_t with_distance; /* export every layer L_i, i in [k_min, k_actual_max] */
+  int32_t sched;         /* vs_sched */
+  uint32_t tile_words;   /* 0 = graph tuning */
+  uint32_t flags;        /* VS_VX_* */
+} vs_vexpand_params;
+
+#define VS_MAX_HOPS 64
+
+typedef struct {
+  uint32_t k_actual_max; /* VExpand::k_actual_max (vexpand.cpp:222-233) */
+  uint32_t tile_words;
+  uint32_t num_tiles;
+  uint32_t reserved;
+  uint64_t te;           /* traversed edges: sum_i sum_u deg(u)*popcount(F_i[.,u]) */
+  uint64_t bytes_alg;    /* compulsory HBM bytes (words touched once) */
+  uint64_t visit_count;  /* ExpandStatus::visit_count after the last hop */
+  double ms_total;       /* device time of init + hops + finalize */
+  double ms_hops;        /* device time of the hop loop only */
+  /* per hop i = 1..k_actual_max at index i-1 */
+  double ms_hop[VS_MAX_HOPS];
+  int32_t mode_hop[VS_MAX_HOPS];   /* 1 = push, 2 = pull (last tile's) */
+  uint64_t te_hop[VS_MAX_HOPS];
+  uint64_t bytes_hop[VS_MAX_HOPS];
+  uint64_t frontier_bits[VS_MAX_HOPS]; /* popcount of F_i (input of hop i) */
+  uint64_t new_bits[VS_MAX_HOPS];      /* popcount of L_i */
+  /* device time of the dominant kernel launches (pull / push) */
+  double ms_pull;
+  double ms_push;
+  uint64_t n_pull;
+  uint64_t n_push;
+  uint64_t bytes_pull; /* algorithmic bytes of the hops run as pull */
+  uint64_t bytes_push;
+} vs_vexpand_stats;
+
+typedef struct vs_result_s *vs_result_t;
+
+/* Multi-source VExpand over the bit-matrix representation
+ * (VExpand::before_execution/after_child/run_internal,
+ *  vexpand.cpp:166-483; Expand<DenseVarStackMatrix>::expand_dense,
+ *  expand.cpp:210-344).  S == 0 returns an empty result with VS_OK (the
+ * reference warns and skips, vexpand.cpp:170-173).  Sources may repeat.
+ * `stats` may be NULL. */
+vs_status vs_vexpand(vs_graph_t g, const uint32_t *sources, uint32_t S,
+                     const vs_vexpand_params *p, vs_result_t *out,
+                     vs_vexpand_stats *stats);
+
+/* FindShortest (vexpand.cpp:16-25, 186-205, 237-276): dist[r] = BFS distance
+ * from sources[r] to targets[r], -1 if not found within k_max hops
+ * (K_MAX_FIND_SHORTEST = 50 in the reference, expand.hh:37). */
+vs_status vs_find_shortest(vs_graph_t g, const uint32_t *sources,
+                           const uint32_t *targets, uint32_t S,
+                           int32_t direction, uint32_t k_max, int32_t *dist,
+                           vs_vexpand_stats *stats);
+
+/* Result = VarRowMatrixTable (one merged matrix) or MultipleMatrixTable (one
+ * matrix per distance) (vexpand.cpp:298-378, matrix_table.hh:44-149). */
+vs_status vs_result_info(vs_result_t r, uint32_t *rows, uint32_t *cols,
+                         uint32_t *num_matrices, uint32_t *k_actual_max);
+/* distance tag of matrix i (with_distance), or UINT32_MAX for a merged one */
+vs_status vs_result_distance(vs_result_t r, uint32_t i, uint32_t *distance);
+/* tuple_count(): popcount of matrix i (bit_matrix_table.cpp:112) */
+vs_status vs_result_count(vs_result_t r, uint32_t i, uint64_t *count);
+/* per-source reachable counts: popcount of each row of matrix i */
+vs_status vs_result_row_counts(vs_result_t r, uint32_t i, uint64_t *counts);
+/* per-row 64-bit digest of matrix i (order-independent hash of set bits) */
+vs_status vs_result_row_digests(vs_result_t r, uint32_t i, uint64_t *digests);
+vs_status vs_result_export_size(vs_result_t r, uint32_t i, int32_t layout,
+                                size_t *bytes);
+vs_status vs_result_export(vs_result_t r, uint32_t i, int32_t layout,
+                           void *host_buf, size_t bytes);
+/* single bit (DenseVarStackMatrix::get, bitmap.cpp:953-956) */
+vs_status vs_result_get(vs_result_t r, uint32_t i, uint32_t row, uint32_t col,
+                        int *bit);
+vs_status vs_result_free(vs_result_t r);
+
+/* ------------------------------------------------------- batched VExpand */
+typedef struct {
+  const uint32_t *sources;
+  uint32_t S;
+  uint32_t k_min;
+  uint32_t k_max;
+  int32_t path_type; /* ANY or SHORTEST */
+} vs_query;
+
+typedef struct {
+  uint32_t tile_words;  /* 0 = graph tuning */
+  int32_t sched;        /* vs_sched for the per-(hop, tile) choice */
+  uint32_t use_graph;   /* capture the launch sequence in a hipGraph */
+  uint32_t replays;     /* >1: replay the captured graph this many times (timing) */
+  size_t mem_budget;    /* device bytes for tile state; 0 = 70% of free */
+  int32_t direction;
+  uint32_t flags;
+} vs_batch_params;
+
+typedef struct {
+  uint64_t te;
+  double ms_total;      /* device time of all waves (last replay) */
+  double ms_capture;
+  uint32_t num_tiles;
+  uint32_t num_waves;   /* groups of tiles resident at once */
+  uint32_t launches;    /* kernel nodes per replay */
+  uint32_t reserved;
+  uint64_t push_decisions;
+  uint64_t pull_decisions;
+} vs_batch_stats;
+
+/* Runs nq independent VExpand queries packed into shared bit-matrix tiles.
+ * Outputs, concatenated over queries in order (row offset of query q =
+ * sum of S of the queries before it): per-row reachable counts and per-row
+ * digests of the merged result.  Either output may be NULL. */
+vs_status vs_vexpand_batch(vs_graph_t g, const vs_query *queries, uint32_t nq,
+                           const vs_batch_params *p, uint64_t *row_counts,
+                           uint64_t *row_digests, vs_batch_stats *stats);
+
+/* ------------------------------------------------------------- Mintersect */
+typedef struct {
+  double ms;            /* device time of count + scan + fill */
+  uint64_t total;       /* sum of counts */
+  uint64_t bytes_alg;
+} vs_mintersect_stats;
+
+/* Multi-way intersection (Mintersect::run_internal, mintersect.cpp:48-345):
+ * for tuple t, the set  N_0(x_t0) ∩ N_1(x_t1) ∩ ... ∩ N_{arity-1}(x_t,arity-1)
+ * where N_d is the adjacency of deps[d] (forward).  tuples is T*arity ids,
+ * row-major.  counts[t] = |set|; when lists != NULL the sets are written in
+ * ascending id order at lists[offsets[t] .. offsets[t+1]) (offsets has T+1
+ * entries) and list_cap must be >= the total, else VS_ERR_CAPACITY with
+ * *total set.  A vertex id >= V of its dependency gives the empty set (the
+ * reference's missing-column case, mintersect.cpp:143-155). */
+vs_status vs_mintersect(const vs_graph_t *deps, uint32_t arity,
+                        const uint32_t *tuples, uint64_t T, uint32_t *counts,
+                        uint64_t *offsets, uint32_t *lists, uint64_t list_cap,
+                        uint64_t *total, vs_mintersect_stats *stats);
+/* Same with every array already in device memory (from vs_dev_alloc). */
+vs_status vs_mintersect_device(const vs_graph_t *deps, uint32_t arity,
+                               const uint32_t *d_tuples, uint64_t T,
+                               uint32_t *d_counts, uint64_t *d_offsets,
+                               uint32_t *d_lists, uint64_t list_cap,
+                               uint64_t *total, vs_mintersect_stats *stats);
+
+/* ------------------------------------------------- learned scheduler (MLP) */
+/* 3-layer FP32 MLP: in -> 64 -> 64 -> out, ReLU, trained and evaluated by
+ * this library's own HIP kernels. */
+typedef struct vs_sched_s *vs_sched_t;
+
+#define VS_MLP_IN 8
+#define VS_MLP_HIDDEN 64
+#define VS_MLP_OUT 8
+
+vs_status vs_sched_create(uint64_t seed, vs_sched_t *out);
+vs_status vs_sched_free(vs_sched_t s);
+vs_status vs_sched_num_params(size_t *n);
+vs_status vs_sched_get_params(vs_sched_t s, float *params, size_t n);
+vs_status vs_sched_set_params(vs_sched_t s, const float *params, size_t n);
+/* Loss and gradient of masked MSE over a batch (one fwd + bwd on the GPU). */
+vs_status vs_sched_loss_grad(vs_sched_t s, const float *X, const float *Y,
+                             const float *M, uint32_t n, float *loss,
+                             float *grad);
+/* Full-batch Adam training for `epochs` steps; loss_curve may be NULL. */
+vs_status vs_sched_train(vs_sched_t s, const float *X, const float *Y,
+                         const float *M, uint32_t n, uint32_t epochs, float lr,
+                         float *loss_curve);
+vs_status vs_sched_predict(vs_sched_t s, const float *X, uint32_t n, float *Y);
+/* Attach (or detach with NULL) a scheduler used by VS_SCHED_LEARNED. */
+vs_status vs_graph_set_scheduler(vs_graph_t g, vs_sched_t s);
+
+/* -------------------------------------------------------------- autotuner */
+/* Times candidate tile widths and push thresholds on the given sources and
+ * stores the fastest in the graph's tuning. */
+vs_status vs_autotune_vexpand(vs_graph_t g, const uint32_t *sources, uint32_t S,
+                              uint32_t k_max, int32_t path_type,
+                              vs_tuning *best, double *best_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VERTEXSURGE_H_ */

@@ -1,0 +1,59 @@
+"""GPU parity of the multi-way intersection against the CPU oracles."""
+import numpy as np
+import pytest
+
+import vertexsurge_amd as vs
+from test_oracle import _mintersect_kat_deps
+
+pytestmark = pytest.mark.gpu
+
+
+def test_reference_kat():
+    (off, adj), base = _mintersect_kat_deps()
+    dep = vs.Graph(off, adj, symmetric=False)
+    counts, offsets, lists, st = vs.mintersect([dep, dep], base)
+    assert counts.tolist() == [1, 1, 1, 0, 0, 0, 0, 0]
+    assert lists.tolist() == [1, 1, 1] and st["total"] == 3
+    assert offsets.tolist() == [0, 1, 2, 3, 3, 3, 3, 3, 3]
+
+
+@pytest.mark.parametrize("scale,T", [(12, 5000), (16, 40000)])
+def test_rmat_paths(orc, scale, T):
+    g = vs.HostCSR.rmat(scale, 16, seed=scale)
+    G = vs.Graph.from_csr(g)
+    tup = vs.sample_paths3(g.offsets, g.adj, T, seed=3)
+    counts, offsets, lists, st = vs.mintersect([G, G, G], tup)
+    c2, o2, l2 = orc.mintersect_sorted([(g.offsets, g.adj)] * 3, tup)
+    assert np.array_equal(counts, c2)
+    assert np.array_equal(offsets, o2)
+    assert np.array_equal(lists, l2)
+    # invariants: ascending, duplicate-free, subset of each list
+    for t in range(0, T, max(1, T // 50)):
+        seg = lists[offsets[t]:offsets[t + 1]]
+        assert np.all(np.diff(seg.astype(np.int64)) > 0)
+        for x in tup[t]:
+            assert np.isin(seg, g.adj[g.offsets[x]:g.offsets[x + 1]]).all()
+
+
+def test_arities_and_bad_ids(orc):
+    g = vs.HostCSR.rmat(11, 16, seed=4)
+    G = vs.Graph.from_csr(g)
+    rng = np.random.default_rng(0)
+    hubs = np.argsort(g.degrees())[-20:]
+    for arity in (1, 2, 4, 8):
+        tup = rng.choice(hubs, size=(500, arity)).astype(np.uint32)
+        tup[::50, 0] = g.V + 5  # an id outside the dependency: empty set
+        counts, offsets, lists, _ = vs.mintersect([G] * arity, tup)
+        c2, o2, l2 = orc.mintersect_sorted([(g.offsets, g.adj)] * arity, tup)
+        assert np.array_equal(counts, c2) and np.array_equal(lists, l2)
+        assert np.all(counts[::50] == 0)
+
+
+def test_counts_only_and_empty():
+    g = vs.HostCSR.rmat(10, 8, seed=1)
+    G = vs.Graph.from_csr(g)
+    tup = vs.sample_paths3(g.offsets, g.adj, 100, seed=1)
+    c, o, l, st = vs.mintersect([G, G, G], tup, with_lists=False)
+    assert l is None and st["total"] == int(c.sum())
+    c, o, l, st = vs.mintersect([G, G, G], np.zeros((0, 3), dtype=np.uint32))
+    assert len(c) == 0 and o.tolist() == [0]

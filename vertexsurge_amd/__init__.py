@@ -1,0 +1,378 @@
+"""MI355X (gfx950) VExpand / multi-way intersection, Python side.
+
+Thin numpy-facing wrappers over the C ABI of ``libvertexsurge.so``
+(include/vertexsurge.h).  All computation happens in the library's HIP
+kernels; this module only moves arrays and handles.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import (DIR_BACKWARD, DIR_BOTH, DIR_FORWARD, LAYOUT_DVSM, LAYOUT_NATIVE,
+                   LAYOUT_ROWMAJOR, PATH_ANY, PATH_FIND_SHORTEST, PATH_SHORTEST,
+                   SCHED_HEURISTIC, SCHED_LEARNED, SCHED_PULL, SCHED_PUSH, VSError)
+
+__all__ = [
+    "HostCSR", "Graph", "VExpandResult", "vexpand", "find_shortest", "mintersect",
+    "sample_sources", "sample_paths3", "Scheduler", "vexpand_batch", "device_count",
+    "PATH_ANY", "PATH_SHORTEST", "PATH_FIND_SHORTEST", "DIR_FORWARD", "DIR_BACKWARD",
+    "DIR_BOTH", "SCHED_HEURISTIC", "SCHED_PUSH", "SCHED_PULL", "SCHED_LEARNED",
+    "LAYOUT_NATIVE", "LAYOUT_DVSM", "LAYOUT_ROWMAJOR", "VSError",
+]
+
+_PATHS = {"any": PATH_ANY, "shortest": PATH_SHORTEST}
+_DIRS = {"forward": DIR_FORWARD, "backward": DIR_BACKWARD, "both": DIR_BOTH}
+_SCHED = {"heuristic": SCHED_HEURISTIC, "push": SCHED_PUSH, "pull": SCHED_PULL,
+          "learned": SCHED_LEARNED}
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _ptr(a: np.ndarray, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    _lib.call("vs_device_count", C.byref(n))
+    return n.value
+
+
+class HostCSR:
+    """Host CSR produced by the library's generators (offsets uint64[V+1], adj uint32[E])."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+        V = C.c_uint32()
+        E = C.c_uint64()
+        off = _lib.u64p()
+        adj = _lib.u32p()
+        _lib.call("vs_csr_info", handle, C.byref(V), C.byref(E), C.byref(off), C.byref(adj))
+        self.V = V.value
+        self.E = E.value
+        self.offsets = np.ctypeslib.as_array(off, shape=(self.V + 1,))
+        self.adj = (np.ctypeslib.as_array(adj, shape=(self.E,)) if self.E
+                    else np.zeros(0, dtype=np.uint32))
+
+    @classmethod
+    def rmat(cls, scale: int, edge_factor: int, a=0.57, b=0.19, c=0.19, seed=1) -> "HostCSR":
+        h = C.c_void_p()
+        _lib.call("vs_rmat_csr", scale, edge_factor, a, b, c, seed, C.byref(h))
+        return cls(h)
+
+    @classmethod
+    def erdos_renyi(cls, num_vertices: int, num_pairs: int, seed=1) -> "HostCSR":
+        h = C.c_void_p()
+        _lib.call("vs_er_csr", num_vertices, num_pairs, seed, C.byref(h))
+        return cls(h)
+
+    @classmethod
+    def from_edges(cls, num_vertices: int, src, dst, symmetrize=True,
+                   drop_self_loops=True) -> "HostCSR":
+        s, d = _u32(src), _u32(dst)
+        h = C.c_void_p()
+        _lib.call("vs_csr_from_edges", num_vertices, _ptr(s, C.c_uint32), _ptr(d, C.c_uint32),
+                  len(s), int(symmetrize), int(drop_self_loops), C.byref(h))
+        return cls(h)
+
+    def degrees(self) -> np.ndarray:
+        return np.diff(self.offsets)
+
+    def close(self):
+        if self._h:
+            _lib.call("vs_csr_free", self._h)
+            self._h = None
+            self.offsets = self.adj = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sample_sources(num_vertices: int, S: int, seed: int) -> np.ndarray:
+    out = np.zeros(S, dtype=np.uint32)
+    _lib.call("vs_sample_sources", num_vertices, S, seed, _ptr(out, C.c_uint32))
+    return out
+
+
+def sample_paths3(offsets: np.ndarray, adj: np.ndarray, T: int, seed: int) -> np.ndarray:
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    a = _u32(adj)
+    out = np.zeros((T, 3), dtype=np.uint32)
+    _lib.call("vs_sample_paths3", _ptr(off, C.c_uint64), _ptr(a, C.c_uint32), len(off) - 1,
+              T, seed, _ptr(out, C.c_uint32))
+    return out
+
+
+class Graph:
+    """CSR resident in HBM (replaces the reference's global Graph* store)."""
+
+    def __init__(self, offsets: np.ndarray, adj: np.ndarray, symmetric: bool = True):
+        self._off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        self._adj = _u32(adj)
+        self.V = len(self._off) - 1
+        self.E = int(self._off[-1])
+        self._h = C.c_void_p()
+        _lib.call("vs_graph_create", _ptr(self._off, C.c_uint64), _ptr(self._adj, C.c_uint32),
+                  self.V, self.E, _lib.GRAPH_SYMMETRIC if symmetric else 0, C.byref(self._h))
+        self._off = self._adj = None  # the library keeps its own copy
+
+    @classmethod
+    def from_csr(cls, csr: HostCSR, symmetric: bool = True) -> "Graph":
+        return cls(csr.offsets, csr.adj, symmetric)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_tuning(self, tile_words=0, push_alpha=16):
+        t = _lib.Tuning(tile_words, push_alpha, 0, 0)
+        _lib.call("vs_graph_set_tuning", self._h, C.byref(t))
+
+    def get_tuning(self) -> dict:
+        t = _lib.Tuning()
+        _lib.call("vs_graph_get_tuning", self._h, C.byref(t))
+        return {"tile_words": t.tile_words, "push_alpha": t.push_alpha}
+
+    def set_scheduler(self, sched: "Scheduler | None"):
+        _lib.call("vs_graph_set_scheduler", self._h, sched.handle if sched else None)
+
+    def trim(self):
+        _lib.call("vs_graph_trim", self._h)
+
+    def close(self):
+        if self._h:
+            _lib.call("vs_graph_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def stats_dict(st: _lib.VexpandStats) -> dict:
+    k = st.k_actual_max
+    return {
+        "k_actual_max": k, "tile_words": st.tile_words, "num_tiles": st.num_tiles,
+        "te": st.te, "bytes_alg": st.bytes_alg, "visit_count": st.visit_count,
+        "ms_total": st.ms_total, "ms_hops": st.ms_hops,
+        "ms_hop": list(st.ms_hop[:k]), "mode_hop": list(st.mode_hop[:k]),
+        "te_hop": list(st.te_hop[:k]), "bytes_hop": list(st.bytes_hop[:k]),
+        "frontier_bits": list(st.frontier_bits[:k]), "new_bits": list(st.new_bits[:k]),
+        "ms_pull": st.ms_pull, "ms_push": st.ms_push, "n_pull": st.n_pull,
+        "n_push": st.n_push, "bytes_pull": st.bytes_pull, "bytes_push": st.bytes_push,
+    }
+
+
+class VExpandResult:
+    """VarRowMatrixTable (one merged matrix) or MultipleMatrixTable (one per hop)."""
+
+    def __init__(self, handle: C.c_void_p, sources: np.ndarray):
+        self._h = handle
+        self.row_map = sources  # the reference sets row_map = start vertices
+        rows, cols, n, k = (C.c_uint32() for _ in range(4))
+        _lib.call("vs_result_info", handle, C.byref(rows), C.byref(cols), C.byref(n), C.byref(k))
+        self.rows, self.cols = rows.value, cols.value
+        self.num_matrices, self.k_actual_max = n.value, k.value
+
+    def distance(self, i: int = 0):
+        d = C.c_uint32()
+        _lib.call("vs_result_distance", self._h, i, C.byref(d))
+        return None if d.value == 0xFFFFFFFF else d.value
+
+    def tuple_count(self, i: int = 0) -> int:
+        c = C.c_uint64()
+        _lib.call("vs_result_count", self._h, i, C.byref(c))
+        return c.value
+
+    def row_counts(self, i: int = 0) -> np.ndarray:
+        out = np.zeros(self.rows, dtype=np.uint64)
+        if self.rows:
+            _lib.call("vs_result_row_counts", self._h, i, _ptr(out, C.c_uint64))
+        return out
+
+    def row_digests(self, i: int = 0) -> np.ndarray:
+        out = np.zeros(self.rows, dtype=np.uint64)
+        if self.rows:
+            _lib.call("vs_result_row_digests", self._h, i, _ptr(out, C.c_uint64))
+        return out
+
+    def export(self, i: int = 0, layout: int = LAYOUT_ROWMAJOR) -> np.ndarray:
+        n = C.c_size_t()
+        _lib.call("vs_result_export_size", self._h, i, layout, C.byref(n))
+        out = np.zeros(n.value // 8, dtype=np.uint64)
+        if n.value:
+            _lib.call("vs_result_export", self._h, i, layout, out.ctypes.data_as(C.c_void_p),
+                      n.value)
+        if layout == LAYOUT_ROWMAJOR:
+            return out.reshape(self.rows, (self.cols + 63) // 64)
+        return out
+
+    def get(self, row: int, col: int, i: int = 0) -> bool:
+        b = C.c_int()
+        _lib.call("vs_result_get", self._h, i, row, col, C.byref(b))
+        return bool(b.value)
+
+    def close(self):
+        if self._h:
+            _lib.call("vs_result_free", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def vexpand(graph: Graph, sources, k_min: int, k_max: int, path: str = "shortest",
+            direction: str = "forward", with_distance: bool = False,
+            sched: str = "heuristic", tile_words: int = 0, timing: bool = False):
+    """Multi-source VExpand; returns (VExpandResult, stats dict)."""
+    src = _u32(sources)
+    p = _lib.VexpandParams(k_min, k_max, _PATHS[path], _DIRS[direction], int(with_distance),
+                           _SCHED[sched], tile_words, _lib.VX_TIMING if timing else 0)
+    st = _lib.VexpandStats()
+    h = C.c_void_p()
+    _lib.call("vs_vexpand", graph.handle, _ptr(src, C.c_uint32), len(src), C.byref(p),
+              C.byref(h), C.byref(st))
+    return VExpandResult(h, src), stats_dict(st)
+
+
+def find_shortest(graph: Graph, sources, targets, k_max: int = 50,
+                  direction: str = "forward") -> np.ndarray:
+    src, tgt = _u32(sources), _u32(targets)
+    if len(src) != len(tgt):
+        raise ValueError("sources and targets differ in length")
+    dist = np.zeros(len(src), dtype=np.int32)
+    st = _lib.VexpandStats()
+    _lib.call("vs_find_shortest", graph.handle, _ptr(src, C.c_uint32), _ptr(tgt, C.c_uint32),
+              len(src), _DIRS[direction], k_max, _ptr(dist, C.c_int32), C.byref(st))
+    return dist
+
+
+def mintersect(deps: list, tuples, with_lists: bool = True):
+    """Multi-way intersection; returns (counts, offsets, lists, stats)."""
+    tup = np.ascontiguousarray(tuples, dtype=np.uint32)
+    if tup.ndim == 1:
+        tup = tup.reshape(-1, len(deps))
+    T, arity = tup.shape
+    if arity != len(deps):
+        raise ValueError("tuple arity differs from the number of dependencies")
+    arr = (C.c_void_p * arity)(*[d.handle for d in deps])
+    counts = np.zeros(T, dtype=np.uint32)
+    offsets = np.zeros(T + 1, dtype=np.uint64)
+    total = C.c_uint64()
+    st = _lib.MintersectStats()
+    _lib.call("vs_mintersect", arr, arity, _ptr(tup, C.c_uint32), T, _ptr(counts, C.c_uint32),
+              _ptr(offsets, C.c_uint64), None, 0, C.byref(total), C.byref(st))
+    lists = None
+    if with_lists:
+        lists = np.zeros(max(total.value, 1), dtype=np.uint32)
+        _lib.call("vs_mintersect", arr, arity, _ptr(tup, C.c_uint32), T,
+                  _ptr(counts, C.c_uint32), _ptr(offsets, C.c_uint64),
+                  _ptr(lists, C.c_uint32), len(lists), C.byref(total), C.byref(st))
+        lists = lists[:total.value]
+    return counts, offsets, lists, {"ms": st.ms, "total": st.total, "bytes_alg": st.bytes_alg}
+
+
+class Scheduler:
+    """The learned push/pull scheduler (FP32 MLP with HIP forward/backward)."""
+
+    def __init__(self, seed: int = 1):
+        self._h = C.c_void_p()
+        _lib.call("vs_sched_create", seed, C.byref(self._h))
+        n = C.c_size_t()
+        _lib.call("vs_sched_num_params", C.byref(n))
+        self.num_params = n.value
+
+    @property
+    def handle(self):
+        return self._h
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.num_params, dtype=np.float32)
+        _lib.call("vs_sched_get_params", self._h, _ptr(p, C.c_float), len(p))
+        return p
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _lib.call("vs_sched_set_params", self._h, _ptr(p, C.c_float), len(p))
+
+    def loss_grad(self, X, Y, M):
+        X, Y, M = (np.ascontiguousarray(a, dtype=np.float32) for a in (X, Y, M))
+        loss = C.c_float()
+        grad = np.zeros(self.num_params, dtype=np.float32)
+        _lib.call("vs_sched_loss_grad", self._h, _ptr(X, C.c_float), _ptr(Y, C.c_float),
+                  _ptr(M, C.c_float), len(X), C.byref(loss), _ptr(grad, C.c_float))
+        return loss.value, grad
+
+    def train(self, X, Y, M, epochs: int, lr: float = 1e-3) -> np.ndarray:
+        X, Y, M = (np.ascontiguousarray(a, dtype=np.float32) for a in (X, Y, M))
+        curve = np.zeros(epochs, dtype=np.float32)
+        _lib.call("vs_sched_train", self._h, _ptr(X, C.c_float), _ptr(Y, C.c_float),
+                  _ptr(M, C.c_float), len(X), epochs, lr, _ptr(curve, C.c_float))
+        return curve
+
+    def predict(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((len(X), _lib.MLP_OUT), dtype=np.float32)
+        _lib.call("vs_sched_predict", self._h, _ptr(X, C.c_float), len(X), _ptr(Y, C.c_float))
+        return Y
+
+    def close(self):
+        if self._h:
+            _lib.call("vs_sched_free", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class BatchQuery:
+    sources: np.ndarray
+    k_min: int
+    k_max: int
+    path: str = "shortest"
+
+
+def vexpand_batch(graph: Graph, queries: list, tile_words: int = 0, sched: str = "heuristic",
+                  use_graph: bool = True, replays: int = 1, mem_budget: int = 0,
+                  direction: str = "forward", want_digests: bool = True):
+    """Runs many VExpand queries packed into shared tiles.  Returns
+    (row_counts, row_digests, stats) concatenated over queries."""
+    keep = [_u32(q.sources) for q in queries]
+    arr = (_lib.Query * len(queries))()
+    for i, q in enumerate(queries):
+        arr[i].sources = _ptr(keep[i], C.c_uint32)
+        arr[i].S = len(keep[i])
+        arr[i].k_min = q.k_min
+        arr[i].k_max = q.k_max
+        arr[i].path_type = _PATHS[q.path]
+    total = sum(len(k) for k in keep)
+    counts = np.zeros(total, dtype=np.uint64)
+    digests = np.zeros(total, dtype=np.uint64) if want_digests else None
+    p = _lib.BatchParams(tile_words, _SCHED[sched], int(use_graph), replays, mem_budget,
+                         _DIRS[direction], 0)
+    st = _lib.BatchStats()
+    _lib.call("vs_vexpand_batch", graph.handle, arr, len(queries), C.byref(p),
+              _ptr(counts, C.c_uint64),
+              _ptr(digests, C.c_uint64) if digests is not None else None, C.byref(st))
+    return counts, digests, {
+        "te": st.te, "ms_total": st.ms_total, "ms_capture": st.ms_capture,
+        "num_tiles": st.num_tiles, "num_waves": st.num_waves, "launches": st.launches,
+        "push_decisions": st.push_decisions, "pull_decisions": st.pull_decisions}

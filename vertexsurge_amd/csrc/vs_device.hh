@@ -1,0 +1,207 @@
+// Device-side internal declarations (hipcc only): graph handle, buffer pool,
+// result handle, and wave-level helpers for gfx950 (wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "vs_internal.hh"
+
+#define VS_HIP(call)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      vs::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,                \
+                    hipGetErrorString(e_));                                   \
+      return VS_ERR_HIP;                                                      \
+    }                                                                         \
+  } while (0)
+
+#define VS_TRY(call)                                                          \
+  do {                                                                        \
+    vs_status s_ = (call);                                                    \
+    if (s_ != VS_OK) return s_;                                               \
+  } while (0)
+
+typedef unsigned long long ull;
+
+namespace vs {
+
+// Work item of the push kernel: vertex id in the low 32 bits, chunk of
+// kPushChunk neighbours in the high 32 bits.  Hubs become many items so one
+// wave never owns a whole hub list.
+constexpr uint32_t kPushChunk = 256;
+constexpr int kWavesPerBlock = 4;
+constexpr int kBlock = 64 * kWavesPerBlock;
+
+// Device buffers of exactly-matching size are recycled per graph.  A buffer
+// comes back "clean" (all zero) or "dirty"; get(zero=true) memsets only dirty
+// ones.
+class Pool {
+ public:
+  vs_status get(size_t bytes, bool zero, hipStream_t s, void **out);
+  void put(void *p, size_t bytes, bool clean);
+  void trim();
+  ~Pool() { trim(); }
+
+ private:
+  std::multimap<size_t, std::pair<void *, bool>> free_;
+};
+
+// One CSR resident in HBM.
+struct DevCSR {
+  uint64_t *off = nullptr;
+  uint32_t *adj = nullptr;
+  uint64_t E = 0;
+  std::vector<uint64_t> hoff; // host copy of offsets (degrees for host stats)
+  bool owned = false;
+};
+
+} // namespace vs
+
+struct vs_sched_s;
+
+struct vs_graph_s {
+  int device = 0;
+  int num_cus = 256;
+  uint32_t V = 0;
+  uint64_t E = 0;
+  bool symmetric = false;
+  vs::DevCSR fwd, rev, both;      // rev/both built on demand for directed graphs
+  std::vector<uint32_t> hadj;     // host copy of adj (directed graphs only)
+  hipStream_t stream = nullptr;
+  vs::Pool pool;
+  vs_tuning tuning{0, 16, 0, 0};
+  vs_sched_s *sched = nullptr;
+  int refs = 1;                   // graph + live results
+  bool destroyed = false;
+};
+
+namespace vs {
+
+// (push CSR, pull CSR) for a direction: push follows edges forward from a
+// frontier vertex, pull gathers over the edges that arrive at a vertex.
+vs_status graph_csrs(vs_graph_s *g, int direction, const DevCSR **push,
+                     const DevCSR **pull);
+void graph_release(vs_graph_s *g);
+
+struct Mat {
+  uint64_t *d = nullptr;
+  size_t bytes = 0;
+  uint32_t distance = UINT32_MAX;
+};
+
+} // namespace vs
+
+struct vs_result_s {
+  vs_graph_s *g = nullptr;
+  uint32_t S = 0, V = 0, tw = 1, ntiles = 0, k_actual_max = 0;
+  std::vector<vs::Mat> mats;
+};
+
+namespace vs {
+
+// ----------------------------------------------------------- device helpers
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+__device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
+  // number of set bits of m below this lane
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
+
+// Orders LDS traffic between the lanes of one wave (no workgroup barrier).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t x, int m) {
+  const uint32_t lo = __shfl_xor(static_cast<uint32_t>(x), m);
+  const uint32_t hi = __shfl_xor(static_cast<uint32_t>(x >> 32), m);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t x, int src) {
+  const uint32_t lo = __shfl(static_cast<uint32_t>(x), src);
+  const uint32_t hi = __shfl(static_cast<uint32_t>(x >> 32), src);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// OR over the 64/TW lane groups that share word index lane % TW.
+template <int TW>
+__device__ __forceinline__ uint64_t group_or(uint64_t x) {
+#pragma unroll
+  for (int m = TW; m < 64; m <<= 1) x |= shfl_xor64(x, m);
+  return x;
+}
+
+__device__ __forceinline__ uint64_t wave_sum(uint64_t x) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) x += shfl_xor64(x, m);
+  return x;
+}
+
+__device__ __forceinline__ uint64_t wave_or(uint64_t x) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) x |= shfl_xor64(x, m);
+  return x;
+}
+
+// Bits of global word w that belong to real source rows (< S).
+__device__ __forceinline__ uint64_t valid_mask(uint32_t w, uint32_t S) {
+  const uint64_t r0 = static_cast<uint64_t>(w) * 64;
+  if (r0 >= S) return 0;
+  const uint64_t n = S - r0;
+  return n >= 64 ? ~0ull : ((1ull << n) - 1);
+}
+
+template <int TW>
+__device__ __forceinline__ uint64_t group_bits(uint64_t bal, int g) {
+  if constexpr (TW == 64) return bal;
+  else return (bal >> (g * TW)) & ((1ull << TW) - 1);
+}
+
+// index of the g-th (0-based) lowest set bit of m, or -1
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int g) {
+  for (int q = 0; q < g && m; ++q) m &= m - 1;
+  return m ? __ffsll(static_cast<long long>(m)) - 1 : -1;
+}
+
+__device__ __forceinline__ uint64_t drop_low_bits(uint64_t m, int n) {
+  for (int q = 0; q < n && m; ++q) m &= m - 1;
+  return m;
+}
+
+__host__ __device__ __forceinline__ uint64_t digest_hash(uint32_t c) {
+  uint64_t z = 0x5EEDC0DEull + (static_cast<uint64_t>(c) + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+inline uint32_t pow2ceil(uint32_t x) {
+  uint32_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+inline uint32_t grid_for(uint64_t waves, int num_cus, int blocks_per_cu = 8) {
+  uint64_t blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+  const uint64_t cap = static_cast<uint64_t>(num_cus) * blocks_per_cu;
+  if (blocks > cap) blocks = cap;
+  if (blocks == 0) blocks = 1;
+  return static_cast<uint32_t>(blocks);
+}
+
+// Result-side kernels (vs_graph.hip).
+vs_status mat_row_counts(vs_graph_s *g, const Mat &m, uint32_t S, uint32_t tw,
+                         uint32_t ntiles, uint64_t *d_counts);
+vs_status mat_row_digests(vs_graph_s *g, const Mat &m, uint32_t S, uint32_t tw,
+                          uint32_t ntiles, uint64_t *d_digests);
+
+} // namespace vs

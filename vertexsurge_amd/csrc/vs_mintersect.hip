@@ -1,0 +1,377 @@
+// Multi-way intersection over sorted adjacency lists.
+//
+// The reference (Mintersect::run_internal, src/objects/operator/
+// mintersect.cpp:48-345) builds, per base tuple, an all-ones bitmap of V bits,
+// ANDs one DenseColMainMatrix column (or a scanned flat dependency) per
+// dependency, then pops the set bits in ascending row order
+// (BitmapIterator, src/include/storage/bitmap.hh:91-126).  Cost per tuple is
+// O(V/64) words per dependency whatever the degrees.
+//
+// Here each dependency is a CSR in HBM with sorted lists, so the same set is
+// the intersection of arity sorted lists.  One wave owns one tuple: the
+// shortest list drives, 64 elements per step (one per lane); each lane looks
+// its element up in every other list by binary search inside the sub-range
+// that the step's first and last elements bound; ballot + mbcnt give the
+// ascending output positions.  Two passes: count -> exclusive scan -> fill.
+#include <algorithm>
+#include <cstring>
+
+#include <rocprim/device/device_scan.hpp>
+
+#include "vs_device.hh"
+
+namespace vs {
+
+constexpr int kMaxArity = 8;
+
+struct MiArgs {
+  uint32_t arity;
+  uint32_t pad;
+  uint64_t T;
+  const uint32_t *tuples;
+  const uint64_t *off[kMaxArity];
+  const uint32_t *adj[kMaxArity];
+  uint32_t V[kMaxArity];
+  uint32_t *counts;
+  const uint64_t *offsets;
+  uint32_t *lists;
+};
+
+// first index in [lo, hi) with a[idx] > x
+__device__ __forceinline__ uint64_t upper_bound_g(const uint32_t *__restrict__ a,
+                                                  uint64_t lo, uint64_t hi, uint32_t x) {
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] <= x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// first index in [lo, hi) with a[idx] >= x
+__device__ __forceinline__ uint64_t lower_bound_g(const uint32_t *__restrict__ a,
+                                                  uint64_t lo, uint64_t hi, uint32_t x) {
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <bool FILL>
+__global__ __launch_bounds__(kBlock) void k_mintersect(MiArgs a) {
+  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
+  for (uint64_t t = blockIdx.x * kWavesPerBlock + wid; t < a.T; t += nwaves) {
+    uint64_t b[kMaxArity], e[kMaxArity];
+    bool empty = false;
+    uint32_t drv = 0;
+    uint64_t best = ~0ull;
+    for (uint32_t d = 0; d < a.arity; ++d) {
+      const uint32_t x = a.tuples[t * a.arity + d];
+      if (x >= a.V[d]) {
+        empty = true;
+        b[d] = e[d] = 0;
+        continue;
+      }
+      b[d] = a.off[d][x];
+      e[d] = a.off[d][x + 1];
+      if (e[d] - b[d] < best) {
+        best = e[d] - b[d];
+        drv = d;
+      }
+    }
+    uint32_t total = 0;
+    uint64_t out = FILL ? a.offsets[t] : 0;
+    if (!empty && best > 0) {
+      const uint32_t *dl = a.adj[drv];
+      for (uint64_t base = b[drv]; base < e[drv]; base += 64) {
+        const uint64_t idx = base + lane;
+        const bool have = idx < e[drv];
+        const uint32_t x = have ? dl[idx] : 0u;
+        const uint64_t last = min<uint64_t>(e[drv], base + 64) - 1;
+        const uint32_t xfirst = dl[base], xlast = dl[last];
+        bool keep = have;
+        for (uint32_t d = 0; d < a.arity && __ballot(keep); ++d) {
+          if (d == drv) continue;
+          // the step's elements lie in [xfirst, xlast]: narrow the list once
+          const uint32_t *ol = a.adj[d];
+          uint64_t lo = b[d], hi = e[d];
+          if (lane == 0) lo = lower_bound_g(ol, b[d], e[d], xfirst);
+          if (lane == 1) hi = upper_bound_g(ol, b[d], e[d], xlast);
+          lo = shfl64(lo, 0);
+          hi = shfl64(hi, 1);
+          if (keep) {
+            const uint64_t p = lower_bound_g(ol, lo, hi, x);
+            keep = p < hi && ol[p] == x;
+          }
+        }
+        const uint64_t m = __ballot(keep);
+        if (FILL && keep) a.lists[out + lane_rank(m)] = x;
+        out += __popcll(m);
+        total += __popcll(m);
+      }
+    }
+    if (!FILL && lane == 0) a.counts[t] = total;
+  }
+}
+
+__global__ void k_last_offset(const uint32_t *counts, uint64_t *offsets, uint64_t T) {
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    offsets[T] = T ? offsets[T - 1] + counts[T - 1] : 0;
+}
+
+// bytes of the distinct adjacency lists touched (4 * deg + 16 each)
+__global__ void k_mark(const uint32_t *tuples, uint64_t n, uint32_t arity,
+                       uint32_t V0, uint64_t *bm) {
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t x = tuples[i];
+    if (x < V0) atomicOr(reinterpret_cast<ull *>(&bm[x >> 6]), 1ull << (x & 63));
+  }
+}
+
+__global__ void k_marked_bytes(const uint64_t *bm, uint32_t V, const uint64_t *off,
+                               ull *bytes) {
+  uint64_t acc = 0;
+  const uint32_t nvb = (V + 63) / 64;
+  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nvb;
+       w += gridDim.x * blockDim.x) {
+    for (uint64_t m = bm[w]; m; m &= m - 1) {
+      const uint32_t v = w * 64 + (__ffsll(static_cast<long long>(m)) - 1);
+      acc += 4 * (off[v + 1] - off[v]) + 16;
+    }
+  }
+  acc = wave_sum(acc);
+  if (lane_id() == 0 && acc) atomicAdd(bytes, static_cast<ull>(acc));
+}
+
+static vs_status mintersect_impl(const vs_graph_t *deps, uint32_t arity,
+                                 const uint32_t *d_tuples, uint64_t T,
+                                 uint32_t *d_counts, uint64_t *d_offsets,
+                                 uint32_t *d_lists, uint64_t list_cap,
+                                 uint64_t *total, vs_mintersect_stats *st) {
+  if (!deps || arity == 0 || arity > kMaxArity) {
+    set_error("vs_mintersect: arity %u not in 1..%d", arity, kMaxArity);
+    return VS_ERR_INVALID;
+  }
+  for (uint32_t d = 0; d < arity; ++d) {
+    if (!deps[d]) { set_error("vs_mintersect: null dependency %u", d); return VS_ERR_INVALID; }
+    if (deps[d]->device != deps[0]->device) {
+      set_error("vs_mintersect: dependencies on different devices");
+      return VS_ERR_INVALID;
+    }
+  }
+  if (st) std::memset(st, 0, sizeof(*st));
+  if (total) *total = 0;
+  if (T == 0) {
+    if (d_offsets) {
+      const uint64_t z = 0;
+      VS_HIP(hipMemcpy(d_offsets, &z, 8, hipMemcpyHostToDevice));
+    }
+    return VS_OK;
+  }
+  if (!d_tuples || !d_counts) { set_error("vs_mintersect: null tuples/counts"); return VS_ERR_INVALID; }
+  vs_graph_s *g = deps[0];
+  VS_HIP(hipSetDevice(g->device));
+  hipStream_t s = g->stream;
+  MiArgs a{};
+  a.arity = arity;
+  a.T = T;
+  a.tuples = d_tuples;
+  for (uint32_t d = 0; d < arity; ++d) {
+    a.off[d] = deps[d]->fwd.off;
+    a.adj[d] = deps[d]->fwd.adj;
+    a.V[d] = deps[d]->V;
+  }
+  a.counts = d_counts;
+  hipEvent_t e0, e1;
+  VS_HIP(hipEventCreate(&e0));
+  VS_HIP(hipEventCreate(&e1));
+  struct EvGuard { hipEvent_t a, b; ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); } } eg{e0, e1};
+  const uint32_t grid = grid_for(T, g->num_cus);
+  VS_HIP(hipEventRecord(e0, s));
+  hipLaunchKernelGGL(k_mintersect<false>, dim3(grid), dim3(kBlock), 0, s, a);
+  VS_HIP(hipGetLastError());
+  uint64_t tot = 0;
+  if (d_offsets) {
+    size_t tmp_bytes = 0;
+    VS_HIP(rocprim::exclusive_scan(nullptr, tmp_bytes, d_counts, d_offsets,
+                                   static_cast<uint64_t>(0), static_cast<size_t>(T),
+                                   rocprim::plus<uint64_t>(), s));
+    void *tmp = nullptr;
+    VS_TRY(g->pool.get(tmp_bytes, false, s, &tmp));
+    hipError_t e = rocprim::exclusive_scan(tmp, tmp_bytes, d_counts, d_offsets,
+                                           static_cast<uint64_t>(0), static_cast<size_t>(T),
+                                           rocprim::plus<uint64_t>(), s);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_last_offset, dim3(1), dim3(64), 0, s, d_counts, d_offsets, T);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&tot, d_offsets + T, 8, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    g->pool.put(tmp, tmp_bytes, false);
+    if (e != hipSuccess) {
+      set_error("vs_mintersect scan: %s", hipGetErrorString(e));
+      return VS_ERR_HIP;
+    }
+    if (total) *total = tot;
+    if (d_lists) {
+      if (tot > list_cap) {
+        set_error("vs_mintersect: list capacity %llu < total %llu",
+                  (unsigned long long)list_cap, (unsigned long long)tot);
+        return VS_ERR_CAPACITY;
+      }
+      a.offsets = d_offsets;
+      a.lists = d_lists;
+      hipLaunchKernelGGL(k_mintersect<true>, dim3(grid), dim3(kBlock), 0, s, a);
+      VS_HIP(hipGetLastError());
+    }
+  }
+  VS_HIP(hipEventRecord(e1, s));
+  VS_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  VS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  if (!d_offsets) {
+    // count-only: total from a host reduction of the counts
+    std::vector<uint32_t> c(T);
+    VS_HIP(hipMemcpy(c.data(), d_counts, T * 4, hipMemcpyDeviceToHost));
+    for (uint32_t x : c) tot += x;
+    if (total) *total = tot;
+  }
+  if (st) {
+    st->ms = ms;
+    st->total = tot;
+    // compulsory bytes: distinct lists touched + tuples + counts + offsets + lists
+    uint64_t b = 4ull * arity * T + 4 * T + (d_offsets ? 8 * (T + 1) : 0) +
+                 (d_lists ? 4 * tot : 0);
+    for (uint32_t d = 0; d < arity; ++d) {
+      bool seen = false;
+      for (uint32_t q = 0; q < d; ++q) seen |= deps[q] == deps[d];
+      if (seen) continue;
+      // one bitmap per distinct dependency graph over all columns bound to it
+      void *bm = nullptr, *cnt = nullptr;
+      const size_t bm_bytes = static_cast<size_t>((deps[d]->V + 63) / 64) * 8;
+      VS_TRY(g->pool.get(bm_bytes, true, s, &bm));
+      VS_TRY(g->pool.get(8, true, s, &cnt));
+      // mark all columns bound to this dependency
+      std::vector<uint32_t> cols;
+      for (uint32_t q = 0; q < arity; ++q) if (deps[q] == deps[d]) cols.push_back(q);
+      if (cols.size() == arity) {
+        hipLaunchKernelGGL(k_mark, dim3(1024), dim3(256), 0, s, d_tuples, T * arity, arity,
+                           deps[d]->V, static_cast<uint64_t *>(bm));
+      } else {
+        std::vector<uint32_t> h(T * arity), sel;
+        VS_HIP(hipMemcpy(h.data(), d_tuples, T * arity * 4, hipMemcpyDeviceToHost));
+        sel.reserve(T * cols.size());
+        for (uint64_t t = 0; t < T; ++t) for (uint32_t q : cols) sel.push_back(h[t * arity + q]);
+        void *dsel = nullptr;
+        VS_TRY(g->pool.get(sel.size() * 4, false, s, &dsel));
+        VS_HIP(hipMemcpyAsync(dsel, sel.data(), sel.size() * 4, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_mark, dim3(1024), dim3(256), 0, s,
+                           static_cast<const uint32_t *>(dsel), static_cast<uint64_t>(sel.size()),
+                           1u, deps[d]->V, static_cast<uint64_t *>(bm));
+        VS_HIP(hipStreamSynchronize(s));
+        g->pool.put(dsel, sel.size() * 4, false);
+      }
+      hipLaunchKernelGGL(k_marked_bytes, dim3(1024), dim3(256), 0, s,
+                         static_cast<const uint64_t *>(bm), deps[d]->V, deps[d]->fwd.off,
+                         static_cast<ull *>(cnt));
+      VS_HIP(hipGetLastError());
+      uint64_t hb = 0;
+      VS_HIP(hipMemcpyAsync(&hb, cnt, 8, hipMemcpyDeviceToHost, s));
+      VS_HIP(hipStreamSynchronize(s));
+      g->pool.put(bm, bm_bytes, false);
+      g->pool.put(cnt, 8, false);
+      b += hb;
+    }
+    st->bytes_alg = b;
+  }
+  return VS_OK;
+}
+
+} // namespace vs
+
+using namespace vs;
+
+extern "C" {
+
+vs_status vs_mintersect_device(const vs_graph_t *deps, uint32_t arity,
+                               const uint32_t *d_tuples, uint64_t T,
+                               uint32_t *d_counts, uint64_t *d_offsets,
+                               uint32_t *d_lists, uint64_t list_cap,
+                               uint64_t *total, vs_mintersect_stats *stats) {
+  clear_error();
+  return mintersect_impl(deps, arity, d_tuples, T, d_counts, d_offsets, d_lists,
+                         list_cap, total, stats);
+}
+
+vs_status vs_mintersect(const vs_graph_t *deps, uint32_t arity,
+                        const uint32_t *tuples, uint64_t T, uint32_t *counts,
+                        uint64_t *offsets, uint32_t *lists, uint64_t list_cap,
+                        uint64_t *total, vs_mintersect_stats *stats) {
+  clear_error();
+  if (!deps || arity == 0 || arity > kMaxArity || !deps[0]) {
+    set_error("vs_mintersect: bad dependencies / arity %u", arity);
+    return VS_ERR_INVALID;
+  }
+  if (T > 0 && (!tuples || !counts)) {
+    set_error("vs_mintersect: null tuples/counts");
+    return VS_ERR_INVALID;
+  }
+  if (lists && !offsets) {
+    set_error("vs_mintersect: lists need offsets");
+    return VS_ERR_INVALID;
+  }
+  vs_graph_s *g = deps[0];
+  VS_HIP(hipSetDevice(g->device));
+  struct Bufs {
+    vs_graph_s *g;
+    void *t = nullptr, *c = nullptr, *o = nullptr, *l = nullptr;
+    size_t tb = 0, cb = 0, ob = 0, lb = 0;
+    ~Bufs() {
+      g->pool.put(t, tb, false);
+      g->pool.put(c, cb, false);
+      g->pool.put(o, ob, false);
+      g->pool.put(l, lb, false);
+    }
+  } b{g};
+  if (T == 0) {
+    if (total) *total = 0;
+    if (offsets) offsets[0] = 0;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    return VS_OK;
+  }
+  b.tb = T * arity * 4;
+  b.cb = T * 4;
+  b.ob = offsets ? (T + 1) * 8 : 0;
+  VS_TRY(g->pool.get(b.tb, false, g->stream, &b.t));
+  VS_TRY(g->pool.get(b.cb, false, g->stream, &b.c));
+  if (b.ob) VS_TRY(g->pool.get(b.ob, false, g->stream, &b.o));
+  VS_HIP(hipMemcpyAsync(b.t, tuples, b.tb, hipMemcpyHostToDevice, g->stream));
+  uint64_t tot = 0;
+  // pass 1: counts + offsets (lists == NULL)
+  VS_TRY(mintersect_impl(deps, arity, static_cast<uint32_t *>(b.t), T,
+                         static_cast<uint32_t *>(b.c), static_cast<uint64_t *>(b.o),
+                         nullptr, 0, &tot, lists ? nullptr : stats));
+  if (total) *total = tot;
+  VS_HIP(hipMemcpy(counts, b.c, b.cb, hipMemcpyDeviceToHost));
+  if (offsets) VS_HIP(hipMemcpy(offsets, b.o, b.ob, hipMemcpyDeviceToHost));
+  if (lists) {
+    if (tot > list_cap) {
+      set_error("vs_mintersect: list capacity %llu < total %llu",
+                (unsigned long long)list_cap, (unsigned long long)tot);
+      return VS_ERR_CAPACITY;
+    }
+    b.lb = std::max<uint64_t>(tot, 1) * 4;
+    VS_TRY(g->pool.get(b.lb, false, g->stream, &b.l));
+    VS_TRY(mintersect_impl(deps, arity, static_cast<uint32_t *>(b.t), T,
+                           static_cast<uint32_t *>(b.c), static_cast<uint64_t *>(b.o),
+                           static_cast<uint32_t *>(b.l), tot, &tot, stats));
+    if (tot) VS_HIP(hipMemcpy(lists, b.l, tot * 4, hipMemcpyDeviceToHost));
+  }
+  return VS_OK;
+}
+
+} // extern "C"
