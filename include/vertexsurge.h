@@ -195,6 +195,10 @@ typedef struct {
   uint64_t n_push;
   uint64_t bytes_pull; /* algorithmic bytes of the hops run as pull */
   uint64_t bytes_push;
+  /* scheduling features of hop i's frontier F_i (index i-1), summed over tiles */
+  uint64_t frontier_act[VS_MAX_HOPS]; /* vertices with a nonzero frontier row */
+  uint64_t frontier_deg[VS_MAX_HOPS]; /* their out-degree sum */
+  double vis_frac[VS_MAX_HOPS];       /* nonzero visited words / all words */
 } vs_vexpand_stats;
 
 typedef struct vs_result_s *vs_result_t;

@@ -1,0 +1,110 @@
+// Drives the C++ operator shims (vertexsurge_amd/host/vs_operators.hh) the way
+// a reference plan would, and prints their outputs as JSON for
+// tests/test_cpp_operators.py to compare with the CPU oracle.
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../vertexsurge_amd/host/vs_operators.hh"
+
+using namespace vertexsurge;
+
+static void print_vec(const char *name, const std::vector<uint64_t> &v, bool last = false) {
+  std::printf("\"%s\": [", name);
+  for (size_t i = 0; i < v.size(); ++i) std::printf("%s%llu", i ? "," : "", (unsigned long long)v[i]);
+  std::printf("]%s\n", last ? "" : ",");
+}
+
+int main() {
+  vs_csr_t csr = nullptr;
+  check(vs_rmat_csr(12, 16, 0.57, 0.19, 0.19, 1, &csr), "vs_rmat_csr");
+  uint32_t V = 0;
+  uint64_t E = 0;
+  const uint64_t *off = nullptr;
+  const uint32_t *adj = nullptr;
+  check(vs_csr_info(csr, &V, &E, &off, &adj), "vs_csr_info");
+  auto rel = std::make_shared<EdgeRelation>(off, adj, V, E, true);
+  GraphStore::instance().add_edge("Person", "knows", "Person", rel);
+  std::vector<VID> starts(200);
+  check(vs_sample_sources(V, 200, 7, starts.data()), "vs_sample_sources");
+
+  std::printf("{\n");
+  // VExpand(Person, Person, knows, Both, 1, 3, Shortest, Dense, "p1.id", {})
+  VExpand vx("Person", "Person", "knows", Direction::Both, 1, 3, ExpandPathType::Shortest,
+             ExpandStrategy::Dense, "p1.id", std::nullopt);
+  vx.run(starts);
+  auto tab = vx.results();
+  std::printf("\"shortest_k_actual\": %zu,\n", vx.k_actual_max());
+  std::printf("\"shortest_tuple_count\": %zu,\n", tab.tuple_count());
+  print_vec("shortest_row_counts", tab.row_counts());
+  std::printf("\"shortest_get_0_first\": %d,\n",
+              (int)tab.get(0, tab.tuples().empty() ? 0 : tab.tuples()[0].second));
+
+  VExpand va("Person", "Person", "knows", Direction::Forward, 0, 2,
+             VExpandPathParameters::Any(), ExpandStrategy::Dense, "p1.id", std::nullopt);
+  va.run(starts);
+  print_vec("any_row_counts", va.results().row_counts());
+
+  VExpand vd("Person", "Person", "knows", Direction::Forward, 1, 3,
+             VExpandPathParameters::Shortest().set_with_distance(), ExpandStrategy::Dense,
+             "p1.id", std::nullopt);
+  vd.run(starts);
+  auto multi = vd.multiple_results();
+  std::vector<uint64_t> dists, per_layer;
+  for (auto &m : multi.mats) {
+    dists.push_back(m.first);
+    per_layer.push_back(m.second.tuple_count());
+  }
+  print_vec("layer_distances", dists);
+  print_vec("layer_tuple_counts", per_layer);
+
+  std::vector<VID> targets(starts.rbegin(), starts.rend());
+  VExpand vf("Person", "Person", "knows", Direction::Forward,
+             VExpandPathParameters::FindShortest(), ExpandStrategy::Dense, "p1.id", "p2.id");
+  vf.run(starts, &targets);
+  std::vector<uint64_t> fd;
+  for (int32_t d : vf.distance()) fd.push_back(static_cast<uint64_t>(static_cast<int64_t>(d) + 1));
+  print_vec("find_shortest_dist_plus1", fd);
+
+  // empty start column: empty result, no error
+  VExpand ve("Person", "Person", "knows", Direction::Forward, 1, 2, ExpandPathType::Shortest,
+             ExpandStrategy::Dense, "p1.id", std::nullopt);
+  ve.run({});
+  std::printf("\"empty_tuple_count\": %zu,\n", ve.results().tuple_count());
+
+  // unknown edge schema -> std::runtime_error
+  bool threw = false;
+  try {
+    VExpand bad("Person", "Person", "likes", Direction::Forward, 1, 2,
+                ExpandPathType::Shortest, ExpandStrategy::Dense, "p1.id", std::nullopt);
+  } catch (const std::runtime_error &) {
+    threw = true;
+  }
+  std::printf("\"unknown_schema_throws\": %d,\n", threw ? 1 : 0);
+
+  // Mintersect KAT (mintersect.cpp:347-409): 3 result tuples
+  {
+    // vertices 1..8 have one neighbour each (1,1,1,2,2,3,3,4); 0 and 9 none
+    std::vector<uint64_t> koff(11, 0);
+    std::vector<uint32_t> kadj = {1, 1, 1, 2, 2, 3, 3, 4};
+    for (int k = 2; k <= 10; ++k) koff[k] = k - 1 > 8 ? 8 : k - 1;
+    auto dep = std::make_shared<EdgeRelation>(koff.data(), kadj.data(), 10, 8, false);
+    Mintersect mi("c", "Person", false);
+    mi.add_dependency(dep);
+    mi.add_dependency(dep);
+    std::vector<VID> base = {1, 1, 2, 1, 3, 1, 4, 2, 5, 2, 6, 3, 7, 3, 8, 4};
+    mi.run(base);
+    std::vector<uint64_t> flat(mi.flat_tuples().begin(), mi.flat_tuples().end());
+    std::printf("\"mintersect_kat_tuples\": %zu,\n", mi.tuple_count());
+    print_vec("mintersect_kat_flat", flat);
+    Mintersect mc("c", "Person", true);
+    mc.add_dependency(dep);
+    mc.add_dependency(dep);
+    mc.run(base);
+    std::printf("\"mintersect_kat_count\": %llu\n", (unsigned long long)mc.count());
+  }
+  std::printf("}\n");
+  GraphStore::instance().clear();
+  vs_csr_free(csr);
+  return 0;
+}

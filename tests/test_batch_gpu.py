@@ -1,0 +1,100 @@
+"""Batched VExpand (shared tiles, persistent kernel, replayed hipGraph) must give
+the same per-row counts and digests as running every query alone, and as the
+CPU oracle."""
+import numpy as np
+import pytest
+
+import vertexsurge_amd as vs
+
+pytestmark = pytest.mark.gpu
+
+
+def make_queries(V, n, seed, smax=700, kmax=(1, 5), kmins=(0, 1)):
+    rng = np.random.default_rng(seed)
+    qs = []
+    for q in range(n):
+        S = int(np.exp(rng.uniform(np.log(1), np.log(smax))))
+        k = int(rng.integers(kmax[0], kmax[1] + 1))
+        kmin = int(rng.choice(kmins))
+        qs.append(vs.BatchQuery(vs.sample_sources(V, S, seed * 1000 + q), kmin, k))
+    return qs
+
+
+@pytest.fixture(scope="module")
+def g14():
+    g = vs.HostCSR.rmat(14, 16, seed=4)
+    return g, vs.Graph.from_csr(g)
+
+
+def expected(orc, g, qs):
+    counts, digests = [], []
+    for q in qs:
+        mats, ka, te, _ = orc.vexpand_rows(g.offsets, g.adj, q.sources, q.k_min, q.k_max, q.path)
+        counts.append(orc.row_counts(mats[0]))
+        digests.append(orc.row_digests(mats[0], g.V))
+    return np.concatenate(counts), np.concatenate(digests)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+@pytest.mark.parametrize("sched", ["heuristic", "push", "pull"])
+def test_batch_matches_oracle(orc, g14, use_graph, sched):
+    g, G = g14
+    qs = make_queries(g.V, 12, seed=5)
+    c, d, st = vs.vexpand_batch(G, qs, sched=sched, use_graph=use_graph, tile_words=8)
+    ec, ed = expected(orc, g, qs)
+    assert np.array_equal(c, ec)
+    assert np.array_equal(d, ed)
+    assert st["num_tiles"] >= 1 and st["launches"] > 0
+
+
+def test_batch_many_waves_and_general_kmin(orc, g14):
+    g, G = g14
+    qs = make_queries(g.V, 10, seed=9, kmins=(0, 1, 2, 3))
+    # a tiny memory budget forces one slot per wave: many graph replays
+    c, d, st = vs.vexpand_batch(G, qs, tile_words=4, mem_budget=1, use_graph=True)
+    ec, ed = expected(orc, g, qs)
+    assert st["num_waves"] == st["num_tiles"] and st["num_waves"] > 1
+    assert np.array_equal(c, ec) and np.array_equal(d, ed)
+
+
+def test_batch_any_and_mixed(orc, g14):
+    g, G = g14
+    qs = make_queries(g.V, 6, seed=3)
+    for q in qs[::2]:
+        q.path = "any"
+    c, d, st = vs.vexpand_batch(G, qs, tile_words=16)
+    ec, ed = expected(orc, g, qs)
+    assert np.array_equal(c, ec) and np.array_equal(d, ed)
+
+
+def test_batch_te_equals_sum_of_plain(g14):
+    g, G = g14
+    qs = make_queries(g.V, 8, seed=11)
+    te = 0
+    for q in qs:
+        _, st = vs.vexpand(G, q.sources, q.k_min, q.k_max, q.path)
+        te += st["te"]
+    c, d, bst = vs.vexpand_batch(G, qs)
+    assert bst["te"] == te
+
+
+def test_batch_replays_are_identical(g14):
+    g, G = g14
+    qs = make_queries(g.V, 8, seed=12)
+    c1, d1, _ = vs.vexpand_batch(G, qs, replays=1)
+    c3, d3, st = vs.vexpand_batch(G, qs, replays=3)
+    assert np.array_equal(c1, c3) and np.array_equal(d1, d3)
+
+
+def test_batch_learned_scheduler_same_bits(orc, g14):
+    g, G = g14
+    s = vs.Scheduler(seed=5)
+    G.set_scheduler(s)
+    try:
+        qs = make_queries(g.V, 8, seed=13)
+        c, d, st = vs.vexpand_batch(G, qs, sched="learned")
+        ec, ed = expected(orc, g, qs)
+        assert np.array_equal(c, ec) and np.array_equal(d, ed)
+        assert st["push_decisions"] + st["pull_decisions"] > 0
+    finally:
+        G.set_scheduler(None)

@@ -171,6 +171,8 @@ def stats_dict(st: _lib.VexpandStats) -> dict:
         "frontier_bits": list(st.frontier_bits[:k]), "new_bits": list(st.new_bits[:k]),
         "ms_pull": st.ms_pull, "ms_push": st.ms_push, "n_pull": st.n_pull,
         "n_push": st.n_push, "bytes_pull": st.bytes_pull, "bytes_push": st.bytes_push,
+        "frontier_act": list(st.frontier_act[:k]), "frontier_deg": list(st.frontier_deg[:k]),
+        "vis_frac": list(st.vis_frac[:k]),
     }
 
 

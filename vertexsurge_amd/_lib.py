@@ -51,7 +51,9 @@ class VexpandStats(C.Structure):
                 ("frontier_bits", C.c_uint64 * MAX_HOPS), ("new_bits", C.c_uint64 * MAX_HOPS),
                 ("ms_pull", C.c_double), ("ms_push", C.c_double),
                 ("n_pull", C.c_uint64), ("n_push", C.c_uint64),
-                ("bytes_pull", C.c_uint64), ("bytes_push", C.c_uint64)]
+                ("bytes_pull", C.c_uint64), ("bytes_push", C.c_uint64),
+                ("frontier_act", C.c_uint64 * MAX_HOPS), ("frontier_deg", C.c_uint64 * MAX_HOPS),
+                ("vis_frac", C.c_double * MAX_HOPS)]
 
 
 class Tuning(C.Structure):

@@ -1,5 +1,763 @@
-// Batched VExpand (many queries packed into shared tiles) and the autotuner.
+// Batched VExpand: many independent queries packed into shared bit-matrix
+// tiles, expanded by one persistent kernel per hop, the whole launch
+// sequence of a wave of tiles captured once into a hipGraph and replayed for
+// every wave.  Also the tile-width / push-threshold autotuner.
+//
+// The reference has no batched path: each VExpand instance expands its own
+// source list (src/objects/operator/vexpand.cpp:166-284).  Source rows are
+// independent through every hop, the dedup and the union, so rows of
+// different queries can share a 64-bit word as long as every row keeps its own
+// hop range; that is what the per-word masks do:
+//   live[i][w]  rows of word w with k_max >= i      (expand at hop i)
+//   racc[i][w]  rows of word w with k_min <= i      (layer i joins the result)
+// A Shortest row whose query has k_min <= 1 reads its result from the visited
+// matrix (minus its source bit when k_min == 1), as the reference does for
+// k_min == 0 (vexpand.cpp:306-310); otherwise layers accumulate into R.
+//
+// Per wave: init (slot -> tile, counters, zero visited/result, L0) ->
+// for each hop: decide (push / pull per tile, on the device, heuristic or
+// MLP) -> build push items -> plan work units -> persistent expand (pull
+// vertex blocks and push items of every tile in one launch) -> push epilogue
+// -> clear the consumed frontier; then per-row counts / digests into the
+// caller's arrays and the wave counter advances.  No step reads the host.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
 #include "vs_kernels.hh"
+#include "vs_sched.hh"
+
+namespace vs {
+
+constexpr uint32_t kMaxSlots = 64;
+
+struct SlotDesc {
+  uint64_t *F[2];
+  uint64_t *bm[2];
+  uint64_t *N;
+  uint64_t *touched;
+  ull *items;
+  uint64_t *Vis;
+  uint64_t *R;
+  HopCounters *ctr;  // [H + 1]
+  uint32_t tile;     // global tile of the current wave, UINT32_MAX if idle
+  int32_t mode;      // this hop: 0 skip, 1 push, 2 pull
+};
+
+struct TileInfo {
+  uint32_t row0;
+  uint32_t rows;
+  uint32_t hmax;
+  uint32_t word0;
+};
+
+struct BatchCtl {
+  uint32_t wave;
+  uint32_t pad;
+  ull unit_prefix[kMaxSlots + 1];
+  ull te;
+  ull push_dec;
+  ull pull_dec;
+};
+
+struct BatchConst {
+  uint32_t V;
+  uint32_t TW;
+  uint32_t H;          // hops of the group (max k_max)
+  uint32_t nslots;
+  uint32_t ntiles;
+  uint32_t Wpad;       // words per hop row of the mask arrays (ntiles * TW)
+  uint32_t S;          // rows of the group
+  int32_t sched;
+  uint32_t alpha;
+  uint32_t use_r;
+  uint64_t E;
+  const uint64_t *poff;
+  const uint32_t *padj;
+  const uint64_t *xoff;
+  const uint32_t *xadj;
+  const uint32_t *src;    // [S]
+  const uint8_t *sub_src; // [S] result = visited minus the source bit
+  const uint64_t *live;   // [(H + 2) * Wpad]
+  const uint64_t *racc;   // [(H + 1) * Wpad] or nullptr
+  SlotDesc *slots;
+  const TileInfo *tiles;
+  BatchCtl *ctl;
+  ull *out_counts;        // [S] (group order), zeroed before the first wave
+  ull *out_digests;       // [S] or nullptr
+  const float *mlp;       // scheduler parameters or nullptr
+};
+
+__device__ __forceinline__ HopArgs slot_args(const BatchConst &c, const SlotDesc &sd,
+                                             uint32_t hop) {
+  HopArgs a;
+  const TileInfo ti = c.tiles[sd.tile];
+  a.V = c.V;
+  a.S = c.S;
+  a.word0 = ti.word0;
+  a.pad = 0;
+  a.poff = c.poff;
+  a.padj = c.padj;
+  a.xoff = c.xoff;
+  a.xadj = c.xadj;
+  a.F = sd.F[(hop - 1) & 1];
+  a.actbm = sd.bm[(hop - 1) & 1];
+  a.Fn = sd.F[hop & 1];
+  a.actbm_n = sd.bm[hop & 1];
+  a.N = sd.N;
+  a.touched = sd.touched;
+  a.Vis = sd.Vis;
+  a.R = c.use_r ? sd.R : nullptr;
+  a.items = sd.items;
+  a.nitems = &sd.ctr[hop - 1].items;
+  a.ctr = &sd.ctr[hop];
+  a.live = c.live + static_cast<size_t>(hop) * c.Wpad + ti.word0;
+  a.live_next = c.live + static_cast<size_t>(hop + 1) * c.Wpad + ti.word0;
+  a.racc = c.racc ? c.racc + static_cast<size_t>(hop) * c.Wpad + ti.word0 : nullptr;
+  return a;
+}
+
+// ---- wave init: slot -> tile, zero counters (one block)
+__global__ void k_bw_assign(BatchConst c) {
+  const uint32_t wave = c.ctl->wave;
+  for (uint32_t s = threadIdx.x; s < c.nslots; s += blockDim.x) {
+    const uint32_t t = wave * c.nslots + s;
+    c.slots[s].tile = t < c.ntiles ? t : UINT32_MAX;
+    c.slots[s].mode = 0;
+  }
+  for (uint32_t i = threadIdx.x; i < c.nslots * (c.H + 1); i += blockDim.x) {
+    HopCounters *h = &c.slots[i / (c.H + 1)].ctr[i % (c.H + 1)];
+    *h = HopCounters{};
+  }
+}
+
+// ---- zero visited / result matrices of active slots (fill kernel: no
+// memset node inside the captured graph)
+__global__ __launch_bounds__(kBlock) void k_bw_zero(BatchConst c) {
+  const size_t words = static_cast<size_t>(c.V) * c.TW;
+  const size_t per = words * (c.use_r ? 2 : 1);
+  const size_t total = per * c.nslots;
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < total;
+       i += stride) {
+    const uint32_t s = static_cast<uint32_t>(i / per);
+    const SlotDesc &sd = c.slots[s];
+    if (sd.tile == UINT32_MAX) continue;
+    const size_t k = i % per;
+    uint64_t *M = k < words ? sd.Vis : sd.R;
+    if (M) M[k % words] = 0;
+  }
+}
+
+// ---- L0 of every active slot, with its counters
+__global__ void k_bw_l0(BatchConst c) {
+  const uint32_t nvb = (c.V + 63) / 64;
+  const uint32_t rows_per_tile = c.TW * 64;
+  const uint32_t total = c.nslots * rows_per_tile;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += gridDim.x * blockDim.x) {
+    const uint32_t s = i / rows_per_tile, lr = i % rows_per_tile;
+    const SlotDesc &sd = c.slots[s];
+    if (sd.tile == UINT32_MAX) continue;
+    const TileInfo ti = c.tiles[sd.tile];
+    if (lr >= ti.rows) continue;
+    const uint32_t row = ti.row0 + lr;
+    const uint32_t v = c.src[row];
+    const uint32_t j = lr / 64;
+    const ull bit = 1ull << (lr % 64);
+    const size_t wi = static_cast<size_t>(v) * c.TW + j;
+    const uint64_t w_live1 = c.live[static_cast<size_t>(1) * c.Wpad + ti.word0 + j];
+    const uint64_t deg = c.xoff[v + 1] - c.xoff[v];
+    HopCounters *h = &sd.ctr[0];
+    const ull oldF = atomicOr(reinterpret_cast<ull *>(&sd.F[0][wi]), bit);
+    if (oldF == 0) atomicAdd(&h->nz, 1ull);
+    if (sd.Vis) {
+      const ull oldV = atomicOr(reinterpret_cast<ull *>(&sd.Vis[wi]), bit);
+      if (oldV == 0) atomicAdd(&h->visnew, 1ull);
+    }
+    if (c.use_r && c.racc && (c.racc[ti.word0 + j] & bit)) {
+      const ull oldR = atomicOr(reinterpret_cast<ull *>(&sd.R[wi]), bit);
+      if (oldR == 0) atomicAdd(&h->rnew, 1ull);
+    }
+    const ull oldB = atomicOr(reinterpret_cast<ull *>(&sd.bm[0][v / 64]), 1ull << (v % 64));
+    if (!(oldB & (1ull << (v % 64)))) {
+      atomicAdd(&h->act, 1ull);
+      atomicAdd(&h->actdeg, static_cast<ull>(deg));
+    }
+    atomicAdd(&h->pop, 1ull);
+    if (w_live1 & bit) atomicAdd(&h->te, static_cast<ull>(deg));
+    (void)nvb;
+  }
+}
+
+// ---- per-hop decision, one wave per slot
+__global__ __launch_bounds__(64) void k_bw_decide(BatchConst c, uint32_t hop) {
+  __shared__ float s_x[kIn];
+  __shared__ float s_h[2 * kHid];
+  const uint32_t s = blockIdx.x;
+  SlotDesc &sd = c.slots[s];
+  const int lane = lane_id();
+  int mode = 0;
+  if (sd.tile != UINT32_MAX) {
+    const TileInfo ti = c.tiles[sd.tile];
+    const HopCounters pc = sd.ctr[hop - 1];
+    if (pc.pop > 0 && hop <= ti.hmax) {
+      if (c.sched == VS_SCHED_PUSH) mode = 1;
+      else if (c.sched == VS_SCHED_PULL) mode = 2;
+      else {
+        const int heur = (pc.actdeg * static_cast<uint64_t>(c.alpha) < c.E) ? 1 : 2;
+        mode = heur;
+        if (c.sched == VS_SCHED_LEARNED && c.mlp) {
+          uint64_t visnz = 0;
+          for (uint32_t h = 0; h < hop; ++h) visnz += sd.ctr[h].visnew;
+          if (lane == 0) {
+            s_x[0] = log2f(1.f + static_cast<float>(pc.pop)) / 32.f;
+            s_x[1] = log2f(1.f + static_cast<float>(pc.act)) / 32.f;
+            s_x[2] = log2f(1.f + static_cast<float>(pc.actdeg)) / 40.f;
+            s_x[3] = static_cast<float>(hop) / 8.f;
+            s_x[4] = static_cast<float>(static_cast<double>(pc.pop) /
+                                        (static_cast<double>(ti.rows) * c.V));
+            s_x[5] = log2f(1.f + static_cast<float>(c.E)) / 40.f;
+            s_x[6] = log2f(1.f + static_cast<float>(c.V)) / 32.f;
+            s_x[7] = sd.Vis ? static_cast<float>(static_cast<double>(visnz) /
+                                                (static_cast<double>(c.V) * c.TW))
+                           : 0.f;
+          }
+          wave_sync();
+          float h1, h2, y;
+          mlp_forward_wave(c.mlp, s_x, s_h, h1, h2, y);
+          const float tp = __shfl(y, kOptPush), tl = __shfl(y, kOptPull);
+          if (isfinite(tp) && isfinite(tl) && fabsf(tp - tl) >= 0.1f) mode = tp < tl ? 1 : 2;
+        }
+      }
+    }
+  }
+  if (lane == 0) {
+    sd.mode = mode;
+    if (mode == 1) atomicAdd(&c.ctl->push_dec, 1ull);
+    if (mode == 2) atomicAdd(&c.ctl->pull_dec, 1ull);
+  }
+}
+
+// ---- push items for push slots (whole blocks per bitmap slice)
+__global__ __launch_bounds__(kBlock) void k_bw_items(BatchConst c, uint32_t hop) {
+  __shared__ ull s_wsum[kWavesPerBlock];
+  __shared__ ull s_base;
+  const uint32_t nvb = (c.V + 63) / 64;
+  const uint32_t nslice = (nvb + kBlock - 1) / kBlock;
+  for (uint32_t u = blockIdx.x; u < c.nslots * nslice; u += gridDim.x) {
+    const uint32_t s = u / nslice;
+    const SlotDesc &sd = c.slots[s];
+    if (sd.mode != 1) continue;  // uniform across the block
+    items_slice(sd.bm[(hop - 1) & 1], nvb, (u % nslice) * kBlock, c.xoff, sd.items,
+                &sd.ctr[hop - 1].items, s_wsum, &s_base);
+  }
+}
+
+// ---- unit prefix over slots (one thread)
+__global__ void k_bw_plan(BatchConst c, uint32_t hop) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint32_t nvb = (c.V + 63) / 64;
+  ull acc = 0;
+  for (uint32_t s = 0; s < c.nslots; ++s) {
+    c.ctl->unit_prefix[s] = acc;
+    const SlotDesc &sd = c.slots[s];
+    if (sd.mode == 2) acc += nvb;
+    else if (sd.mode == 1) acc += sd.ctr[hop - 1].items;
+  }
+  c.ctl->unit_prefix[c.nslots] = acc;
+}
+
+// ---- the persistent expansion: pull vertex-blocks and push items of all
+// slots; work unit u belongs to the slot whose prefix range holds it
+template <int TW, bool SHORTEST>
+__global__ __launch_bounds__(kBlock) void k_bw_expand(BatchConst c, uint32_t hop) {
+  __shared__ uint32_t s_nbr[kWavesPerBlock][64];
+  const int wid = threadIdx.x >> 6;
+  const ull total = c.ctl->unit_prefix[c.nslots];
+  const ull nwaves = static_cast<ull>(gridDim.x) * kWavesPerBlock;
+  uint32_t cur = UINT32_MAX;
+  HopArgs a;
+  LaneMasks mk;
+  int mode = 0;
+  LocalCtr lc;
+  for (ull u = blockIdx.x * kWavesPerBlock + wid; u < total; u += nwaves) {
+    uint32_t s = cur == UINT32_MAX ? 0 : cur;
+    while (s + 1 <= c.nslots && c.ctl->unit_prefix[s + 1] <= u) ++s;
+    while (c.ctl->unit_prefix[s] > u) --s;
+    if (s != cur) {
+      if (cur != UINT32_MAX && mode == 2) flush_ctr(lc, a.ctr);
+      cur = s;
+      a = slot_args(c, c.slots[s], hop);
+      mode = c.slots[s].mode;
+      mk = lane_masks(a, lane_id() % TW);
+    }
+    const ull local = u - c.ctl->unit_prefix[s];
+    if (mode == 2) pull_block<TW, SHORTEST>(a, static_cast<uint32_t>(local), s_nbr[wid], mk, lc);
+    else push_item<TW>(a, a.items[local]);
+  }
+  if (cur != UINT32_MAX && mode == 2) flush_ctr(lc, a.ctr);
+}
+
+// ---- push epilogue and frontier clear, grid-stride over (slot, vertex block)
+template <int TW, bool SHORTEST>
+__global__ __launch_bounds__(kBlock) void k_bw_epi(BatchConst c, uint32_t hop) {
+  const int wid = threadIdx.x >> 6;
+  const uint32_t nvb = (c.V + 63) / 64;
+  const ull total = static_cast<ull>(c.nslots) * nvb;
+  const ull nwaves = static_cast<ull>(gridDim.x) * kWavesPerBlock;
+  uint32_t cur = UINT32_MAX;
+  HopArgs a;
+  LaneMasks mk;
+  LocalCtr lc;
+  for (ull u = blockIdx.x * kWavesPerBlock + wid; u < total; u += nwaves) {
+    const uint32_t s = static_cast<uint32_t>(u / nvb);
+    if (c.slots[s].mode != 1) continue;
+    if (s != cur) {
+      if (cur != UINT32_MAX) flush_ctr(lc, a.ctr);
+      cur = s;
+      a = slot_args(c, c.slots[s], hop);
+      mk = lane_masks(a, lane_id() % TW);
+    }
+    epi_block<TW, SHORTEST>(a, static_cast<uint32_t>(u % nvb), mk, lc);
+  }
+  if (cur != UINT32_MAX) flush_ctr(lc, a.ctr);
+}
+
+template <int TW>
+__global__ __launch_bounds__(kBlock) void k_bw_clear(BatchConst c, uint32_t which) {
+  const int wid = threadIdx.x >> 6;
+  const uint32_t nvb = (c.V + 63) / 64;
+  const ull total = static_cast<ull>(c.nslots) * nvb;
+  const ull nwaves = static_cast<ull>(gridDim.x) * kWavesPerBlock;
+  for (ull u = blockIdx.x * kWavesPerBlock + wid; u < total; u += nwaves) {
+    const SlotDesc &sd = c.slots[u / nvb];
+    if (sd.tile == UINT32_MAX) continue;
+    clear_block<TW>(sd.F[which & 1], sd.bm[which & 1], static_cast<uint32_t>(u % nvb));
+  }
+}
+
+// ---- per-row popcount and digest of each active slot's result
+template <int TW>
+__global__ __launch_bounds__(kBlock) void k_bw_rows(BatchConst c) {
+  constexpr int G = 64 / TW;
+  constexpr uint32_t CV = 8192;
+  constexpr int NB = 14;
+  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const int g = lane / TW, j = lane % TW;
+  const uint32_t nch = (c.V + CV - 1) / CV;
+  const ull total = static_cast<ull>(c.nslots) * nch;
+  const ull nwaves = static_cast<ull>(gridDim.x) * kWavesPerBlock;
+  for (ull it = blockIdx.x * kWavesPerBlock + wid; it < total; it += nwaves) {
+    const SlotDesc &sd = c.slots[it / nch];
+    if (sd.tile == UINT32_MAX) continue;
+    const TileInfo ti = c.tiles[sd.tile];
+    const uint64_t *M = c.use_r ? sd.R : sd.Vis;
+    const uint32_t ch = static_cast<uint32_t>(it % nch);
+    const uint32_t v0 = ch * CV, v1 = min(c.V, v0 + CV);
+    uint64_t cnt[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) cnt[k] = 0;
+    for (uint32_t v = v0 + g; v < v1; v += G) {
+      uint64_t carry = M[static_cast<size_t>(v) * TW + j];
+#pragma unroll
+      for (int k = 0; k < NB; ++k) {
+        const uint64_t t2 = cnt[k] & carry;
+        cnt[k] ^= carry;
+        carry = t2;
+      }
+    }
+    for (int b = 0; b < 64; ++b) {
+      uint32_t n = 0;
+#pragma unroll
+      for (int k = 0; k < NB; ++k) n |= static_cast<uint32_t>((cnt[k] >> b) & 1) << k;
+#pragma unroll
+      for (int m = TW; m < 64; m <<= 1) n += __shfl_xor(n, m);
+      const uint32_t lr = j * 64 + b;
+      if (g == 0 && n && lr < ti.rows) atomicAdd(&c.out_counts[ti.row0 + lr], static_cast<ull>(n));
+    }
+  }
+  if (!c.out_digests) return;
+  // digests: lane = row bit, walk 64-vertex blocks of each word
+  const uint32_t nvb = (c.V + 63) / 64;
+  constexpr uint32_t BPW = 16;
+  const uint32_t nitem_v = (nvb + BPW - 1) / BPW;
+  const ull total2 = static_cast<ull>(c.nslots) * TW * nitem_v;
+  for (ull it = blockIdx.x * kWavesPerBlock + wid; it < total2; it += nwaves) {
+    const uint32_t iv = static_cast<uint32_t>(it % nitem_v);
+    const uint32_t sj = static_cast<uint32_t>(it / nitem_v);
+    const SlotDesc &sd = c.slots[sj / TW];
+    if (sd.tile == UINT32_MAX) continue;
+    const TileInfo ti = c.tiles[sd.tile];
+    const uint32_t jj = sj % TW;
+    if (jj * 64 >= ti.rows) continue;
+    const uint64_t *M = (c.use_r ? sd.R : sd.Vis) + jj;
+    uint64_t acc = 0;
+    for (uint32_t vb = iv * BPW; vb < min(nvb, (iv + 1) * BPW); ++vb) {
+      const uint32_t v = vb * 64 + lane;
+      const uint64_t mine = v < c.V ? M[static_cast<size_t>(v) * TW] : 0;
+      uint64_t any = __ballot(mine != 0);
+      while (any) {
+        const int l = __ffsll(static_cast<long long>(any)) - 1;
+        any &= any - 1;
+        const uint64_t x = shfl64(mine, l);
+        if ((x >> lane) & 1) acc += digest_hash(vb * 64 + l);
+      }
+    }
+    const uint32_t lr = jj * 64 + lane;
+    if (acc && lr < ti.rows) atomicAdd(&c.out_digests[ti.row0 + lr], static_cast<ull>(acc));
+  }
+}
+
+// ---- visited-trick correction, TE accumulation, advance the wave
+__global__ void k_bw_finish(BatchConst c) {
+  const uint32_t rows_per_tile = c.TW * 64;
+  const uint32_t total = c.nslots * rows_per_tile;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += gridDim.x * blockDim.x) {
+    const SlotDesc &sd = c.slots[i / rows_per_tile];
+    if (sd.tile == UINT32_MAX) continue;
+    const TileInfo ti = c.tiles[sd.tile];
+    const uint32_t lr = i % rows_per_tile;
+    if (lr >= ti.rows) continue;
+    const uint32_t row = ti.row0 + lr;
+    if (!c.use_r && c.sub_src[row]) {
+      atomicAdd(&c.out_counts[row], ~0ull);  // minus one
+      if (c.out_digests) atomicAdd(&c.out_digests[row], 0ull - digest_hash(c.src[row]));
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ull te = 0;
+    for (uint32_t s = 0; s < c.nslots; ++s) {
+      if (c.slots[s].tile == UINT32_MAX) continue;
+      for (uint32_t h = 0; h < c.H; ++h) te += c.slots[s].ctr[h].te;
+    }
+    c.ctl->te += te;
+  }
+}
+
+__global__ void k_bw_advance(BatchCtl *ctl) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) ctl->wave += 1;
+}
+
+#define VS_TW_BATCH(tw, FN, ...)          \
+  switch (tw) {                           \
+    case 1: FN<1>(__VA_ARGS__); break;    \
+    case 2: FN<2>(__VA_ARGS__); break;    \
+    case 4: FN<4>(__VA_ARGS__); break;    \
+    case 8: FN<8>(__VA_ARGS__); break;    \
+    case 16: FN<16>(__VA_ARGS__); break;  \
+    case 32: FN<32>(__VA_ARGS__); break;  \
+    default: FN<64>(__VA_ARGS__); break;  \
+  }
+
+template <int TW>
+static void enqueue_hop_tw(const BatchConst &c, bool shortest, uint32_t hop,
+                           uint32_t grid, hipStream_t s) {
+  if (shortest) {
+    hipLaunchKernelGGL((k_bw_expand<TW, true>), dim3(grid), dim3(kBlock), 0, s, c, hop);
+    hipLaunchKernelGGL((k_bw_epi<TW, true>), dim3(grid), dim3(kBlock), 0, s, c, hop);
+  } else {
+    hipLaunchKernelGGL((k_bw_expand<TW, false>), dim3(grid), dim3(kBlock), 0, s, c, hop);
+    hipLaunchKernelGGL((k_bw_epi<TW, false>), dim3(grid), dim3(kBlock), 0, s, c, hop);
+  }
+  hipLaunchKernelGGL((k_bw_clear<TW>), dim3(grid), dim3(kBlock), 0, s, c, hop - 1);
+}
+
+template <int TW>
+static void enqueue_final_tw(const BatchConst &c, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_bw_clear<TW>), dim3(grid), dim3(kBlock), 0, s, c, c.H);
+  hipLaunchKernelGGL((k_bw_rows<TW>), dim3(grid), dim3(kBlock), 0, s, c);
+}
+
+// Enqueues the whole launch sequence of one wave; returns the number of
+// kernel launches.
+static uint32_t enqueue_wave(const BatchConst &c, bool shortest, uint32_t grid,
+                             hipStream_t s) {
+  uint32_t n = 0;
+  hipLaunchKernelGGL(k_bw_assign, dim3(1), dim3(256), 0, s, c);
+  hipLaunchKernelGGL(k_bw_zero, dim3(grid), dim3(kBlock), 0, s, c);
+  hipLaunchKernelGGL(k_bw_l0, dim3(std::max<uint32_t>(1, (c.nslots * c.TW * 64 + 255) / 256)),
+                     dim3(256), 0, s, c);
+  n += 3;
+  const uint32_t nvb = (c.V + 63) / 64;
+  const uint32_t item_grid =
+      std::min<uint32_t>(c.nslots * ((nvb + kBlock - 1) / kBlock), grid);
+  for (uint32_t hop = 1; hop <= c.H; ++hop) {
+    hipLaunchKernelGGL(k_bw_decide, dim3(c.nslots), dim3(64), 0, s, c, hop);
+    hipLaunchKernelGGL(k_bw_items, dim3(std::max<uint32_t>(1, item_grid)), dim3(kBlock), 0, s, c, hop);
+    hipLaunchKernelGGL(k_bw_plan, dim3(1), dim3(64), 0, s, c, hop);
+    VS_TW_BATCH(c.TW, enqueue_hop_tw, c, shortest, hop, grid, s);
+    n += 6;
+  }
+  VS_TW_BATCH(c.TW, enqueue_final_tw, c, grid, s);
+  hipLaunchKernelGGL(k_bw_finish, dim3(std::max<uint32_t>(1, (c.nslots * c.TW * 64 + 255) / 256)),
+                     dim3(256), 0, s, c);
+  hipLaunchKernelGGL(k_bw_advance, dim3(1), dim3(64), 0, s, c.ctl);
+  n += 4;
+  return n;
+}
+
+struct DevVec {
+  void *p = nullptr;
+  ~DevVec() { if (p) (void)hipFree(p); }
+  vs_status alloc(size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      set_error("batch: device allocation of %zu bytes failed", bytes);
+      return VS_ERR_NOMEM;
+    }
+    return VS_OK;
+  }
+};
+
+// One group = queries of one path type.
+static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &qs,
+                           const vs_batch_params &p, bool shortest,
+                           std::vector<uint64_t> &counts, std::vector<uint64_t> &digests,
+                           bool want_digests, vs_batch_stats &st) {
+  const DevCSR *push = nullptr, *pull = nullptr;
+  VS_TRY(graph_csrs(g, p.direction, &push, &pull));
+  const uint32_t V = g->V, nvb = (V + 63) / 64;
+  uint64_t S64 = 0;
+  uint32_t H = 0;
+  bool use_r = !shortest;
+  for (auto *q : qs) {
+    S64 += q->S;
+    H = std::max(H, q->k_max);
+    if (q->k_min >= 2) use_r = true;
+  }
+  if (S64 == 0) return VS_OK;
+  if (S64 > 0xFFFFFFF0ull) { set_error("batch: too many rows"); return VS_ERR_INVALID; }
+  const uint32_t S = static_cast<uint32_t>(S64);
+  const uint32_t W = (S + 63) / 64;
+  const uint32_t TW = choose_tile_words(p.tile_words ? p.tile_words : g->tuning.tile_words, W);
+  const uint32_t T = (W + TW - 1) / TW;
+  const uint32_t Wpad = T * TW;
+  // host tables
+  std::vector<uint32_t> src(S);
+  std::vector<uint8_t> sub(S, 0);
+  std::vector<uint32_t> kmin(S), kmax(S);
+  {
+    uint32_t r = 0;
+    for (auto *q : qs) {
+      for (uint32_t i = 0; i < q->S; ++i, ++r) {
+        if (q->sources[i] >= V) {
+          set_error("batch: source %u out of range (V=%u)", q->sources[i], V);
+          return VS_ERR_INVALID;
+        }
+        src[r] = q->sources[i];
+        kmin[r] = q->k_min;
+        kmax[r] = q->k_max;
+        sub[r] = (!use_r && q->k_min == 1) ? 1 : 0;
+      }
+    }
+  }
+  std::vector<uint64_t> live(static_cast<size_t>(H + 2) * Wpad, 0);
+  std::vector<uint64_t> racc;
+  if (use_r) racc.assign(static_cast<size_t>(H + 1) * Wpad, 0);
+  for (uint32_t r = 0; r < S; ++r) {
+    const uint32_t w = r / 64;
+    const uint64_t bit = 1ull << (r % 64);
+    for (uint32_t i = 0; i <= std::min(kmax[r], H + 1); ++i)
+      live[static_cast<size_t>(i) * Wpad + w] |= bit;
+    if (use_r)
+      for (uint32_t i = kmin[r]; i <= kmax[r]; ++i)
+        racc[static_cast<size_t>(i) * Wpad + w] |= bit;
+  }
+  std::vector<TileInfo> tiles(T);
+  for (uint32_t t = 0; t < T; ++t) {
+    tiles[t].row0 = t * TW * 64;
+    tiles[t].rows = std::min<uint32_t>(S - tiles[t].row0, TW * 64);
+    tiles[t].word0 = t * TW;
+    uint32_t h = 0;
+    for (uint32_t r = tiles[t].row0; r < tiles[t].row0 + tiles[t].rows; ++r)
+      h = std::max(h, kmax[r]);
+    tiles[t].hmax = h;
+  }
+  // slots that fit the memory budget
+  const size_t mat = static_cast<size_t>(V) * TW * 8;
+  const size_t items_cap = static_cast<size_t>(V) + push->E / kPushChunk + 64;
+  const size_t per_slot = mat * (shortest ? 4 : 3) + (use_r ? mat : 0) + 3 * nvb * 8 +
+                          items_cap * 8 + (H + 1) * sizeof(HopCounters);
+  size_t freeb = 0, totalb = 0;
+  VS_HIP(hipMemGetInfo(&freeb, &totalb));
+  const size_t budget = p.mem_budget ? p.mem_budget : freeb / 10 * 7;
+  uint32_t nslots = static_cast<uint32_t>(std::min<size_t>(budget / per_slot, T));
+  nslots = std::min<uint32_t>(nslots, kMaxSlots);
+  if (nslots == 0 && per_slot <= freeb) nslots = 1;  // the budget is a soft cap
+  if (nslots == 0) {
+    set_error("batch: one tile needs %zu bytes, budget %zu", per_slot, budget);
+    return VS_ERR_NOMEM;
+  }
+  const uint32_t nwaves = (T + nslots - 1) / nslots;
+  hipStream_t s = g->stream;
+  // device arrays
+  DevVec d_src, d_sub, d_live, d_racc, d_tiles, d_slots, d_ctl, d_counts, d_dig;
+  VS_TRY(d_src.alloc(S * 4ull));
+  VS_TRY(d_sub.alloc(S));
+  VS_TRY(d_live.alloc(live.size() * 8));
+  if (use_r) VS_TRY(d_racc.alloc(racc.size() * 8));
+  VS_TRY(d_tiles.alloc(T * sizeof(TileInfo)));
+  VS_TRY(d_slots.alloc(nslots * sizeof(SlotDesc)));
+  VS_TRY(d_ctl.alloc(sizeof(BatchCtl)));
+  VS_TRY(d_counts.alloc(S * 8ull));
+  if (want_digests) VS_TRY(d_dig.alloc(S * 8ull));
+  VS_HIP(hipMemcpyAsync(d_src.p, src.data(), S * 4ull, hipMemcpyHostToDevice, s));
+  VS_HIP(hipMemcpyAsync(d_sub.p, sub.data(), S, hipMemcpyHostToDevice, s));
+  VS_HIP(hipMemcpyAsync(d_live.p, live.data(), live.size() * 8, hipMemcpyHostToDevice, s));
+  if (use_r) VS_HIP(hipMemcpyAsync(d_racc.p, racc.data(), racc.size() * 8, hipMemcpyHostToDevice, s));
+  VS_HIP(hipMemcpyAsync(d_tiles.p, tiles.data(), T * sizeof(TileInfo), hipMemcpyHostToDevice, s));
+  // slot buffers (clean: zero once, kept clean by the kernels)
+  std::vector<DevVec> bufs;
+  bufs.reserve(nslots * 10);
+  std::vector<SlotDesc> sd(nslots);
+  auto get = [&](size_t bytes, bool zero, void **out) -> vs_status {
+    bufs.emplace_back();
+    VS_TRY(bufs.back().alloc(bytes));
+    if (zero) VS_HIP(hipMemsetAsync(bufs.back().p, 0, bytes, s));
+    *out = bufs.back().p;
+    return VS_OK;
+  };
+  for (uint32_t i = 0; i < nslots; ++i) {
+    void *q;
+    VS_TRY(get(mat, true, &q)); sd[i].F[0] = static_cast<uint64_t *>(q);
+    VS_TRY(get(mat, true, &q)); sd[i].F[1] = static_cast<uint64_t *>(q);
+    VS_TRY(get(nvb * 8ull, true, &q)); sd[i].bm[0] = static_cast<uint64_t *>(q);
+    VS_TRY(get(nvb * 8ull, true, &q)); sd[i].bm[1] = static_cast<uint64_t *>(q);
+    VS_TRY(get(mat, true, &q)); sd[i].N = static_cast<uint64_t *>(q);
+    VS_TRY(get(nvb * 8ull, true, &q)); sd[i].touched = static_cast<uint64_t *>(q);
+    VS_TRY(get(items_cap * 8, false, &q)); sd[i].items = static_cast<ull *>(q);
+    sd[i].Vis = nullptr;
+    sd[i].R = nullptr;
+    if (shortest) { VS_TRY(get(mat, false, &q)); sd[i].Vis = static_cast<uint64_t *>(q); }
+    if (use_r) { VS_TRY(get(mat, false, &q)); sd[i].R = static_cast<uint64_t *>(q); }
+    VS_TRY(get((H + 1) * sizeof(HopCounters), true, &q));
+    sd[i].ctr = static_cast<HopCounters *>(q);
+    sd[i].tile = UINT32_MAX;
+    sd[i].mode = 0;
+  }
+  VS_HIP(hipMemcpyAsync(d_slots.p, sd.data(), nslots * sizeof(SlotDesc), hipMemcpyHostToDevice, s));
+  BatchCtl ctl0{};
+  VS_HIP(hipMemcpyAsync(d_ctl.p, &ctl0, sizeof(ctl0), hipMemcpyHostToDevice, s));
+  VS_HIP(hipMemsetAsync(d_counts.p, 0, S * 8ull, s));
+  if (want_digests) VS_HIP(hipMemsetAsync(d_dig.p, 0, S * 8ull, s));
+
+  BatchConst c{};
+  c.V = V;
+  c.TW = TW;
+  c.H = H;
+  c.nslots = nslots;
+  c.ntiles = T;
+  c.Wpad = Wpad;
+  c.S = S;
+  c.sched = p.sched;
+  c.alpha = g->tuning.push_alpha ? g->tuning.push_alpha : 16;
+  c.use_r = use_r ? 1 : 0;
+  c.E = push->E;
+  c.poff = pull->off;
+  c.padj = pull->adj;
+  c.xoff = push->off;
+  c.xadj = push->adj;
+  c.src = static_cast<const uint32_t *>(d_src.p);
+  c.sub_src = static_cast<const uint8_t *>(d_sub.p);
+  c.live = static_cast<const uint64_t *>(d_live.p);
+  c.racc = use_r ? static_cast<const uint64_t *>(d_racc.p) : nullptr;
+  c.slots = static_cast<SlotDesc *>(d_slots.p);
+  c.tiles = static_cast<const TileInfo *>(d_tiles.p);
+  c.ctl = static_cast<BatchCtl *>(d_ctl.p);
+  c.out_counts = static_cast<ull *>(d_counts.p);
+  c.out_digests = want_digests ? static_cast<ull *>(d_dig.p) : nullptr;
+  c.mlp = (p.sched == VS_SCHED_LEARNED && g->sched) ? sched_params(g->sched) : nullptr;
+  const uint32_t grid = static_cast<uint32_t>(g->num_cus) * 8;
+
+  hipEvent_t e0, e1;
+  VS_HIP(hipEventCreate(&e0));
+  VS_HIP(hipEventCreate(&e1));
+  struct EvG { hipEvent_t a, b; ~EvG() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); } } evg{e0, e1};
+  VS_HIP(hipStreamSynchronize(s));
+  uint32_t launches = 0;
+  double ms = 0.0;
+  const uint32_t replays = std::max<uint32_t>(1, p.replays);
+  if (p.use_graph) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const auto c0 = std::chrono::steady_clock::now();
+    VS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    launches = enqueue_wave(c, shortest, grid, s);
+    hipError_t ce = hipStreamEndCapture(s, &graph);
+    if (ce != hipSuccess) {
+      set_error("batch: graph capture failed: %s", hipGetErrorString(ce));
+      return VS_ERR_HIP;
+    }
+    ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ce != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      set_error("batch: graph instantiate failed: %s", hipGetErrorString(ce));
+      return VS_ERR_HIP;
+    }
+    st.ms_capture += std::chrono::duration<double, std::milli>(
+                         std::chrono::steady_clock::now() - c0).count();
+    hipError_t re = hipSuccess;
+    for (uint32_t rep = 0; rep < replays && re == hipSuccess; ++rep) {
+      if (rep > 0) {  // timing replays: restart from wave 0 with clean outputs
+        re = hipMemcpyAsync(d_ctl.p, &ctl0, sizeof(ctl0), hipMemcpyHostToDevice, s);
+        if (re == hipSuccess) re = hipMemsetAsync(d_counts.p, 0, S * 8ull, s);
+        if (re == hipSuccess && want_digests) re = hipMemsetAsync(d_dig.p, 0, S * 8ull, s);
+      }
+      if (re == hipSuccess) re = hipEventRecord(e0, s);
+      for (uint32_t w = 0; w < nwaves && re == hipSuccess; ++w) re = hipGraphLaunch(exec, s);
+      if (re == hipSuccess) re = hipEventRecord(e1, s);
+      if (re == hipSuccess) re = hipStreamSynchronize(s);
+    }
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (re != hipSuccess) {
+      set_error("batch: graph replay failed: %s", hipGetErrorString(re));
+      return VS_ERR_HIP;
+    }
+  } else {
+    for (uint32_t rep = 0; rep < replays; ++rep) {
+      if (rep > 0) {
+        VS_HIP(hipMemcpyAsync(d_ctl.p, &ctl0, sizeof(ctl0), hipMemcpyHostToDevice, s));
+        VS_HIP(hipMemsetAsync(d_counts.p, 0, S * 8ull, s));
+        if (want_digests) VS_HIP(hipMemsetAsync(d_dig.p, 0, S * 8ull, s));
+      }
+      VS_HIP(hipEventRecord(e0, s));
+      for (uint32_t w = 0; w < nwaves; ++w) launches = enqueue_wave(c, shortest, grid, s);
+      VS_HIP(hipGetLastError());
+      VS_HIP(hipEventRecord(e1, s));
+      VS_HIP(hipStreamSynchronize(s));
+    }
+  }
+  float fms = 0.f;
+  VS_HIP(hipEventElapsedTime(&fms, e0, e1));
+  ms = fms;
+  BatchCtl ctl{};
+  VS_HIP(hipMemcpy(&ctl, d_ctl.p, sizeof(ctl), hipMemcpyDeviceToHost));
+  std::vector<uint64_t> hc(S), hd;
+  VS_HIP(hipMemcpy(hc.data(), d_counts.p, S * 8ull, hipMemcpyDeviceToHost));
+  if (want_digests) {
+    hd.resize(S);
+    VS_HIP(hipMemcpy(hd.data(), d_dig.p, S * 8ull, hipMemcpyDeviceToHost));
+  }
+  counts.insert(counts.end(), hc.begin(), hc.end());
+  if (want_digests) digests.insert(digests.end(), hd.begin(), hd.end());
+  st.te += ctl.te;
+  st.ms_total += ms;
+  st.num_tiles += T;
+  st.num_waves += nwaves;
+  st.launches += launches;
+  st.push_decisions += ctl.push_dec / replays;
+  st.pull_decisions += ctl.pull_dec / replays;
+  return VS_OK;
+}
+
+} // namespace vs
 
 using namespace vs;
 
@@ -8,18 +766,100 @@ extern "C" {
 vs_status vs_vexpand_batch(vs_graph_t g, const vs_query *queries, uint32_t nq,
                            const vs_batch_params *p, uint64_t *row_counts,
                            uint64_t *row_digests, vs_batch_stats *stats) {
-  (void)g; (void)queries; (void)nq; (void)p; (void)row_counts; (void)row_digests;
-  (void)stats;
-  set_error("vs_vexpand_batch: not built yet");
-  return VS_ERR_UNSUPPORTED;
+  clear_error();
+  if (!g || !p || (nq > 0 && !queries)) {
+    set_error("vs_vexpand_batch: null argument");
+    return VS_ERR_INVALID;
+  }
+  if (p->tile_words > 64 || (p->tile_words & (p->tile_words - 1))) {
+    set_error("vs_vexpand_batch: tile_words %u is not a power of two <= 64", p->tile_words);
+    return VS_ERR_INVALID;
+  }
+  if (p->sched < VS_SCHED_HEURISTIC || p->sched > VS_SCHED_LEARNED) {
+    set_error("vs_vexpand_batch: unknown sched %d", p->sched);
+    return VS_ERR_INVALID;
+  }
+  for (uint32_t q = 0; q < nq; ++q) {
+    const vs_query &Q = queries[q];
+    if (Q.path_type != VS_PATH_ANY && Q.path_type != VS_PATH_SHORTEST) {
+      set_error("vs_vexpand_batch: query %u: path_type %d not batchable", q, Q.path_type);
+      return VS_ERR_INVALID;
+    }
+    if (Q.k_max >= VS_MAX_HOPS || (Q.S > 0 && !Q.sources)) {
+      set_error("vs_vexpand_batch: query %u: bad k_max or null sources", q);
+      return VS_ERR_INVALID;
+    }
+  }
+  VS_HIP(hipSetDevice(g->device));
+  vs_batch_stats st{};
+  std::vector<uint64_t> out_c, out_d;
+  std::vector<uint64_t> row_off(nq + 1, 0);
+  for (uint32_t q = 0; q < nq; ++q) row_off[q + 1] = row_off[q] + queries[q].S;
+  const bool want_d = row_digests != nullptr;
+  for (int path : {VS_PATH_SHORTEST, VS_PATH_ANY}) {
+    std::vector<const vs_query *> qs;
+    std::vector<uint32_t> ids;
+    for (uint32_t q = 0; q < nq; ++q)
+      if (queries[q].path_type == path) { qs.push_back(&queries[q]); ids.push_back(q); }
+    if (qs.empty()) continue;
+    out_c.clear();
+    out_d.clear();
+    VS_TRY(run_group(g, qs, *p, path == VS_PATH_SHORTEST, out_c, out_d, want_d, st));
+    uint64_t r = 0;
+    for (uint32_t id : ids) {
+      for (uint32_t i = 0; i < queries[id].S; ++i, ++r) {
+        if (row_counts) row_counts[row_off[id] + i] = out_c[r];
+        if (want_d) row_digests[row_off[id] + i] = out_d[r];
+      }
+    }
+  }
+  if (stats) *stats = st;
+  return VS_OK;
 }
 
+// Candidate storage tile widths x push thresholds, timed on the caller's
+// sources (Shortest or Any, hops 1..k_max), best kept in the graph tuning.
 vs_status vs_autotune_vexpand(vs_graph_t g, const uint32_t *sources, uint32_t S,
                               uint32_t k_max, int32_t path_type, vs_tuning *best,
                               double *best_ms) {
-  (void)g; (void)sources; (void)S; (void)k_max; (void)path_type; (void)best; (void)best_ms;
-  set_error("vs_autotune_vexpand: not built yet");
-  return VS_ERR_UNSUPPORTED;
+  clear_error();
+  if (!g || !sources || S == 0 || k_max == 0) {
+    set_error("vs_autotune_vexpand: need a graph, sources and k_max >= 1");
+    return VS_ERR_INVALID;
+  }
+  const uint32_t W = (S + 63) / 64;
+  const uint32_t tw_max = choose_tile_words(64, W);
+  vs_tuning saved = g->tuning, bt = g->tuning;
+  double bms = 1e300;
+  for (uint32_t tw = std::max<uint32_t>(1, tw_max / 4); tw <= tw_max; tw *= 2) {
+    for (uint32_t alpha : {4u, 16u, 64u}) {
+      vs_tuning t{tw, alpha, 0, 0};
+      g->tuning = t;
+      vs_vexpand_params pp{};
+      pp.k_min = 1;
+      pp.k_max = k_max;
+      pp.path_type = path_type;
+      pp.direction = VS_DIR_FORWARD;
+      pp.sched = VS_SCHED_HEURISTIC;
+      double ms = 0;
+      for (int rep = 0; rep < 2; ++rep) {  // first run warms the pool
+        vs_vexpand_stats st;
+        vs_result_t r = nullptr;
+        vs_status s = vs_vexpand(g, sources, S, &pp, &r, &st);
+        if (s != VS_OK) { g->tuning = saved; return s; }
+        vs_result_free(r);
+        ms = st.ms_total;
+      }
+      if (ms < bms) {
+        bms = ms;
+        bt = t;
+      }
+    }
+  }
+  g->tuning = bt;
+  if (best) *best = bt;
+  if (best_ms) *best_ms = bms;
+  return VS_OK;
 }
 
 } // extern "C"

@@ -21,27 +21,32 @@
 // Both produce identical bits.  Frontier buffers and their active bitmaps are
 // all-zero outside their active rows, and are cleared through the bitmap
 // after use, so no hop pays for a full-matrix memset.
+//
+// Row masks: `live` bits are the rows that still expand at this hop (rows
+// < S, and in a batch the rows whose query has k_max >= hop); `racc` bits are
+// the rows whose result takes this hop's layer (k_min <= hop); `live_next`
+// are the rows that will expand at the next hop (traversed-edge count).
 #pragma once
 
 #include "vs_device.hh"
 
 namespace vs {
 
-// Per (hop, tile) counters, filled by the kernel that writes layer L_i.
+// Per (hop, tile) counters, filled by the kernels that write layer L_i.
 struct HopCounters {
   ull pop;     // popcount of L_i
   ull nz;      // nonzero words of L_i
   ull act;     // vertices with a nonzero L_i row
   ull actdeg;  // sum of push-degree over those vertices
-  ull te;      // sum over rows of push-degree * popcount(row): TE of hop i+1
+  ull te;      // sum over rows live at hop i+1 of push-degree * bits: TE of hop i+1
   ull visnew;  // visited words that went from zero to nonzero
   ull rnew;    // result words that went from zero to nonzero
-  ull items;   // push work items built from L_i (written by k_items)
+  ull items;   // push work items built from L_i (written by the item builder)
 };
 
 struct HopArgs {
   uint32_t V;
-  uint32_t S;      // total source rows of the query (valid-bit mask)
+  uint32_t S;      // rows of the tile set (valid-bit mask when live == nullptr)
   uint32_t word0;  // global word index of this tile's word 0
   uint32_t pad;
   const uint64_t *poff;  // pull CSR: edges arriving at v
@@ -59,6 +64,9 @@ struct HopArgs {
   const ull *items;      // push items built from actbm
   const ull *nitems;
   HopCounters *ctr;      // counters of L_i
+  const uint64_t *live;      // per-word rows expanding now, or nullptr (rows < S)
+  const uint64_t *live_next; // per-word rows expanding next hop, or nullptr
+  const uint64_t *racc;      // per-word rows accumulating into R, or nullptr (all)
 };
 
 struct LocalCtr {
@@ -78,21 +86,39 @@ __device__ __forceinline__ void flush_ctr(LocalCtr &c, HopCounters *ctr) {
     if (visnew) atomicAdd(&ctr->visnew, static_cast<ull>(visnew));
     if (rnew) atomicAdd(&ctr->rnew, static_cast<ull>(rnew));
   }
+  c = LocalCtr{};
+}
+
+// Lane masks for word j of the tile.
+struct LaneMasks {
+  uint64_t live, live_next, racc;
+};
+
+__device__ __forceinline__ LaneMasks lane_masks(const HopArgs &a, int j) {
+  LaneMasks m;
+  const uint64_t v = valid_mask(a.word0 + j, a.S);
+  m.live = a.live ? a.live[j] : v;
+  m.live_next = a.live_next ? a.live_next[j] : v;
+  m.racc = a.racc ? a.racc[j] : ~0ull;
+  return m;
 }
 
 // Writes one new word nb (nonzero) of L_i at index wi and updates Vis / R.
 template <bool SHORTEST>
 __device__ __forceinline__ void emit_word(const HopArgs &a, size_t wi, uint64_t nb,
-                                          uint64_t vis, LocalCtr &c) {
+                                          uint64_t vis, uint64_t racc, LocalCtr &c) {
   a.Fn[wi] = nb;
   if (SHORTEST) {
     a.Vis[wi] = vis | nb;
     c.visnew += (vis == 0);
   }
   if (a.R) {
-    const uint64_t r = a.R[wi];
-    if ((r | nb) != r) a.R[wi] = r | nb;
-    c.rnew += (r == 0);
+    const uint64_t add = nb & racc;
+    if (add) {
+      const uint64_t r = a.R[wi];
+      if ((r | add) != r) a.R[wi] = r | add;
+      c.rnew += (r == 0);
+    }
   }
   c.pop += __popcll(nb);
   c.nz += 1;
@@ -105,41 +131,39 @@ __device__ __forceinline__ void emit_word(const HopArgs &a, size_t wi, uint64_t 
 // 64 at a time, filtered by the frontier's active bitmap, compacted into LDS
 // with ballot + mbcnt, and gathered with TW-word coalesced loads.
 template <int TW, bool SHORTEST>
-__global__ __launch_bounds__(kBlock) void k_pull(HopArgs a) {
+__device__ __forceinline__ void pull_block(const HopArgs &a, uint32_t vb, uint32_t *nbr,
+                                           const LaneMasks &mk, LocalCtr &c) {
   constexpr int G = 64 / TW;
-  __shared__ uint32_t s_nbr[kWavesPerBlock][64];
-  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const int lane = lane_id();
   const int g = lane / TW, j = lane % TW;
-  const uint64_t valid = valid_mask(a.word0 + j, a.S);
-  const uint32_t nvb = (a.V + 63) / 64;
-  const uint32_t nwaves = gridDim.x * kWavesPerBlock;
   const bool same_csr = a.xoff == a.poff;
-  uint32_t *nbr = s_nbr[wid];
-  LocalCtr c;
-  for (uint32_t vb = blockIdx.x * kWavesPerBlock + wid; vb < nvb; vb += nwaves) {
-    uint64_t actmask = 0;
-    const uint32_t vend = min(64u, a.V - vb * 64);
-    for (uint32_t i = 0; i < vend; ++i) {
-      const uint32_t v = vb * 64 + i;
-      const size_t wi = static_cast<size_t>(v) * TW + j;
-      uint64_t vis = 0;
-      if (SHORTEST) vis = a.Vis[wi];
-      const uint64_t need = valid & ~vis;
-      if (__ballot(need != 0) == 0) continue;
-      const uint64_t b = a.poff[v], e = a.poff[v + 1];
-      uint64_t acc = 0;
-      for (uint64_t base = b; base < e; base += 64) {
-        const uint64_t idx = base + lane;
-        uint32_t u = 0;
-        bool act = false;
-        if (idx < e) {
-          u = a.padj[idx];
-          act = (a.actbm[u >> 6] >> (u & 63)) & 1;
-        }
-        const uint64_t m = __ballot(act);
-        const uint32_t n = __popcll(m);
-        if (act) nbr[lane_rank(m)] = u;
-        wave_sync();
+  uint64_t actmask = 0;
+  const uint32_t vend = min(64u, a.V - vb * 64);
+  for (uint32_t i = 0; i < vend; ++i) {
+    const uint32_t v = vb * 64 + i;
+    const size_t wi = static_cast<size_t>(v) * TW + j;
+    uint64_t vis = 0;
+    if (SHORTEST) vis = a.Vis[wi];
+    const uint64_t need = mk.live & ~vis;
+    if (__ballot(need != 0) == 0) continue;
+    const uint64_t b = a.poff[v], e = a.poff[v + 1];
+    uint64_t acc = 0;
+    // lanes whose word already has every needed row stop loading (their
+    // share of the row is then not fetched)
+    bool todo = need != 0;
+    for (uint64_t base = b; base < e; base += 64) {
+      const uint64_t idx = base + lane;
+      uint32_t u = 0;
+      bool act = false;
+      if (idx < e) {
+        u = a.padj[idx];
+        act = (a.actbm[u >> 6] >> (u & 63)) & 1;
+      }
+      const uint64_t m = __ballot(act);
+      const uint32_t n = __popcll(m);
+      if (act) nbr[lane_rank(m)] = u;
+      wave_sync();
+      if (todo) {
         uint32_t k = g;
         for (; k + 3 * G < n; k += 4 * G) {
           const uint32_t u0 = nbr[k], u1 = nbr[k + G], u2 = nbr[k + 2 * G],
@@ -151,138 +175,226 @@ __global__ __launch_bounds__(kBlock) void k_pull(HopArgs a) {
           acc |= (x0 | x1) | (x2 | x3);
         }
         for (; k < n; k += G) acc |= a.F[static_cast<size_t>(nbr[k]) * TW + j];
-        wave_sync();
-        acc = group_or<TW>(acc);
-        if (__ballot((acc & need) != need) == 0) break;  // nothing left to find
       }
+      wave_sync();
       acc = group_or<TW>(acc);
-      const uint64_t nb = acc & need;
-      if (g == 0 && nb) emit_word<SHORTEST>(a, wi, nb, vis, c);
-      if (__ballot(g == 0 && nb != 0)) {
-        actmask |= 1ull << i;
-        const uint64_t deg = same_csr ? e - b : a.xoff[v + 1] - a.xoff[v];
-        if (lane == 0) {
-          c.act += 1;
-          c.actdeg += deg;
-        }
-        if (g == 0) c.te += deg * __popcll(nb);
-      }
+      todo = (acc & need) != need;
+      if (__ballot(todo) == 0) break;  // nothing left to find
     }
-    if (lane == 0) a.actbm_n[vb] = actmask;
+    acc = group_or<TW>(acc);
+    const uint64_t nb = acc & need;
+    if (g == 0 && nb) emit_word<SHORTEST>(a, wi, nb, vis, mk.racc, c);
+    if (__ballot(g == 0 && nb != 0)) {
+      actmask |= 1ull << i;
+      const uint64_t deg = same_csr ? e - b : a.xoff[v + 1] - a.xoff[v];
+      if (lane == 0) {
+        c.act += 1;
+        c.actdeg += deg;
+      }
+      if (g == 0) c.te += deg * __popcll(nb & mk.live_next);
+    }
   }
-  flush_ctr(c, a.ctr);
+  if (lane == 0) a.actbm_n[vb] = actmask;
 }
 
 // ------------------------------------------------------------------- push
-// One wave per work item (vertex, chunk of kPushChunk out-neighbours).
+// One work item = (vertex, chunk of kPushChunk out-neighbours).
 template <int TW>
-__global__ __launch_bounds__(kBlock) void k_push(HopArgs a) {
+__device__ __forceinline__ void push_item(const HopArgs &a, ull item) {
   constexpr int G = 64 / TW;
-  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const int lane = lane_id();
   const int g = lane / TW, j = lane % TW;
-  const uint64_t nitems = *a.nitems;
-  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
-  for (uint64_t it = blockIdx.x * kWavesPerBlock + wid; it < nitems; it += nwaves) {
-    const ull item = a.items[it];
-    const uint32_t u = static_cast<uint32_t>(item);
-    const uint32_t ch = static_cast<uint32_t>(item >> 32);
-    const uint64_t ue = a.xoff[u + 1];
-    const uint64_t b = a.xoff[u] + static_cast<uint64_t>(ch) * kPushChunk;
-    const uint64_t e = min(b + kPushChunk, ue);
-    const uint64_t f = a.F[static_cast<size_t>(u) * TW + j];
-    for (uint64_t base = b; base < e; base += 64) {
-      const uint64_t idx = base + lane;
-      const uint32_t vv = idx < e ? a.xadj[idx] : 0u;
-      const uint32_t n = static_cast<uint32_t>(min<uint64_t>(64, e - base));
-      for (uint32_t k0 = 0; k0 < n; k0 += G) {
-        const uint32_t k = k0 + g;
-        const uint32_t v = __shfl(vv, k < n ? static_cast<int>(k) : 0);
-        if (k < n) {
-          if (f) atomicOr(reinterpret_cast<ull *>(&a.N[static_cast<size_t>(v) * TW + j]),
-                          static_cast<ull>(f));
-          if (j == 0)
-            atomicOr(reinterpret_cast<ull *>(&a.touched[v >> 6]), 1ull << (v & 63));
-        }
+  const uint32_t u = static_cast<uint32_t>(item);
+  const uint32_t ch = static_cast<uint32_t>(item >> 32);
+  const uint64_t ue = a.xoff[u + 1];
+  const uint64_t b = a.xoff[u] + static_cast<uint64_t>(ch) * kPushChunk;
+  const uint64_t e = min(b + kPushChunk, ue);
+  const uint64_t f = a.F[static_cast<size_t>(u) * TW + j];
+  for (uint64_t base = b; base < e; base += 64) {
+    const uint64_t idx = base + lane;
+    const uint32_t vv = idx < e ? a.xadj[idx] : 0u;
+    const uint32_t n = static_cast<uint32_t>(min<uint64_t>(64, e - base));
+    for (uint32_t k0 = 0; k0 < n; k0 += G) {
+      const uint32_t k = k0 + g;
+      const uint32_t v = __shfl(vv, k < n ? static_cast<int>(k) : 0);
+      if (k < n) {
+        if (f) atomicOr(reinterpret_cast<ull *>(&a.N[static_cast<size_t>(v) * TW + j]),
+                        static_cast<ull>(f));
+        if (j == 0)
+          atomicOr(reinterpret_cast<ull *>(&a.touched[v >> 6]), 1ull << (v & 63));
       }
     }
   }
 }
 
-// Push epilogue: for every touched vertex, consume N (re-zeroing it), apply
+// Push epilogue for one touched-bitmap word: consume N (re-zeroing it), apply
 // dedup / visit / union, write L_i and its active bitmap word.
 template <int TW, bool SHORTEST>
-__global__ __launch_bounds__(kBlock) void k_push_epi(HopArgs a) {
+__device__ __forceinline__ void epi_block(const HopArgs &a, uint32_t vb,
+                                          const LaneMasks &mk, LocalCtr &c) {
   constexpr int G = 64 / TW;
-  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const int lane = lane_id();
   const int g = lane / TW, j = lane % TW;
-  const uint64_t valid = valid_mask(a.word0 + j, a.S);
+  uint64_t m = a.touched[vb];
+  if (!m) return;
+  uint64_t actmask = 0;
+  while (m) {
+    const int i = nth_set_bit(m, g);
+    m = drop_low_bits(m, G);
+    uint64_t nb = 0;
+    uint32_t v = 0;
+    if (i >= 0) {
+      v = vb * 64 + i;
+      const size_t wi = static_cast<size_t>(v) * TW + j;
+      const uint64_t n = a.N[wi];
+      if (n) a.N[wi] = 0;
+      uint64_t vis = 0;
+      if (SHORTEST) vis = a.Vis[wi];
+      nb = n & mk.live & ~vis;
+      if (nb) emit_word<SHORTEST>(a, wi, nb, vis, mk.racc, c);
+    }
+    const uint64_t gb = group_bits<TW>(__ballot(nb != 0), g);
+    if (i >= 0 && gb) {
+      const uint64_t deg = a.xoff[v + 1] - a.xoff[v];
+      if (j == 0) {
+        actmask |= 1ull << i;
+        c.act += 1;
+        c.actdeg += deg;
+      }
+      c.te += deg * __popcll(nb & mk.live_next);
+    }
+  }
+  actmask = wave_or(actmask);
+  if (lane == 0) {
+    a.actbm_n[vb] = actmask;
+    a.touched[vb] = 0;
+  }
+}
+
+// Zero the active rows of a frontier buffer and its bitmap word vb.
+template <int TW>
+__device__ __forceinline__ void clear_block(uint64_t *F, uint64_t *actbm, uint32_t vb) {
+  constexpr int G = 64 / TW;
+  const int lane = lane_id();
+  const int g = lane / TW, j = lane % TW;
+  uint64_t m = actbm[vb];
+  if (!m) return;
+  while (m) {
+    const int i = nth_set_bit(m, g);
+    m = drop_low_bits(m, G);
+    if (i >= 0) F[(static_cast<size_t>(vb) * 64 + i) * TW + j] = 0;
+  }
+  if (lane == 0) actbm[vb] = 0;
+}
+
+// Item builder for one 256-word slice of an active bitmap: per-thread counts
+// of kPushChunk-neighbour chunks, block-level exclusive scan, one atomicAdd
+// per block.  Must be called by the whole block (uses __syncthreads).
+__device__ __forceinline__ ull shfl_up64(ull x, int d) {
+  const uint32_t lo = __shfl_up(static_cast<uint32_t>(x), d);
+  const uint32_t hi = __shfl_up(static_cast<uint32_t>(x >> 32), d);
+  return (static_cast<ull>(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ void items_slice(const uint64_t *actbm, uint32_t nvb,
+                                            uint32_t base, const uint64_t *xoff,
+                                            ull *items, ull *nitems, ull *s_wsum,
+                                            ull *s_base) {
+  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const uint32_t wb = base + threadIdx.x;
+  const uint64_t m = wb < nvb ? actbm[wb] : 0;
+  ull cnt = 0;
+  for (uint64_t mm = m; mm; mm &= mm - 1) {
+    const uint32_t v = wb * 64 + (__ffsll(static_cast<long long>(mm)) - 1);
+    const uint64_t d = xoff[v + 1] - xoff[v];
+    cnt += (d + kPushChunk - 1) / kPushChunk;
+  }
+  ull x = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const ull y = shfl_up64(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_wsum[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ull tot = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) {
+      const ull t = s_wsum[w];
+      s_wsum[w] = tot;
+      tot += t;
+    }
+    *s_base = tot ? atomicAdd(nitems, tot) : 0;
+  }
+  __syncthreads();
+  ull pos = *s_base + s_wsum[wid] + x - cnt;
+  for (uint64_t mm = m; mm; mm &= mm - 1) {
+    const uint32_t v = wb * 64 + (__ffsll(static_cast<long long>(mm)) - 1);
+    const uint64_t d = xoff[v + 1] - xoff[v];
+    const uint64_t nch = (d + kPushChunk - 1) / kPushChunk;
+    for (uint64_t q = 0; q < nch; ++q) items[pos++] = v | (q << 32);
+  }
+  __syncthreads();
+}
+
+// --------------------------------------------------------- plain kernels
+template <int TW, bool SHORTEST>
+__global__ __launch_bounds__(kBlock) void k_pull(HopArgs a) {
+  __shared__ uint32_t s_nbr[kWavesPerBlock][64];
+  const int wid = threadIdx.x >> 6;
+  const LaneMasks mk = lane_masks(a, lane_id() % TW);
   const uint32_t nvb = (a.V + 63) / 64;
   const uint32_t nwaves = gridDim.x * kWavesPerBlock;
   LocalCtr c;
-  for (uint32_t vb = blockIdx.x * kWavesPerBlock + wid; vb < nvb; vb += nwaves) {
-    uint64_t m = a.touched[vb];
-    if (!m) continue;
-    uint64_t actmask = 0;
-    while (m) {
-      const int i = nth_set_bit(m, g);
-      m = drop_low_bits(m, G);
-      uint64_t nb = 0;
-      uint32_t v = 0;
-      if (i >= 0) {
-        v = vb * 64 + i;
-        const size_t wi = static_cast<size_t>(v) * TW + j;
-        const uint64_t n = a.N[wi];
-        if (n) a.N[wi] = 0;
-        uint64_t vis = 0;
-        if (SHORTEST) vis = a.Vis[wi];
-        nb = n & valid & ~vis;
-        if (nb) emit_word<SHORTEST>(a, wi, nb, vis, c);
-      }
-      const uint64_t gb = group_bits<TW>(__ballot(nb != 0), g);
-      if (i >= 0 && gb) {
-        const uint64_t deg = a.xoff[v + 1] - a.xoff[v];
-        if (j == 0) {
-          actmask |= 1ull << i;
-          c.act += 1;
-          c.actdeg += deg;
-        }
-        c.te += deg * __popcll(nb);
-      }
-    }
-    actmask = wave_or(actmask);
-    if (lane == 0) {
-      a.actbm_n[vb] = actmask;
-      a.touched[vb] = 0;
-    }
-  }
+  for (uint32_t vb = blockIdx.x * kWavesPerBlock + wid; vb < nvb; vb += nwaves)
+    pull_block<TW, SHORTEST>(a, vb, s_nbr[wid], mk, c);
   flush_ctr(c, a.ctr);
 }
 
-// Zero the active rows of a frontier buffer and its bitmap.
+template <int TW>
+__global__ __launch_bounds__(kBlock) void k_push(HopArgs a) {
+  const int wid = threadIdx.x >> 6;
+  const uint64_t nitems = *a.nitems;
+  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
+  for (uint64_t it = blockIdx.x * kWavesPerBlock + wid; it < nitems; it += nwaves)
+    push_item<TW>(a, a.items[it]);
+}
+
+template <int TW, bool SHORTEST>
+__global__ __launch_bounds__(kBlock) void k_push_epi(HopArgs a) {
+  const int wid = threadIdx.x >> 6;
+  const LaneMasks mk = lane_masks(a, lane_id() % TW);
+  const uint32_t nvb = (a.V + 63) / 64;
+  const uint32_t nwaves = gridDim.x * kWavesPerBlock;
+  LocalCtr c;
+  for (uint32_t vb = blockIdx.x * kWavesPerBlock + wid; vb < nvb; vb += nwaves)
+    epi_block<TW, SHORTEST>(a, vb, mk, c);
+  flush_ctr(c, a.ctr);
+}
+
 template <int TW>
 __global__ __launch_bounds__(kBlock) void k_clear(uint64_t *F, uint64_t *actbm,
                                                  uint32_t V) {
-  constexpr int G = 64 / TW;
-  const int lane = lane_id(), wid = threadIdx.x >> 6;
-  const int g = lane / TW, j = lane % TW;
+  const int wid = threadIdx.x >> 6;
   const uint32_t nvb = (V + 63) / 64;
   const uint32_t nwaves = gridDim.x * kWavesPerBlock;
-  for (uint32_t vb = blockIdx.x * kWavesPerBlock + wid; vb < nvb; vb += nwaves) {
-    uint64_t m = actbm[vb];
-    if (!m) continue;
-    while (m) {
-      const int i = nth_set_bit(m, g);
-      m = drop_low_bits(m, G);
-      if (i >= 0) F[(static_cast<size_t>(vb) * 64 + i) * TW + j] = 0;
-    }
-    if (lane == 0) actbm[vb] = 0;
-  }
+  for (uint32_t vb = blockIdx.x * kWavesPerBlock + wid; vb < nvb; vb += nwaves)
+    clear_block<TW>(F, actbm, vb);
 }
 
-// Active bitmap -> push work items (vertex | chunk << 32); one thread per
-// bitmap word, block-level exclusive scan, one atomicAdd per block.
 __global__ __launch_bounds__(kBlock) void k_items(const uint64_t *actbm, uint32_t V,
                                                  const uint64_t *xoff, ull *items,
                                                  ull *nitems);
+
+// Launchers (vs_vexpand.hip).
+void launch_pull(uint32_t tw, bool shortest, uint32_t grid, hipStream_t s,
+                 const HopArgs &a);
+void launch_push(uint32_t tw, bool shortest, uint32_t grid, hipStream_t s,
+                 const HopArgs &a);
+void launch_clear(uint32_t tw, uint32_t grid, hipStream_t s, uint64_t *F,
+                  uint64_t *bm, uint32_t V);
+void launch_items(uint32_t grid, hipStream_t s, const uint64_t *bm, uint32_t V,
+                  const uint64_t *xoff, ull *items, ull *nitems);
+uint32_t choose_tile_words(uint32_t requested, uint32_t W);
 
 } // namespace vs

@@ -186,6 +186,8 @@ static vs_status sched_predict_dev(vs_sched_s *s, const float *d_x, uint32_t n,
   return VS_OK;
 }
 
+const float *sched_params(vs_sched_s *s) { return s ? s->d_params : nullptr; }
+
 int sched_decide(vs_sched_s *s, const SchedFeatures &f, const vs_tuning &t) {
   const int fallback = heuristic_decide(f, t);
   if (!s) return fallback;

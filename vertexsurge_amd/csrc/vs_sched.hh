@@ -38,6 +38,8 @@ inline int heuristic_decide(const SchedFeatures &f, const vs_tuning &t) {
 void sched_features(const SchedFeatures &f, float *x);
 // 1 = push, 2 = pull; falls back to the heuristic if the MLP is unusable.
 int sched_decide(vs_sched_s *s, const SchedFeatures &f, const vs_tuning &t);
+// Device pointer of the scheduler's parameters (for in-graph inference).
+const float *sched_params(vs_sched_s *s);
 
 // MLP parameter layout (floats): W1[H][IN], b1[H], W2[H][H], b2[H],
 // W3[OUT][H], b3[OUT].

@@ -21,55 +21,14 @@
 
 namespace vs {
 
-__device__ __forceinline__ ull shfl_up64(ull x, int d) {
-  const uint32_t lo = __shfl_up(static_cast<uint32_t>(x), d);
-  const uint32_t hi = __shfl_up(static_cast<uint32_t>(x >> 32), d);
-  return (static_cast<ull>(hi) << 32) | lo;
-}
-
 __global__ __launch_bounds__(kBlock) void k_items(const uint64_t *actbm, uint32_t V,
                                                  const uint64_t *xoff, ull *items,
                                                  ull *nitems) {
   __shared__ ull s_wsum[kWavesPerBlock];
   __shared__ ull s_base;
   const uint32_t nvb = (V + 63) / 64;
-  const int lane = lane_id(), wid = threadIdx.x >> 6;
-  for (uint32_t base = blockIdx.x * kBlock; base < nvb; base += gridDim.x * kBlock) {
-    const uint32_t wb = base + threadIdx.x;
-    const uint64_t m = wb < nvb ? actbm[wb] : 0;
-    ull cnt = 0;
-    for (uint64_t mm = m; mm; mm &= mm - 1) {
-      const uint32_t v = wb * 64 + (__ffsll(static_cast<long long>(mm)) - 1);
-      const uint64_t d = xoff[v + 1] - xoff[v];
-      cnt += (d + kPushChunk - 1) / kPushChunk;
-    }
-    ull x = cnt;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const ull y = shfl_up64(x, o);
-      if (lane >= o) x += y;
-    }
-    if (lane == 63) s_wsum[wid] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      ull tot = 0;
-      for (int w = 0; w < kWavesPerBlock; ++w) {
-        const ull t = s_wsum[w];
-        s_wsum[w] = tot;
-        tot += t;
-      }
-      s_base = tot ? atomicAdd(nitems, tot) : 0;
-    }
-    __syncthreads();
-    ull pos = s_base + s_wsum[wid] + x - cnt;
-    for (uint64_t mm = m; mm; mm &= mm - 1) {
-      const uint32_t v = wb * 64 + (__ffsll(static_cast<long long>(mm)) - 1);
-      const uint64_t d = xoff[v + 1] - xoff[v];
-      const uint64_t nch = (d + kPushChunk - 1) / kPushChunk;
-      for (uint64_t c = 0; c < nch; ++c) items[pos++] = v | (c << 32);
-    }
-    __syncthreads();
-  }
+  for (uint32_t base = blockIdx.x * kBlock; base < nvb; base += gridDim.x * kBlock)
+    items_slice(actbm, nvb, base, xoff, items, nitems, s_wsum, &s_base);
 }
 
 // L0: row r has only its source vertex (ExpandStatus::init_first_m_layer,
@@ -488,6 +447,9 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
       a.Vis = Vis ? Vis + t * tile_words : nullptr;
       a.R = acc_r ? R + t * tile_words : nullptr;
       a.ctr = d_ctr + static_cast<size_t>(hop) * T + t;
+      a.live = nullptr;
+      a.live_next = nullptr;
+      a.racc = nullptr;
       hipEvent_t k0 = timing ? evs.make() : nullptr, k1 = timing ? evs.make() : nullptr;
       if (mode == 1) {
         VS_TRY(ensure_push());
@@ -534,13 +496,15 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
     VS_HIP(hipStreamSynchronize(s));
     k_actual = hop;
     uint64_t newbits = 0, prev_vis = vis_nz, prev_r = r_nz, te = 0, bytes = 0;
-    uint64_t frontier_bits = 0;
+    uint64_t frontier_bits = 0, frontier_act = 0, frontier_deg = 0;
     for (uint32_t t = 0; t < T; ++t) {
       newbits += C(hop, t).pop;
       vis_nz += C(hop, t).visnew;
       r_nz += C(hop, t).rnew;
       te += C(hop - 1, t).te;
       frontier_bits += C(hop - 1, t).pop;
+      frontier_act += C(hop - 1, t).act;
+      frontier_deg += C(hop - 1, t).actdeg;
     }
     const bool any_union = !shortest && acc_r;
     for (uint32_t t = 0; t < T; ++t) {
@@ -562,6 +526,11 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
       st->bytes_hop[hop - 1] = bytes;
       st->bytes_alg += bytes;
       st->frontier_bits[hop - 1] = frontier_bits;
+      st->frontier_act[hop - 1] = frontier_act;
+      st->frontier_deg[hop - 1] = frontier_deg;
+      st->vis_frac[hop - 1] = shortest ? static_cast<double>(prev_vis) /
+                                             (static_cast<double>(T) * tile_words)
+                                       : 0.0;
       st->new_bits[hop - 1] = newbits;
       int m = 0;
       for (uint32_t t = 0; t < T; ++t) if (tile_mode[t]) m = (m == 0 || m == tile_mode[t]) ? tile_mode[t] : 3;

@@ -1,0 +1,339 @@
+// Host-side C++ operators over the C ABI: the VExpand and Mintersect
+// interfaces of VertexSurge (src/include/operator/big_operators.hh:40-183)
+// with their parameter lists, output table kinds and error behaviour, running
+// on the MI355X through libvertexsurge.so.
+//
+// The reference resolves the edge relation from a global schema by
+// "<src>_<edge>_<dst>" (vexpand.cpp:57-60); GraphStore keeps the same lookup:
+// register each edge relation's CSR under that name once, and operators find
+// it by their type names.  Start vertices come from the dependency's flat
+// column in the reference (vexpand.cpp:168-176); here run() takes that column.
+// Errors become std::runtime_error (the reference asserts or exits).
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/vertexsurge.h"
+
+namespace vertexsurge {
+
+using VID = uint32_t;
+using TypeName = std::string;
+using VariableName = std::string;
+
+// src/include/operator/expand.hh:25-29
+enum class ExpandPathType { Any, Shortest, FindShortest };
+// src/include/common/common.hh:154-159
+enum class Direction { Forward, Backward, Both, Undecide };
+// src/include/operator/expand.hh:51-55
+enum class ExpandStrategy { Adaptive, CSR, Dense };
+
+// src/include/operator/expand.hh:31-49
+struct VExpandPathParameters {
+  ExpandPathType path_type = ExpandPathType::Shortest;
+  static const size_t K_MAX_FIND_SHORTEST = 50;
+  bool with_distance = false;
+  bool distinct_target = false;
+  static VExpandPathParameters Any() { return {ExpandPathType::Any}; }
+  static VExpandPathParameters Shortest() { return {ExpandPathType::Shortest}; }
+  static VExpandPathParameters FindShortest() {
+    VExpandPathParameters p{ExpandPathType::FindShortest};
+    p.with_distance = true;
+    return p;
+  }
+  VExpandPathParameters set_with_distance() {
+    with_distance = true;
+    return *this;
+  }
+};
+
+inline void check(vs_status s, const char *what) {
+  if (s != VS_OK)
+    throw std::runtime_error(std::string(what) + ": " + vs_last_error());
+}
+
+// One edge relation resident in HBM.
+class EdgeRelation {
+ public:
+  EdgeRelation(const uint64_t *offsets, const uint32_t *adj, uint32_t num_vertices,
+               uint64_t num_edges, bool symmetric) {
+    check(vs_graph_create(offsets, adj, num_vertices, num_edges,
+                          symmetric ? VS_GRAPH_SYMMETRIC : 0u, &g_),
+          "vs_graph_create");
+  }
+  ~EdgeRelation() { vs_graph_destroy(g_); }
+  EdgeRelation(const EdgeRelation &) = delete;
+  EdgeRelation &operator=(const EdgeRelation &) = delete;
+  vs_graph_t handle() const { return g_; }
+  uint32_t vertex_count() const {
+    uint32_t v = 0;
+    check(vs_graph_info(g_, &v, nullptr, nullptr), "vs_graph_info");
+    return v;
+  }
+
+ private:
+  vs_graph_t g_ = nullptr;
+};
+
+// The global store of edge relations (replaces PhysicalOperator::graph_store).
+class GraphStore {
+ public:
+  static GraphStore &instance() {
+    static GraphStore s;
+    return s;
+  }
+  void add_edge(const TypeName &src, const TypeName &edge, const TypeName &dst,
+                std::shared_ptr<EdgeRelation> rel) {
+    rels_[key(src, edge, dst)] = std::move(rel);
+  }
+  std::shared_ptr<EdgeRelation> get_edge(const TypeName &src, const TypeName &edge,
+                                         const TypeName &dst) const {
+    auto it = rels_.find(key(src, edge, dst));
+    if (it == rels_.end())
+      throw std::runtime_error("no edge schema " + key(src, edge, dst));  // schema.cpp:217
+    return it->second;
+  }
+  void clear() { rels_.clear(); }
+
+ private:
+  static std::string key(const TypeName &s, const TypeName &e, const TypeName &d) {
+    return s + "_" + e + "_" + d;
+  }
+  std::map<std::string, std::shared_ptr<EdgeRelation>> rels_;
+};
+
+// VarRowMatrixTable: rows = start vertices (row_map), columns = vertices;
+// the bit matrix stays in HBM (matrix_table.hh:44-83).
+class VarRowMatrixTable {
+ public:
+  VarRowMatrixTable(std::shared_ptr<vs_result_s> res, uint32_t index,
+                    std::vector<VID> row_map)
+      : res_(std::move(res)), index_(index), row_map_(std::move(row_map)) {}
+  const std::vector<VID> &get_row_map() const { return row_map_; }
+  // bit_matrix_table.cpp:112
+  size_t tuple_count() const {
+    uint64_t c = 0;
+    check(vs_result_count(res_.get(), index_, &c), "vs_result_count");
+    return c;
+  }
+  // DenseVarStackMatrix::get (bitmap.cpp:953-956)
+  bool get(uint32_t r, uint32_t c) const {
+    int b = 0;
+    check(vs_result_get(res_.get(), index_, r, c, &b), "vs_result_get");
+    return b != 0;
+  }
+  std::vector<uint64_t> row_counts() const {
+    std::vector<uint64_t> out(row_map_.size());
+    if (!out.empty())
+      check(vs_result_row_counts(res_.get(), index_, out.data()), "vs_result_row_counts");
+    return out;
+  }
+  std::vector<uint64_t> export_words(vs_layout layout) const {
+    size_t n = 0;
+    check(vs_result_export_size(res_.get(), index_, layout, &n), "vs_result_export_size");
+    std::vector<uint64_t> out(n / 8);
+    if (n) check(vs_result_export(res_.get(), index_, layout, out.data(), n), "vs_result_export");
+    return out;
+  }
+  // (start, target) pairs in the reference iterator's order for one stack:
+  // row-major here (row ascending, then column ascending).
+  std::vector<std::pair<VID, VID>> tuples() const {
+    const auto rm = export_words(VS_LAYOUT_ROWMAJOR);
+    std::vector<std::pair<VID, VID>> out;
+    if (row_map_.empty()) return out;
+    const size_t nw = rm.size() / row_map_.size();
+    for (size_t r = 0; r < row_map_.size(); ++r)
+      for (size_t w = 0; w < nw; ++w)
+        for (uint64_t m = rm[r * nw + w]; m; m &= m - 1)
+          out.emplace_back(row_map_[r], static_cast<VID>(w * 64 + __builtin_ctzll(m)));
+    return out;
+  }
+
+ private:
+  std::shared_ptr<vs_result_s> res_;
+  uint32_t index_;
+  std::vector<VID> row_map_;
+};
+
+// MultipleMatrixTable: one matrix per distance (bit_matrix_table.cpp:291-340).
+struct MultipleMatrixTable {
+  std::vector<std::pair<size_t, VarRowMatrixTable>> mats;
+};
+
+class VExpand {
+ public:
+  // big_operators.hh:101-105 / vexpand.cpp:7-14
+  VExpand(TypeName src_type, TypeName dst_type, TypeName edge_type, Direction dir,
+          size_t k_min, size_t k_max, ExpandPathType path_type, ExpandStrategy strategy,
+          VariableName variable_name, std::optional<VariableName> target_variable_name)
+      : VExpand(src_type, dst_type, edge_type, dir, k_min, k_max,
+                VExpandPathParameters{path_type}, strategy, variable_name,
+                target_variable_name) {}
+
+  // For FindShortest, big_operators.hh:107-111 / vexpand.cpp:16-25
+  VExpand(TypeName src_type, TypeName dst_type, TypeName edge_type, Direction dir,
+          VExpandPathParameters path_params, ExpandStrategy strategy,
+          VariableName variable_name, VariableName target_variable_name)
+      : VExpand(src_type, dst_type, edge_type, dir, 1,
+                VExpandPathParameters::K_MAX_FIND_SHORTEST, path_params, strategy,
+                variable_name, std::optional<VariableName>(target_variable_name)) {
+    if (path_params.path_type != ExpandPathType::FindShortest || !path_params.with_distance)
+      throw std::runtime_error("FindShortest constructor needs FindShortest + with_distance");
+  }
+
+  // big_operators.hh:113-118 / vexpand.cpp:27-34
+  VExpand(TypeName src_type, TypeName dst_type, TypeName edge_type, Direction dir,
+          size_t k_min, size_t k_max, VExpandPathParameters path_params,
+          ExpandStrategy strategy, VariableName variable_name,
+          std::optional<VariableName> target_variable_name)
+      : k_min_(k_min), k_max_(k_max), path_params_(path_params), strategy_(strategy),
+        start_variable_name_(std::move(variable_name)),
+        target_variable_name_(std::move(target_variable_name)) {
+    rel_ = GraphStore::instance().get_edge(src_type, edge_type, dst_type);
+    if (dir == Direction::Undecide) dir = Direction::Forward;
+    dir_ = dir;
+    if (strategy != ExpandStrategy::Dense && strategy != ExpandStrategy::Adaptive)
+      throw std::runtime_error("only the Dense (bit-matrix) strategy runs on the GPU");
+    if (k_max >= VS_MAX_HOPS) throw std::runtime_error("k_max too large");
+  }
+
+  // Runs with the dependency's start column (and target column for
+  // FindShortest).  An empty start column gives an empty result, as the
+  // reference's early exit (vexpand.cpp:170-173).
+  void run(const std::vector<VID> &starts, const std::vector<VID> *targets = nullptr) {
+    starts_ = starts;
+    if (path_params_.path_type == ExpandPathType::FindShortest) {
+      if (!targets || targets->size() != starts.size())
+        throw std::runtime_error("FindShortest needs one target per start vertex");
+      distance_.assign(starts.size(), -1);
+      if (!starts.empty())
+        check(vs_find_shortest(rel_->handle(), starts.data(), targets->data(),
+                               static_cast<uint32_t>(starts.size()), dir_code(),
+                               static_cast<uint32_t>(k_max_), distance_.data(), &stats_),
+              "vs_find_shortest");
+      k_actual_max_ = stats_.k_actual_max;
+      return;
+    }
+    vs_vexpand_params p{};
+    p.k_min = static_cast<uint32_t>(k_min_);
+    p.k_max = static_cast<uint32_t>(k_max_);
+    p.path_type = path_params_.path_type == ExpandPathType::Any ? VS_PATH_ANY : VS_PATH_SHORTEST;
+    p.direction = dir_code();
+    p.with_distance = path_params_.with_distance ? 1 : 0;
+    p.sched = VS_SCHED_HEURISTIC;
+    vs_result_t r = nullptr;
+    check(vs_vexpand(rel_->handle(), starts.data(), static_cast<uint32_t>(starts.size()), &p,
+                     &r, &stats_),
+          "vs_vexpand");
+    res_ = std::shared_ptr<vs_result_s>(r, [](vs_result_s *x) { vs_result_free(x); });
+    k_actual_max_ = stats_.k_actual_max;
+  }
+
+  size_t k_actual_max() const { return k_actual_max_; }
+  const vs_vexpand_stats &stats() const { return stats_; }
+
+  // VarRowMatrixTable result (merge_to_MT)
+  VarRowMatrixTable results() const {
+    if (!res_ || path_params_.with_distance)
+      throw std::runtime_error("no merged matrix result");
+    return VarRowMatrixTable(res_, 0, starts_);
+  }
+  // MultipleMatrixTable result (copy_to_multi_MT)
+  MultipleMatrixTable multiple_results() const {
+    if (!res_ || !path_params_.with_distance)
+      throw std::runtime_error("no per-distance result");
+    MultipleMatrixTable m;
+    uint32_t n = 0;
+    check(vs_result_info(res_.get(), nullptr, nullptr, &n, nullptr), "vs_result_info");
+    for (uint32_t i = 0; i < n; ++i) {
+      uint32_t d = 0;
+      check(vs_result_distance(res_.get(), i, &d), "vs_result_distance");
+      m.mats.emplace_back(d, VarRowMatrixTable(res_, i, starts_));
+    }
+    return m;
+  }
+  // FindShortest: the dependency's tuples extended with "distance" (-1 = none)
+  const std::vector<int32_t> &distance() const { return distance_; }
+
+ private:
+  int32_t dir_code() const {
+    return dir_ == Direction::Backward ? VS_DIR_BACKWARD
+           : dir_ == Direction::Both   ? VS_DIR_BOTH
+                                       : VS_DIR_FORWARD;
+  }
+  size_t k_min_, k_max_, k_actual_max_ = 0;
+  VExpandPathParameters path_params_;
+  ExpandStrategy strategy_;
+  Direction dir_ = Direction::Forward;
+  VariableName start_variable_name_;
+  std::optional<VariableName> target_variable_name_;
+  std::shared_ptr<EdgeRelation> rel_;
+  std::vector<VID> starts_;
+  std::shared_ptr<vs_result_s> res_;
+  std::vector<int32_t> distance_;
+  vs_vexpand_stats stats_{};
+};
+
+// Mintersect (big_operators.hh:161-183): base tuples of `arity` bound
+// variables; dependency d maps x_d to its neighbour set over the new vertex.
+class Mintersect {
+ public:
+  Mintersect(VariableName new_vertex_var_name, TypeName new_vertex_type, bool only_count)
+      : new_vertex_var_name_(std::move(new_vertex_var_name)),
+        new_vertex_type_(std::move(new_vertex_type)), only_count_(only_count) {}
+
+  // One dependency per bound base column (addDependency order).
+  void add_dependency(std::shared_ptr<EdgeRelation> rel) { deps_.push_back(std::move(rel)); }
+
+  // base: T rows of arity = deps.size() vertex ids, row-major.
+  void run(const std::vector<VID> &base) {
+    const uint32_t arity = static_cast<uint32_t>(deps_.size());
+    if (arity == 0 || base.size() % arity) throw std::runtime_error("bad base tuples");
+    const uint64_t T = base.size() / arity;
+    std::vector<vs_graph_t> hs;
+    for (auto &d : deps_) hs.push_back(d->handle());
+    counts_.assign(T, 0);
+    offsets_.assign(T + 1, 0);
+    uint64_t total = 0;
+    check(vs_mintersect(hs.data(), arity, base.data(), T, counts_.data(), offsets_.data(),
+                        nullptr, 0, &total, nullptr),
+          "vs_mintersect");
+    total_num_ = total;
+    tuples_.clear();
+    if (only_count_) return;
+    std::vector<uint32_t> lists(total);
+    if (total)
+      check(vs_mintersect(hs.data(), arity, base.data(), T, counts_.data(), offsets_.data(),
+                          lists.data(), total, &total, nullptr),
+            "vs_mintersect");
+    // base columns + new vertex, tuples in base order, new ids ascending
+    for (uint64_t t = 0; t < T; ++t)
+      for (uint64_t k = offsets_[t]; k < offsets_[t + 1]; ++k) {
+        for (uint32_t d = 0; d < arity; ++d) tuples_.push_back(base[t * arity + d]);
+        tuples_.push_back(lists[k]);
+      }
+  }
+  // only_count result: the "count" Int64 column (mintersect.cpp:336-340)
+  uint64_t count() const { return total_num_; }
+  const std::vector<VID> &flat_tuples() const { return tuples_; }
+  size_t tuple_count() const { return only_count_ ? 1 : total_num_; }
+
+ private:
+  VariableName new_vertex_var_name_;
+  TypeName new_vertex_type_;
+  bool only_count_;
+  std::vector<std::shared_ptr<EdgeRelation>> deps_;
+  std::vector<uint32_t> counts_;
+  std::vector<uint64_t> offsets_;
+  std::vector<VID> tuples_;
+  uint64_t total_num_ = 0;
+};
+
+}  // namespace vertexsurge
