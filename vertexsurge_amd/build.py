@@ -71,6 +71,10 @@ def build(verbose: bool = True) -> Path:
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
           *map(str, objs), "-lgomp", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
+    # C++ operator shims test driver (links the library just built)
+    _run([GXX, "-O2", "-std=c++17", "-Wall", "-o", str(BUILD / "test_operators"),
+          str(ROOT / "tests" / "cpp" / "test_operators.cpp"), f"-L{PKG}", "-lvertexsurge",
+          "-Wl,-rpath,$ORIGIN/.."])
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -87,6 +87,9 @@ struct BatchConst {
   ull *out_counts;        // [S] (group order), zeroed before the first wave
   ull *out_digests;       // [S] or nullptr
   const float *mlp;       // scheduler parameters or nullptr
+  const ull *pitems;      // static pull items of the pull CSR
+  uint64_t npitems;
+  uint32_t has_heavy;     // some in-list spans several pull items
 };
 
 __device__ __forceinline__ HopArgs slot_args(const BatchConst &c, const SlotDesc &sd,
@@ -115,6 +118,8 @@ __device__ __forceinline__ HopArgs slot_args(const BatchConst &c, const SlotDesc
   a.live = c.live + static_cast<size_t>(hop) * c.Wpad + ti.word0;
   a.live_next = c.live + static_cast<size_t>(hop + 1) * c.Wpad + ti.word0;
   a.racc = c.racc ? c.racc + static_cast<size_t>(hop) * c.Wpad + ti.word0 : nullptr;
+  a.pitems = c.pitems;
+  a.npitems = c.npitems;
   return a;
 }
 
@@ -263,7 +268,7 @@ __global__ void k_bw_plan(BatchConst c, uint32_t hop) {
   for (uint32_t s = 0; s < c.nslots; ++s) {
     c.ctl->unit_prefix[s] = acc;
     const SlotDesc &sd = c.slots[s];
-    if (sd.mode == 2) acc += nvb;
+    if (sd.mode == 2) acc += c.npitems;
     else if (sd.mode == 1) acc += sd.ctr[hop - 1].items;
   }
   c.ctl->unit_prefix[c.nslots] = acc;
@@ -294,7 +299,7 @@ __global__ __launch_bounds__(kBlock) void k_bw_expand(BatchConst c, uint32_t hop
       mk = lane_masks(a, lane_id() % TW);
     }
     const ull local = u - c.ctl->unit_prefix[s];
-    if (mode == 2) pull_block<TW, SHORTEST>(a, static_cast<uint32_t>(local), s_nbr[wid], mk, lc);
+    if (mode == 2) pull_item<TW, SHORTEST>(a, a.pitems[local], s_nbr[wid], mk, lc);
     else push_item<TW>(a, a.items[local]);
   }
   if (cur != UINT32_MAX && mode == 2) flush_ctr(lc, a.ctr);
@@ -313,7 +318,9 @@ __global__ __launch_bounds__(kBlock) void k_bw_epi(BatchConst c, uint32_t hop) {
   LocalCtr lc;
   for (ull u = blockIdx.x * kWavesPerBlock + wid; u < total; u += nwaves) {
     const uint32_t s = static_cast<uint32_t>(u / nvb);
-    if (c.slots[s].mode != 1) continue;
+    // push slots, and pull slots whose hubs left partial words in N
+    const int md = c.slots[s].mode;
+    if (md != 1 && !(md == 2 && c.has_heavy)) continue;
     if (s != cur) {
       if (cur != UINT32_MAX) flush_ctr(lc, a.ctr);
       cur = s;
@@ -673,6 +680,10 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   c.out_counts = static_cast<ull *>(d_counts.p);
   c.out_digests = want_digests ? static_cast<ull *>(d_dig.p) : nullptr;
   c.mlp = (p.sched == VS_SCHED_LEARNED && g->sched) ? sched_params(g->sched) : nullptr;
+  VS_TRY(ensure_pull_items(*const_cast<DevCSR *>(pull), V));
+  c.pitems = pull->pitems;
+  c.npitems = pull->npitems;
+  c.has_heavy = pull->has_heavy ? 1 : 0;
   const uint32_t grid = static_cast<uint32_t>(g->num_cus) * 8;
 
   hipEvent_t e0, e1;

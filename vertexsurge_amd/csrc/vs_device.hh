@@ -34,6 +34,10 @@ namespace vs {
 // kPushChunk neighbours in the high 32 bits.  Hubs become many items so one
 // wave never owns a whole hub list.
 constexpr uint32_t kPushChunk = 256;
+// Pull work item: destination vertex | chunk of kPullChunk in-neighbours.
+// A vertex with more in-edges (a hub) spans several items on different waves;
+// their partial ORs meet in the push accumulator and the epilogue finishes it.
+constexpr uint32_t kPullChunk = 512;
 constexpr int kWavesPerBlock = 4;
 constexpr int kBlock = 64 * kWavesPerBlock;
 
@@ -58,7 +62,15 @@ struct DevCSR {
   uint64_t E = 0;
   std::vector<uint64_t> hoff; // host copy of offsets (degrees for host stats)
   bool owned = false;
+  // static pull work items over this CSR's lists (built on first pull)
+  unsigned long long *pitems = nullptr;
+  uint64_t npitems = 0;
+  bool has_heavy = false;     // some list longer than kPullChunk
 };
+
+// Builds the pull items of a CSR once (vertex | chunk << 32 per kPullChunk
+// in-edges, vertices without edges omitted).
+vs_status ensure_pull_items(DevCSR &c, uint32_t V);
 
 } // namespace vs
 

@@ -77,7 +77,35 @@ static void free_csr(DevCSR &d) {
     (void)hipFree(d.off);
     (void)hipFree(d.adj);
   }
+  if (d.pitems) (void)hipFree(d.pitems);
   d = DevCSR{};
+}
+
+vs_status ensure_pull_items(DevCSR &c, uint32_t V) {
+  if (c.pitems) return VS_OK;
+  std::vector<unsigned long long> it;
+  it.reserve(static_cast<size_t>(V) + c.E / kPullChunk + 1);
+  bool heavy = false;
+  for (uint32_t v = 0; v < V; ++v) {
+    const uint64_t d = c.hoff[v + 1] - c.hoff[v];
+    const uint64_t nch = (d + kPullChunk - 1) / kPullChunk;
+    heavy |= nch > 1;
+    for (uint64_t q = 0; q < nch; ++q) it.push_back(v | (q << 32));
+  }
+  void *p = nullptr;
+  VS_HIP(hipMalloc(&p, std::max<size_t>(it.size(), 1) * 8));
+  if (!it.empty()) {
+    hipError_t e = hipMemcpy(p, it.data(), it.size() * 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(p);
+      set_error("pull items upload: %s", hipGetErrorString(e));
+      return VS_ERR_HIP;
+    }
+  }
+  c.pitems = static_cast<unsigned long long *>(p);
+  c.npitems = it.size();
+  c.has_heavy = heavy;
+  return VS_OK;
 }
 
 vs_status graph_csrs(vs_graph_s *g, int direction, const DevCSR **push,

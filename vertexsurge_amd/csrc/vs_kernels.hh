@@ -63,6 +63,8 @@ struct HopArgs {
   uint64_t *R;           // result accumulator or nullptr
   const ull *items;      // push items built from actbm
   const ull *nitems;
+  const ull *pitems;     // pull items (static per CSR)
+  uint64_t npitems;
   HopCounters *ctr;      // counters of L_i
   const uint64_t *live;      // per-word rows expanding now, or nullptr (rows < S)
   const uint64_t *live_next; // per-word rows expanding next hop, or nullptr
@@ -125,76 +127,85 @@ __device__ __forceinline__ void emit_word(const HopArgs &a, size_t wi, uint64_t 
 }
 
 // ------------------------------------------------------------------- pull
-// A wave owns 64 consecutive destination vertices (so it alone writes their
-// active-bitmap word) and walks them one at a time.  Lanes = 64/TW groups x
-// TW words; the groups split the in-neighbour list.  Neighbour ids are read
-// 64 at a time, filtered by the frontier's active bitmap, compacted into LDS
-// with ballot + mbcnt, and gathered with TW-word coalesced loads.
+// One work item = (destination vertex v, chunk of kPullChunk in-edges).
+// Items are strided over the waves, so the hubs that R-MAT packs into the
+// low vertex ids spread over the whole GPU.  Lanes = 64/TW groups x TW words;
+// the groups split the chunk.  Neighbour ids are read 64 at a time, filtered
+// by the frontier's active bitmap, compacted into LDS with ballot + mbcnt, and
+// gathered with TW-word coalesced loads (four in flight per lane).  A lane
+// stops loading once its word holds every needed row, the wave once all do.
+// A vertex with one item finishes in place (dedup / visit / union); a hub's
+// items OR their partial words into the push accumulator N and mark the
+// touched bitmap, and the push epilogue finishes it.
 template <int TW, bool SHORTEST>
-__device__ __forceinline__ void pull_block(const HopArgs &a, uint32_t vb, uint32_t *nbr,
-                                           const LaneMasks &mk, LocalCtr &c) {
+__device__ __forceinline__ void pull_item(const HopArgs &a, ull item, uint32_t *nbr,
+                                          const LaneMasks &mk, LocalCtr &c) {
   constexpr int G = 64 / TW;
   const int lane = lane_id();
   const int g = lane / TW, j = lane % TW;
-  const bool same_csr = a.xoff == a.poff;
-  uint64_t actmask = 0;
-  const uint32_t vend = min(64u, a.V - vb * 64);
-  for (uint32_t i = 0; i < vend; ++i) {
-    const uint32_t v = vb * 64 + i;
-    const size_t wi = static_cast<size_t>(v) * TW + j;
-    uint64_t vis = 0;
-    if (SHORTEST) vis = a.Vis[wi];
-    const uint64_t need = mk.live & ~vis;
-    if (__ballot(need != 0) == 0) continue;
-    const uint64_t b = a.poff[v], e = a.poff[v + 1];
-    uint64_t acc = 0;
-    // lanes whose word already has every needed row stop loading (their
-    // share of the row is then not fetched)
-    bool todo = need != 0;
-    for (uint64_t base = b; base < e; base += 64) {
-      const uint64_t idx = base + lane;
-      uint32_t u = 0;
-      bool act = false;
-      if (idx < e) {
-        u = a.padj[idx];
-        act = (a.actbm[u >> 6] >> (u & 63)) & 1;
-      }
-      const uint64_t m = __ballot(act);
-      const uint32_t n = __popcll(m);
-      if (act) nbr[lane_rank(m)] = u;
-      wave_sync();
-      if (todo) {
-        uint32_t k = g;
-        for (; k + 3 * G < n; k += 4 * G) {
-          const uint32_t u0 = nbr[k], u1 = nbr[k + G], u2 = nbr[k + 2 * G],
-                         u3 = nbr[k + 3 * G];
-          const uint64_t x0 = a.F[static_cast<size_t>(u0) * TW + j];
-          const uint64_t x1 = a.F[static_cast<size_t>(u1) * TW + j];
-          const uint64_t x2 = a.F[static_cast<size_t>(u2) * TW + j];
-          const uint64_t x3 = a.F[static_cast<size_t>(u3) * TW + j];
-          acc |= (x0 | x1) | (x2 | x3);
-        }
-        for (; k < n; k += G) acc |= a.F[static_cast<size_t>(nbr[k]) * TW + j];
-      }
-      wave_sync();
-      acc = group_or<TW>(acc);
-      todo = (acc & need) != need;
-      if (__ballot(todo) == 0) break;  // nothing left to find
+  const uint32_t v = static_cast<uint32_t>(item);
+  const uint32_t ch = static_cast<uint32_t>(item >> 32);
+  const size_t wi = static_cast<size_t>(v) * TW + j;
+  uint64_t vis = 0;
+  if (SHORTEST) vis = a.Vis[wi];
+  const uint64_t need = mk.live & ~vis;
+  if (__ballot(need != 0) == 0) return;
+  const uint64_t lb = a.poff[v], le = a.poff[v + 1];
+  const bool heavy = le - lb > kPullChunk;
+  const uint64_t b = lb + static_cast<uint64_t>(ch) * kPullChunk;
+  const uint64_t e = min(b + kPullChunk, le);
+  uint64_t acc = 0;
+  bool todo = need != 0;
+  for (uint64_t base = b; base < e; base += 64) {
+    const uint64_t idx = base + lane;
+    uint32_t u = 0;
+    bool act = false;
+    if (idx < e) {
+      u = a.padj[idx];
+      act = (a.actbm[u >> 6] >> (u & 63)) & 1;
     }
+    const uint64_t m = __ballot(act);
+    const uint32_t n = __popcll(m);
+    if (act) nbr[lane_rank(m)] = u;
+    wave_sync();
+    if (todo) {
+      uint32_t k = g;
+      for (; k + 3 * G < n; k += 4 * G) {
+        const uint32_t u0 = nbr[k], u1 = nbr[k + G], u2 = nbr[k + 2 * G],
+                       u3 = nbr[k + 3 * G];
+        const uint64_t x0 = a.F[static_cast<size_t>(u0) * TW + j];
+        const uint64_t x1 = a.F[static_cast<size_t>(u1) * TW + j];
+        const uint64_t x2 = a.F[static_cast<size_t>(u2) * TW + j];
+        const uint64_t x3 = a.F[static_cast<size_t>(u3) * TW + j];
+        acc |= (x0 | x1) | (x2 | x3);
+      }
+      for (; k < n; k += G) acc |= a.F[static_cast<size_t>(nbr[k]) * TW + j];
+    }
+    wave_sync();
     acc = group_or<TW>(acc);
-    const uint64_t nb = acc & need;
-    if (g == 0 && nb) emit_word<SHORTEST>(a, wi, nb, vis, mk.racc, c);
-    if (__ballot(g == 0 && nb != 0)) {
-      actmask |= 1ull << i;
-      const uint64_t deg = same_csr ? e - b : a.xoff[v + 1] - a.xoff[v];
-      if (lane == 0) {
-        c.act += 1;
-        c.actdeg += deg;
-      }
-      if (g == 0) c.te += deg * __popcll(nb & mk.live_next);
-    }
+    todo = (acc & need) != need;
+    if (__ballot(todo) == 0) break;  // nothing left to find
   }
-  if (lane == 0) a.actbm_n[vb] = actmask;
+  acc = group_or<TW>(acc);
+  const uint64_t nb = acc & need;
+  const bool row_new = __ballot(g == 0 && nb != 0) != 0;
+  if (heavy) {
+    if (g == 0 && nb)
+      atomicOr(reinterpret_cast<ull *>(&a.N[wi]), static_cast<ull>(nb));
+    if (row_new && lane == 0)
+      atomicOr(reinterpret_cast<ull *>(&a.touched[v >> 6]), 1ull << (v & 63));
+    return;
+  }
+  if (g == 0 && nb) emit_word<SHORTEST>(a, wi, nb, vis, mk.racc, c);
+  if (row_new) {
+    const uint64_t deg = a.xoff == a.poff ? le - lb : a.xoff[v + 1] - a.xoff[v];
+    if (lane == 0) {
+      c.act += 1;
+      c.actdeg += deg;
+      atomicOr(reinterpret_cast<ull *>(&a.actbm_n[v >> 6]), 1ull << (v & 63));
+    }
+    if (g == 0) c.te += deg * __popcll(nb & mk.live_next);
+  }
 }
 
 // ------------------------------------------------------------------- push
@@ -266,7 +277,8 @@ __device__ __forceinline__ void epi_block(const HopArgs &a, uint32_t vb,
   }
   actmask = wave_or(actmask);
   if (lane == 0) {
-    a.actbm_n[vb] = actmask;
+    // pull items of light vertices set bits of this word too
+    if (actmask) atomicOr(reinterpret_cast<ull *>(&a.actbm_n[vb]), static_cast<ull>(actmask));
     a.touched[vb] = 0;
   }
 }
@@ -343,11 +355,10 @@ __global__ __launch_bounds__(kBlock) void k_pull(HopArgs a) {
   __shared__ uint32_t s_nbr[kWavesPerBlock][64];
   const int wid = threadIdx.x >> 6;
   const LaneMasks mk = lane_masks(a, lane_id() % TW);
-  const uint32_t nvb = (a.V + 63) / 64;
-  const uint32_t nwaves = gridDim.x * kWavesPerBlock;
+  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
   LocalCtr c;
-  for (uint32_t vb = blockIdx.x * kWavesPerBlock + wid; vb < nvb; vb += nwaves)
-    pull_block<TW, SHORTEST>(a, vb, s_nbr[wid], mk, c);
+  for (uint64_t it = blockIdx.x * kWavesPerBlock + wid; it < a.npitems; it += nwaves)
+    pull_item<TW, SHORTEST>(a, a.pitems[it], s_nbr[wid], mk, c);
   flush_ctr(c, a.ctr);
 }
 
@@ -391,6 +402,8 @@ void launch_pull(uint32_t tw, bool shortest, uint32_t grid, hipStream_t s,
                  const HopArgs &a);
 void launch_push(uint32_t tw, bool shortest, uint32_t grid, hipStream_t s,
                  const HopArgs &a);
+void launch_epi(uint32_t tw, bool shortest, uint32_t grid, hipStream_t s,
+                const HopArgs &a);
 void launch_clear(uint32_t tw, uint32_t grid, hipStream_t s, uint64_t *F,
                   uint64_t *bm, uint32_t V);
 void launch_items(uint32_t grid, hipStream_t s, const uint64_t *bm, uint32_t V,

@@ -118,6 +118,17 @@ void launch_push(uint32_t tw, bool shortest, uint32_t grid, hipStream_t s,
 }
 
 template <int TW>
+static void launch_epi_tw(bool shortest, uint32_t grid, hipStream_t s, const HopArgs &a) {
+  if (shortest) hipLaunchKernelGGL((k_push_epi<TW, true>), dim3(grid), dim3(kBlock), 0, s, a);
+  else hipLaunchKernelGGL((k_push_epi<TW, false>), dim3(grid), dim3(kBlock), 0, s, a);
+}
+
+void launch_epi(uint32_t tw, bool shortest, uint32_t grid, hipStream_t s,
+                const HopArgs &a) {
+  VS_TW_CALL(tw, launch_epi_tw, shortest, grid, s, a);
+}
+
+template <int TW>
 static void launch_clear_tw(uint32_t grid, hipStream_t s, uint64_t *F, uint64_t *bm,
                             uint32_t V) {
   hipLaunchKernelGGL((k_clear<TW>), dim3(grid), dim3(kBlock), 0, s, F, bm, V);
@@ -376,7 +387,7 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
     VS_HIP(hipStreamSynchronize(s));  // d0 leaves scope
   }
 
-  const uint32_t pull_grid = grid_for(nvb, g->num_cus);
+  const uint32_t pull_grid = static_cast<uint32_t>(g->num_cus) * 8;
   const uint32_t clear_grid = grid_for(nvb, g->num_cus);
   const uint32_t item_grid = std::max<uint32_t>(1, std::min<uint32_t>((nvb + kBlock - 1) / kBlock, g->num_cus * 4));
   const uint32_t push_grid = static_cast<uint32_t>(g->num_cus) * 8;
@@ -431,7 +442,7 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
                    : heuristic_decide(f, g->tuning);
       }
       tile_mode[t] = mode;
-      HopArgs a;
+      HopArgs a{};
       a.V = V;
       a.S = S;
       a.word0 = t * TW;
@@ -463,13 +474,22 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
         launch_push(TW, shortest, push_grid, s, a);
         if (timing) VS_HIP(hipEventRecord(k1, s));
       } else {
-        a.N = nullptr;
-        a.touched = nullptr;
+        VS_TRY(ensure_pull_items(*const_cast<DevCSR *>(pull), V));
+        a.pitems = pull->pitems;
+        a.npitems = pull->npitems;
         a.items = nullptr;
         a.nitems = nullptr;
+        a.N = nullptr;
+        a.touched = nullptr;
+        if (pull->has_heavy) {  // hub partials meet in N, finished by the epilogue
+          VS_TRY(ensure_push());
+          a.N = N + t * tile_words;
+          a.touched = touched + static_cast<size_t>(t) * nvb;
+        }
         if (timing) VS_HIP(hipEventRecord(k0, s));
         launch_pull(TW, shortest, pull_grid, s, a);
         if (timing) VS_HIP(hipEventRecord(k1, s));
+        if (pull->has_heavy) launch_epi(TW, shortest, push_grid, s, a);
       }
       VS_HIP(hipGetLastError());
       if (timing && st) {
