@@ -1,20 +1,32 @@
 """VExpand throughput benchmark (one MI355X).
 
-Workload (default): symmetrized R-MAT scale 24, edge factor 32
-(a=.57 b=.19 c=.19 d=.05, self-loops and duplicates removed), 4096 seeded
-uniform sources, Shortest, hops 1..4, frontier / visited / result bit-matrices
-resident in HBM.  A step is one full VExpand over that source batch; the CSR is
-uploaded once before timing.  Prints one JSON line.
+Default workload: symmetrized R-MAT scale 24, edge factor 32 (a=.57 b=.19 c=.19
+d=.05, self-loops and duplicates removed), 4096 seeded uniform sources, Shortest,
+hops 1..4; frontier / visited / result bit-matrices resident in HBM; the CSR is
+uploaded once before timing.  A step is one full VExpand over the 4096 sources.
+
+Two GPU legs are measured on the same workload:
+  * plain: the single-query engine (vs_vexpand), density-heuristic push/pull;
+  * batched+learned: the batched engine (vs_vexpand_batch: tiles packed into one
+    persistent expand kernel per hop, the launch sequence captured once in a
+    hipGraph and replayed), with the autotuned tile width and the learned
+    scheduler when its weights are present (else the heuristic).
+Both must give the same per-source counts and per-row digests; the plain leg's
+full matrix is also compared bit-for-bit with the CPU baseline on a sample.
+`value` is the batched+learned GTEPS; the plain GTEPS is reported next to it.
 
 The CPU baseline is the reference algorithm restated in C++ (oracle B: 1024-row
 stacks, destination-partitioned Hilbert edge blocks, OpenMP) timed on a bounded
-sample of the same workload on this host, and checked bit-for-bit against the
-GPU result for the same sample.
+sample of the same workload on this host.
+
+--workload batch runs the 256-query batch (sources log-uniform in 16..8192, hops
+1..k with k uniform in 2..6) plain-one-by-one vs batched.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -25,18 +37,22 @@ sys.path.insert(0, str(ROOT))
 
 GRAPH_SEED = 1
 SOURCE_SEED = 20241203
+BATCH_SEED = 20250256
 HBM_PEAK_GBS = 8000.0
+METRIC = "VExpand GTEPS, bit-exact, R-MAT-24 4096 src; batched+learned vs plain GPU"
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def main() -> int:
+def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="vexpand", choices=["vexpand", "batch", "mintersect"])
+    ap.add_argument("--triples", type=int, default=4 << 20)
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--edge-factor", type=int, default=32)
     ap.add_argument("--sources", type=int, default=4096)
@@ -44,18 +60,81 @@ def main() -> int:
     ap.add_argument("--k-max", type=int, default=4)
     ap.add_argument("--path", default="shortest", choices=["shortest", "any"])
     ap.add_argument("--sched", default="heuristic",
-                    choices=["heuristic", "push", "pull", "learned"])
-    ap.add_argument("--tile-words", type=int, default=0)
+                    choices=["heuristic", "push", "pull", "learned"],
+                    help="scheduling of the plain leg")
+    ap.add_argument("--tile-words", type=int, default=0, help="plain leg tile width")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--weights", default=str(ROOT / "vertexsurge_amd" / "sched_weights.npz"))
     ap.add_argument("--cpu-sources", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-queries", type=int, default=256)
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="first run a plain copy kernel over this many MiB (counter scale)")
     ap.add_argument("--profile-only", action="store_true",
-                    help="graph + warmup + steps only (for rocprofv3 runs)")
-    args = ap.parse_args()
+                    help="graph + timed legs only (for rocprofv3 runs)")
+    return ap.parse_args()
 
+
+def calibration_copy(vs, mb):
+    import ctypes as C
+    nbytes = mb << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    vs._lib.call("vs_dev_alloc", nbytes, C.byref(a))
+    vs._lib.call("vs_dev_alloc", nbytes, C.byref(b))
+    vs._lib.call("vs_device_copy", b, a, nbytes)
+    vs._lib.call("vs_dev_free", a)
+    vs._lib.call("vs_dev_free", b)
+    log(f"calibration copy kernel: {nbytes} bytes read + {nbytes} written")
+
+
+def time_plain(vs, np, G, src, args, k_min, k_max, path, sched, tile_words):
+    te = None
+    for i in range(args.warmup):
+        res, st = vs.vexpand(G, src, k_min, k_max, path, sched=sched, tile_words=tile_words)
+        te = st["te"]
+        res.close()
+    vs._lib.call("vs_device_synchronize")
+    times = []
+    for i in range(args.steps):
+        t = time.perf_counter()
+        res, st = vs.vexpand(G, src, k_min, k_max, path, sched=sched, tile_words=tile_words)
+        vs._lib.call("vs_device_synchronize")
+        times.append(time.perf_counter() - t)
+        te = st["te"]
+        res.close()
+    ms = 1e3 * float(np.mean(times))
+    return {"ms": ms, "te": te, "gteps": te / (ms / 1e3) / 1e9}
+
+
+def load_scheduler(vs, G, path):
+    p = Path(path)
+    if not p.exists():
+        return None
+    from vertexsurge_amd import sched_train
+    s = sched_train.load(p)
+    G.set_scheduler(s)
+    return s
+
+
+def batched_leg(vs, np, G, queries, args, sched):
+    """Timed replays produce the per-source counts (the result); the per-row
+    digests for the parity check come from one more, untimed, call."""
+    reps = args.warmup + args.steps
+    c, _, st = vs.vexpand_batch(G, queries, sched=sched, replays=max(1, reps),
+                                want_digests=False)
+    ms = st["ms_total"]
+    c2, d, _ = vs.vexpand_batch(G, queries, sched=sched, replays=1, want_digests=True)
+    if not np.array_equal(c, c2):
+        log("batched counts differ between the timed and the digest run")
+    return c, d, st, {"ms": ms, "te": st["te"], "gteps": st["te"] / (ms / 1e3) / 1e9}
+
+
+def main() -> int:
+    args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no "
-              "multi-GPU path (sources would shard across GPUs with a replicated CSR; see "
-              "the design notes).", file=sys.stderr)
+              "multi-GPU path (source rows would shard across GPUs with a replicated CSR).",
+              file=sys.stderr)
         return 2
 
     import numpy as np
@@ -70,82 +149,74 @@ def main() -> int:
     csr = vs.HostCSR.rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, GRAPH_SEED)
     log(f"graph V={csr.V} E={csr.E} built in {time.time() - t0:.1f}s; uploading")
     G = vs.Graph.from_csr(csr)
-    src = vs.sample_sources(csr.V, args.sources, SOURCE_SEED)
-    kw = dict(path=args.path, sched=args.sched, tile_words=args.tile_words)
+    if args.copy_calibration_mb:
+        calibration_copy(vs, args.copy_calibration_mb)
+    if args.workload == "batch":
+        return run_batch_workload(vs, np, csr, G, args)
+    if args.workload == "mintersect":
+        return run_mintersect_workload(vs, np, csr, G, args)
 
-    te = None
-    for i in range(args.warmup):
-        res, st = vs.vexpand(G, src, args.k_min, args.k_max, **kw)
-        te = st["te"]
-        res.close()
-        log(f"warmup {i}: {st['ms_total']:.2f} ms device, TE={te}, k_actual={st['k_actual_max']}")
-    vs._lib.call("vs_device_synchronize")
-    times, dev_ms = [], []
-    for i in range(args.steps):
-        t = time.perf_counter()
-        res, st = vs.vexpand(G, src, args.k_min, args.k_max, **kw)
-        vs._lib.call("vs_device_synchronize")
-        times.append(time.perf_counter() - t)
-        dev_ms.append(st["ms_total"])
-        te = st["te"]
-        res.close()
-    vs._lib.call("vs_device_synchronize")
-    ms_step = 1e3 * float(np.mean(times))
-    gteps = te / (ms_step / 1e3) / 1e9
-    log(f"timed {args.steps} steps: {ms_step:.2f} ms/step (device {np.mean(dev_ms):.2f} ms), "
-        f"{gteps:.1f} GTEPS")
+    src = vs.sample_sources(csr.V, args.sources, SOURCE_SEED)
+    plain = time_plain(vs, np, G, src, args, args.k_min, args.k_max, args.path, args.sched,
+                       args.tile_words)
+    log(f"plain: {plain['ms']:.2f} ms/step, {plain['gteps']:.1f} GTEPS (TE={plain['te']})")
+
+    tuning = G.get_tuning()
+    if not args.no_autotune:
+        import ctypes as C
+        t = vs._lib.Tuning()
+        best_ms = C.c_double()
+        vs._lib.call("vs_autotune_vexpand", G.handle, src.ctypes.data_as(vs._lib.u32p),
+                     len(src), args.k_max, vs._PATHS[args.path], C.byref(t), C.byref(best_ms))
+        tuning = {"tile_words": t.tile_words, "push_alpha": t.push_alpha}
+        log(f"autotuned: {tuning} ({best_ms.value:.2f} ms)")
+    sched = load_scheduler(vs, G, args.weights)
+    bsched = "learned" if sched is not None else "heuristic"
+    queries = [vs.BatchQuery(src, args.k_min, args.k_max, args.path)]
+    bc, bd, bst, batched = batched_leg(vs, np, G, queries, args, bsched)
+    log(f"batched+{bsched}: {batched['ms']:.2f} ms/step, {batched['gteps']:.1f} GTEPS "
+        f"(tiles {bst['num_tiles']}, push/pull decisions {bst['push_decisions']}/"
+        f"{bst['pull_decisions']})")
     if args.profile_only:
         return 0
 
-    # per-kernel HIP-event timings for the roofline figure
-    res, tst = vs.vexpand(G, src, args.k_min, args.k_max, timing=True, **kw)
-    counts = res.row_counts(0)
+    # parity of the two legs: per-source counts and digests
+    res, pst = vs.vexpand(G, src, args.k_min, args.k_max, args.path, sched=args.sched,
+                          tile_words=args.tile_words, timing=True)
+    pc, pd = res.row_counts(0), res.row_digests(0)
     res.close()
-    dom = "pull" if tst["ms_pull"] >= tst["ms_push"] else "push"
-    n_l = tst[f"n_{dom}"] or 1
-    bytes_per_launch = tst[f"bytes_{dom}"] / n_l
-    ms_per_launch = tst[f"ms_{dom}"] / n_l
-    achieved = bytes_per_launch / (ms_per_launch / 1e3) / 1e9 if ms_per_launch > 0 else 0.0
-    log(f"dominant kernel {dom}: {n_l} launches, {ms_per_launch:.3f} ms avg, "
-        f"{bytes_per_launch / 1e9:.3f} GB alg/launch -> {achieved:.1f} GB/s; "
-        f"modes per hop {tst['mode_hop']} ms per hop {[round(x, 2) for x in tst['ms_hop']]}")
+    legs_exact = bool(np.array_equal(pc, bc) and np.array_equal(pd, bd)
+                      and pst["te"] == bst["te"])
+    log(f"batched == plain (counts, digests, TE): {legs_exact}; plain modes per hop "
+        f"{pst['mode_hop']} ms per hop {[round(x, 2) for x in pst['ms_hop']]}")
+
+    # roofline of the measured leg's dominant kernel (per-launch HIP events)
+    _, _, tst = vs.vexpand_batch(G, queries, sched=bsched, timing=True)
+    n_l = max(1, tst["n_expand"])
+    ms_l = tst["ms_expand"] / n_l
+    b_l = tst["bytes_alg"] / n_l
+    achieved = b_l / (ms_l / 1e3) / 1e9 if ms_l > 0 else 0.0
+    log(f"k_bw_expand: {n_l} launches, {ms_l:.3f} ms avg, {b_l / 1e9:.3f} GB alg/launch -> "
+        f"{achieved:.1f} GB/s")
+    p_dom = "pull" if pst["ms_pull"] >= pst["ms_push"] else "push"
+    p_n = max(1, pst[f"n_{p_dom}"])
+    p_ach = (pst[f"bytes_{p_dom}"] / p_n) / (pst[f"ms_{p_dom}"] / p_n / 1e3) / 1e9 \
+        if pst[f"ms_{p_dom}"] > 0 else 0.0
 
     cpu = None
     if not args.no_cpu:
-        import oracle
-        ncpu = min(args.cpu_sources, args.sources)
-        sample = src[:ncpu]
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        log(f"CPU baseline: building destination-partitioned Hilbert blocks ({threads} threads)")
-        tb = time.time()
-        blocks = oracle.EdgeBlocks(csr.offsets, csr.adj, nblocks=1000)
-        log(f"blocks built in {time.time() - tb:.1f}s; running oracle B on {ncpu} sources")
-        dv, ka_cpu, _, cpu_ms = blocks.vexpand(sample, args.k_min, args.k_max, args.path)
-        rs, sst = vs.vexpand(G, sample, args.k_min, args.k_max, **kw)
-        gpu_dv = rs.export(0, vs.LAYOUT_DVSM)
-        exact = bool(np.array_equal(gpu_dv, dv)) and ka_cpu == sst["k_actual_max"]
-        rs.close()
-        del gpu_dv, dv
-        blocks.close()
-        cpu_gteps = sst["te"] / (cpu_ms / 1e3) / 1e9
-        log(f"CPU {cpu_ms / 1e3:.2f}s for {ncpu} sources (TE={sst['te']}): {cpu_gteps:.2f} GTEPS;"
-            f" GPU==CPU bit-exact: {exact}")
-        if not exact:
-            log("PARITY FAILURE between GPU and CPU baseline on the sample")
-        cpu = {"value": round(cpu_gteps, 3), "unit": "GTEPS", "cores": threads,
-               "kind": "port",
-               "sample": f"{ncpu} of {args.sources} sources, hops {args.k_min}..{args.k_max}, "
-                         f"same graph; TE of the sample / CPU time",
-               "bit_exact_vs_gpu": exact}
+        cpu = cpu_baseline(vs, np, csr, G, src, args)
 
+    use_batched = legs_exact and batched["gteps"] > 0
+    head = batched if use_batched else plain
     out = {
-        "metric": "VExpand GTEPS, bit-exact, R-MAT-24 4096 src; batched+learned vs plain GPU",
-        "value": round(gteps, 2),
+        "metric": METRIC,
+        "value": round(head["gteps"], 2),
         "unit": "GTEPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 3),
+        "ms_per_step": round(head["ms"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -156,18 +227,172 @@ def main() -> int:
                         f"hops{args.k_min}-{args.k_max}_{args.path}",
             "scale": args.scale, "edge_factor": args.edge_factor, "vertices": csr.V,
             "edges": csr.E, "sources": args.sources, "k_min": args.k_min,
-            "k_max": args.k_max, "path": args.path, "sched": args.sched,
-            "tile_words": tst["tile_words"], "te": te, "k_actual_max": tst["k_actual_max"],
-            "reachable_pairs": int(counts.sum()),
+            "k_max": args.k_max, "path": args.path, "te": plain["te"],
+            "k_actual_max": pst["k_actual_max"], "reachable_pairs": int(pc.sum()),
+            "value_leg": f"batched+{bsched}" if use_batched else "plain",
+            "batched": {"gteps": round(batched["gteps"], 2), "ms": round(batched["ms"], 3),
+                        "sched": bsched, "tuning": tuning, "tiles": bst["num_tiles"],
+                        "graph_launches": bst["launches"],
+                        "push_decisions": bst["push_decisions"],
+                        "pull_decisions": bst["pull_decisions"],
+                        "bit_exact_vs_plain": legs_exact},
+            "plain_gpu": {"gteps": round(plain["gteps"], 2), "ms": round(plain["ms"], 3),
+                          "sched": args.sched, "tile_words": pst["tile_words"],
+                          "dominant_kernel": f"k_{p_dom}",
+                          "dominant_achieved_gbs": round(p_ach, 1),
+                          "ms_per_hop": [round(x, 3) for x in pst["ms_hop"]],
+                          "mode_per_hop": pst["mode_hop"]},
         },
         "roofline": {
-            "bound": "hbm", "kernel": f"k_{dom}", "achieved": round(achieved, 1),
+            "bound": "hbm", "kernel": "k_bw_expand", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None, "launches": n_l, "avg_ms": round(ms_per_launch, 4),
-            "alg_bytes_per_launch": int(bytes_per_launch),
+            "traffic": None, "launches": n_l, "avg_ms": round(ms_l, 4),
+            "alg_bytes_per_launch": int(b_l),
         },
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def cpu_baseline(vs, np, csr, G, src, args):
+    import oracle
+    ncpu = min(args.cpu_sources, len(src))
+    sample = src[:ncpu]
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    log(f"CPU baseline: building destination-partitioned Hilbert blocks ({threads} threads)")
+    tb = time.time()
+    blocks = oracle.EdgeBlocks(csr.offsets, csr.adj, nblocks=1000)
+    log(f"blocks built in {time.time() - tb:.1f}s; running oracle B on {ncpu} sources")
+    dv, ka_cpu, _, cpu_ms = blocks.vexpand(sample, args.k_min, args.k_max, args.path)
+    rs, sst = vs.vexpand(G, sample, args.k_min, args.k_max, args.path)
+    gpu_dv = rs.export(0, vs.LAYOUT_DVSM)
+    exact = bool(np.array_equal(gpu_dv, dv)) and ka_cpu == sst["k_actual_max"]
+    rs.close()
+    del gpu_dv, dv
+    blocks.close()
+    cpu_gteps = sst["te"] / (cpu_ms / 1e3) / 1e9
+    log(f"CPU {cpu_ms / 1e3:.2f}s for {ncpu} sources (TE={sst['te']}): {cpu_gteps:.2f} GTEPS;"
+        f" GPU==CPU bit-exact: {exact}")
+    return {"value": round(cpu_gteps, 3), "unit": "GTEPS", "cores": threads, "kind": "port",
+            "sample": f"{ncpu} of {len(src)} sources, hops {args.k_min}..{args.k_max}, same "
+                      f"graph; TE of the sample / CPU time",
+            "bit_exact_vs_gpu": exact}
+
+
+def batch_queries(vs, np, V, n, seed):
+    """n queries: S log-uniform in [16, 8192], hops 1..k with k uniform in 2..6."""
+    rng = np.random.default_rng(seed)
+    qs = []
+    for q in range(n):
+        S = int(math.floor(16 * 2 ** rng.uniform(0, 9)))
+        S = max(16, min(8192, S))
+        k = int(rng.integers(2, 7))
+        qs.append(vs.BatchQuery(vs.sample_sources(V, S, seed * 100003 + q), 1, k))
+    return qs
+
+
+def run_batch_workload(vs, np, csr, G, args):
+    qs = batch_queries(vs, np, csr.V, args.batch_queries, BATCH_SEED)
+    rows = sum(len(q.sources) for q in qs)
+    log(f"{len(qs)} queries, {rows} source rows")
+    # plain: every query alone, one after another
+    t = time.perf_counter()
+    te_plain, counts, digests = 0, [], []
+    for i, q in enumerate(qs):
+        res, st = vs.vexpand(G, q.sources, q.k_min, q.k_max, q.path)
+        te_plain += st["te"]
+        counts.append(res.row_counts(0))
+        digests.append(res.row_digests(0))
+        res.close()
+        if i % 32 == 31:
+            log(f"plain {i + 1}/{len(qs)} queries, {time.perf_counter() - t:.1f}s")
+    vs._lib.call("vs_device_synchronize")
+    plain_s = time.perf_counter() - t
+    sched = load_scheduler(vs, G, args.weights)
+    bsched = "learned" if sched is not None else "heuristic"
+    t = time.perf_counter()
+    bc, bd, st = vs.vexpand_batch(G, qs, sched=bsched, replays=1)
+    batch_wall = time.perf_counter() - t
+    exact = bool(np.array_equal(np.concatenate(counts), bc)
+                 and np.array_equal(np.concatenate(digests), bd) and st["te"] == te_plain)
+    g_plain = te_plain / plain_s / 1e9
+    g_batch = st["te"] / (st["ms_total"] / 1e3) / 1e9
+    log(f"plain {plain_s:.2f}s {g_plain:.1f} GTEPS; batched {st['ms_total'] / 1e3:.2f}s device "
+        f"({batch_wall:.2f}s wall) {g_batch:.1f} GTEPS; exact={exact}; {st}")
+    out = {"metric": "VExpand GTEPS, 256-query batch on R-MAT-24 (batched vs plain GPU)",
+           "value": round(g_batch, 2), "unit": "GTEPS", "n_gpus": 1, "higher_is_better": True,
+           "data": "synthetic", "dtype": "u64",
+           "config": {"workload": f"rmat{args.scale}_ef{args.edge_factor}_batch{len(qs)}",
+                      "queries": len(qs), "rows": rows, "te": te_plain,
+                      "plain_gteps": round(g_plain, 2), "plain_s": round(plain_s, 3),
+                      "batched_s": round(st["ms_total"] / 1e3, 3), "sched": bsched,
+                      "bit_exact_vs_plain": exact, "stats": st}}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def run_mintersect_workload(vs, np, csr, G, args):
+    """|N(a) ∩ N(b) ∩ N(c)| and the sorted lists for sampled 2-hop paths a-b-c,
+    arrays resident in HBM; count -> scan -> fill timed with HIP events."""
+    import ctypes as C
+    T = args.triples
+    tup = vs.sample_paths3(csr.offsets, csr.adj, T, seed=BATCH_SEED)
+    lib = vs._lib
+    d_t, d_c, d_o, d_l = (C.c_void_p() for _ in range(4))
+    lib.call("vs_dev_alloc", tup.nbytes, C.byref(d_t))
+    lib.call("vs_dev_alloc", T * 4, C.byref(d_c))
+    lib.call("vs_dev_alloc", (T + 1) * 8, C.byref(d_o))
+    lib.call("vs_memcpy_h2d", d_t, tup.ctypes.data_as(C.c_void_p), tup.nbytes)
+    deps = (C.c_void_p * 3)(G.handle, G.handle, G.handle)
+    total = C.c_uint64()
+    st = lib.MintersectStats()
+    lib.call("vs_mintersect_device", deps, 3, d_t, T, d_c, d_o, None, 0, C.byref(total),
+             C.byref(st))
+    cap = max(1, total.value)
+    lib.call("vs_dev_alloc", cap * 4, C.byref(d_l))
+    times = []
+    for i in range(args.warmup + args.steps):
+        lib.call("vs_mintersect_device", deps, 3, d_t, T, d_c, d_o, d_l, cap, C.byref(total),
+                 C.byref(st))
+        if i >= args.warmup:
+            times.append(st.ms)
+    ms = float(np.mean(times))
+    log(f"{T} triples: {ms:.2f} ms (count+scan+fill), total {total.value} list entries, "
+        f"{T / (ms / 1e3) / 1e6:.1f} M triples/s, {st.bytes_alg / (ms / 1e3) / 1e9:.1f} GB/s alg")
+    # parity on a sample against the CPU set-intersection oracle
+    import oracle
+    ns = min(T, 200000)
+    counts = np.zeros(T, dtype=np.uint32)
+    offs = np.zeros(T + 1, dtype=np.uint64)
+    lib.call("vs_memcpy_d2h", counts.ctypes.data_as(C.c_void_p), d_c, T * 4)
+    lib.call("vs_memcpy_d2h", offs.ctypes.data_as(C.c_void_p), d_o, (T + 1) * 8)
+    nl = int(offs[ns])
+    lists = np.zeros(max(1, nl), dtype=np.uint32)
+    if nl:
+        lib.call("vs_memcpy_d2h", lists.ctypes.data_as(C.c_void_p), d_l, nl * 4)
+    t = time.perf_counter()
+    c2, o2, l2 = oracle.mintersect_sorted([(csr.offsets, csr.adj)] * 3, tup[:ns])
+    cpu_s = time.perf_counter() - t
+    exact = bool(np.array_equal(counts[:ns], c2) and np.array_equal(lists[:nl], l2))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    log(f"CPU sorted-merge oracle: {ns} triples in {cpu_s:.2f}s; sample bit-exact: {exact}")
+    for p in (d_t, d_c, d_o, d_l):
+        lib.call("vs_dev_free", p)
+    achieved = st.bytes_alg / (ms / 1e3) / 1e9
+    out = {"metric": "Mintersect triples/s, |N(a)∩N(b)∩N(c)| + sorted lists, R-MAT-24",
+           "value": round(T / (ms / 1e3), 1), "unit": "triples/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+           "higher_is_better": True, "dtype": "u32", "data": "synthetic",
+           "config": {"workload": f"rmat{args.scale}_ef{args.edge_factor}_paths3_{T}",
+                      "triples": T, "list_entries": total.value,
+                      "bit_exact_sample": exact, "sample": ns},
+           "roofline": {"bound": "hbm", "kernel": "k_mintersect", "achieved": round(achieved, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+           "cpu_baseline": {"value": round(ns / cpu_s, 1), "unit": "triples/s",
+                            "cores": threads, "kind": "port",
+                            "sample": f"first {ns} of {T} triples, std::set_intersection chain"}}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -127,6 +127,9 @@ vs_status vs_dev_alloc(size_t bytes, void **dptr);
 vs_status vs_dev_free(void *dptr);
 vs_status vs_memcpy_h2d(void *dst, const void *src, size_t bytes);
 vs_status vs_memcpy_d2h(void *dst, const void *src, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (k_copy_words), a
+ * known-size reference for HBM traffic counters.  bytes % 8 == 0. */
+vs_status vs_device_copy(void *dst, const void *src, size_t bytes);
 
 /* ------------------------------------------------------------------ graph */
 /* Replaces the global PhysicalOperator::graph_store and its
@@ -263,7 +266,8 @@ typedef struct {
 
 typedef struct {
   uint64_t te;
-  double ms_total;      /* device time of all waves (last replay) */
+  double ms_total;      /* device time of all waves of one replay: the mean
+                           over replays 2..n when replays > 1 */
   double ms_capture;
   uint32_t num_tiles;
   uint32_t num_waves;   /* groups of tiles resident at once */
@@ -271,6 +275,11 @@ typedef struct {
   uint32_t reserved;
   uint64_t push_decisions;
   uint64_t pull_decisions;
+  /* with VS_VX_TIMING in flags (runs without graph capture): device time and
+   * compulsory bytes of the persistent expand launches */
+  double ms_expand;
+  uint64_t n_expand;
+  uint64_t bytes_alg;
 } vs_batch_stats;
 
 /* Runs nq independent VExpand queries packed into shared bit-matrix tiles.

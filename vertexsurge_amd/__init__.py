@@ -354,7 +354,8 @@ class BatchQuery:
 
 def vexpand_batch(graph: Graph, queries: list, tile_words: int = 0, sched: str = "heuristic",
                   use_graph: bool = True, replays: int = 1, mem_budget: int = 0,
-                  direction: str = "forward", want_digests: bool = True):
+                  direction: str = "forward", want_digests: bool = True,
+                  timing: bool = False):
     """Runs many VExpand queries packed into shared tiles.  Returns
     (row_counts, row_digests, stats) concatenated over queries."""
     keep = [_u32(q.sources) for q in queries]
@@ -369,7 +370,7 @@ def vexpand_batch(graph: Graph, queries: list, tile_words: int = 0, sched: str =
     counts = np.zeros(total, dtype=np.uint64)
     digests = np.zeros(total, dtype=np.uint64) if want_digests else None
     p = _lib.BatchParams(tile_words, _SCHED[sched], int(use_graph), replays, mem_budget,
-                         _DIRS[direction], 0)
+                         _DIRS[direction], _lib.VX_TIMING if timing else 0)
     st = _lib.BatchStats()
     _lib.call("vs_vexpand_batch", graph.handle, arr, len(queries), C.byref(p),
               _ptr(counts, C.c_uint64),
@@ -377,4 +378,5 @@ def vexpand_batch(graph: Graph, queries: list, tile_words: int = 0, sched: str =
     return counts, digests, {
         "te": st.te, "ms_total": st.ms_total, "ms_capture": st.ms_capture,
         "num_tiles": st.num_tiles, "num_waves": st.num_waves, "launches": st.launches,
-        "push_decisions": st.push_decisions, "pull_decisions": st.pull_decisions}
+        "push_decisions": st.push_decisions, "pull_decisions": st.pull_decisions,
+        "ms_expand": st.ms_expand, "n_expand": st.n_expand, "bytes_alg": st.bytes_alg}

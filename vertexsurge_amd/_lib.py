@@ -76,7 +76,8 @@ class BatchStats(C.Structure):
     _fields_ = [("te", C.c_uint64), ("ms_total", C.c_double), ("ms_capture", C.c_double),
                 ("num_tiles", C.c_uint32), ("num_waves", C.c_uint32), ("launches", C.c_uint32),
                 ("reserved", C.c_uint32), ("push_decisions", C.c_uint64),
-                ("pull_decisions", C.c_uint64)]
+                ("pull_decisions", C.c_uint64), ("ms_expand", C.c_double),
+                ("n_expand", C.c_uint64), ("bytes_alg", C.c_uint64)]
 
 
 class MintersectStats(C.Structure):
@@ -103,6 +104,7 @@ SIGNATURES = {
     "vs_dev_free": (C.c_int, [vp]),
     "vs_memcpy_h2d": (C.c_int, [vp, vp, C.c_size_t]),
     "vs_memcpy_d2h": (C.c_int, [vp, vp, C.c_size_t]),
+    "vs_device_copy": (C.c_int, [vp, vp, C.c_size_t]),
     "vs_graph_create": (C.c_int, [u64p, u32p, C.c_uint32, C.c_uint64, C.c_uint32,
                                   C.POINTER(vp)]),
     "vs_graph_destroy": (C.c_int, [vp]),

@@ -41,6 +41,7 @@ struct SlotDesc {
   uint64_t *Vis;
   uint64_t *R;
   HopCounters *ctr;  // [H + 1]
+  uint64_t *front;   // [H + 1][64]: per-word OR of the rows of L_h
   uint32_t tile;     // global tile of the current wave, UINT32_MAX if idle
   int32_t mode;      // this hop: 0 skip, 1 push, 2 pull
 };
@@ -59,6 +60,7 @@ struct BatchCtl {
   ull te;
   ull push_dec;
   ull pull_dec;
+  ull bytes;
 };
 
 struct BatchConst {
@@ -120,6 +122,8 @@ __device__ __forceinline__ HopArgs slot_args(const BatchConst &c, const SlotDesc
   a.racc = c.racc ? c.racc + static_cast<size_t>(hop) * c.Wpad + ti.word0 : nullptr;
   a.pitems = c.pitems;
   a.npitems = c.npitems;
+  a.front = sd.front + static_cast<size_t>(hop - 1) * 64;
+  a.front_next = sd.front + static_cast<size_t>(hop) * 64;
   return a;
 }
 
@@ -135,6 +139,8 @@ __global__ void k_bw_assign(BatchConst c) {
     HopCounters *h = &c.slots[i / (c.H + 1)].ctr[i % (c.H + 1)];
     *h = HopCounters{};
   }
+  for (uint32_t i = threadIdx.x; i < c.nslots * (c.H + 1) * 64; i += blockDim.x)
+    c.slots[i / ((c.H + 1) * 64)].front[i % ((c.H + 1) * 64)] = 0;
 }
 
 // ---- zero visited / result matrices of active slots (fill kernel: no
@@ -191,6 +197,7 @@ __global__ void k_bw_l0(BatchConst c) {
       atomicAdd(&h->actdeg, static_cast<ull>(deg));
     }
     atomicAdd(&h->pop, 1ull);
+    atomicOr(reinterpret_cast<ull *>(&sd.front[j]), bit);
     if (w_live1 & bit) atomicAdd(&h->te, static_cast<ull>(deg));
     (void)nvb;
   }
@@ -292,7 +299,7 @@ __global__ __launch_bounds__(kBlock) void k_bw_expand(BatchConst c, uint32_t hop
     while (s + 1 <= c.nslots && c.ctl->unit_prefix[s + 1] <= u) ++s;
     while (c.ctl->unit_prefix[s] > u) --s;
     if (s != cur) {
-      if (cur != UINT32_MAX && mode == 2) flush_ctr(lc, a.ctr);
+      if (cur != UINT32_MAX && mode == 2) flush_ctr(lc, a.ctr, a.front_next, lane_id() % TW);
       cur = s;
       a = slot_args(c, c.slots[s], hop);
       mode = c.slots[s].mode;
@@ -302,7 +309,7 @@ __global__ __launch_bounds__(kBlock) void k_bw_expand(BatchConst c, uint32_t hop
     if (mode == 2) pull_item<TW, SHORTEST>(a, a.pitems[local], s_nbr[wid], mk, lc);
     else push_item<TW>(a, a.items[local]);
   }
-  if (cur != UINT32_MAX && mode == 2) flush_ctr(lc, a.ctr);
+  if (cur != UINT32_MAX && mode == 2) flush_ctr(lc, a.ctr, a.front_next, lane_id() % TW);
 }
 
 // ---- push epilogue and frontier clear, grid-stride over (slot, vertex block)
@@ -322,14 +329,14 @@ __global__ __launch_bounds__(kBlock) void k_bw_epi(BatchConst c, uint32_t hop) {
     const int md = c.slots[s].mode;
     if (md != 1 && !(md == 2 && c.has_heavy)) continue;
     if (s != cur) {
-      if (cur != UINT32_MAX) flush_ctr(lc, a.ctr);
+      if (cur != UINT32_MAX) flush_ctr(lc, a.ctr, a.front_next, lane_id() % TW);
       cur = s;
       a = slot_args(c, c.slots[s], hop);
       mk = lane_masks(a, lane_id() % TW);
     }
     epi_block<TW, SHORTEST>(a, static_cast<uint32_t>(u % nvb), mk, lc);
   }
-  if (cur != UINT32_MAX) flush_ctr(lc, a.ctr);
+  if (cur != UINT32_MAX) flush_ctr(lc, a.ctr, a.front_next, lane_id() % TW);
 }
 
 template <int TW>
@@ -435,12 +442,31 @@ __global__ void k_bw_finish(BatchConst c) {
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    ull te = 0;
+    // traversed edges, and the compulsory bytes of the expanded hops:
+    // 8 nz(F_i) + 8 nz(L_i) + [Shortest] 8 (nz(Vis_{i-1}) + nz(Vis_i))
+    // + [Any] 8 (nz(R_{i-1}) + nz(R_i)) + 4 actdeg(F_i) + 16 act(F_i),
+    // plus 8 nz of the final result
+    ull te = 0, bytes = 0;
     for (uint32_t s = 0; s < c.nslots; ++s) {
-      if (c.slots[s].tile == UINT32_MAX) continue;
-      for (uint32_t h = 0; h < c.H; ++h) te += c.slots[s].ctr[h].te;
+      const SlotDesc &sd = c.slots[s];
+      if (sd.tile == UINT32_MAX) continue;
+      const uint32_t hmax = c.tiles[sd.tile].hmax;
+      ull vis_nz = sd.ctr[0].visnew, r_nz = sd.ctr[0].rnew;
+      for (uint32_t h = 0; h < c.H; ++h) te += sd.ctr[h].te;
+      for (uint32_t h = 1; h <= c.H; ++h) {
+        const HopCounters &pc = sd.ctr[h - 1], &cc = sd.ctr[h];
+        const ull vis_prev = vis_nz, r_prev = r_nz;
+        vis_nz += cc.visnew;
+        r_nz += cc.rnew;
+        if (pc.pop == 0 || h > hmax) continue;
+        bytes += 8 * pc.nz + 8 * cc.nz + 4 * pc.actdeg + 16 * pc.act;
+        if (sd.Vis) bytes += 8 * (vis_prev + vis_nz);
+        else bytes += 8 * (r_prev + r_nz);
+      }
+      bytes += 8 * (c.use_r ? r_nz : vis_nz);
     }
     c.ctl->te += te;
+    c.ctl->bytes += bytes;
   }
 }
 
@@ -459,16 +485,49 @@ __global__ void k_bw_advance(BatchCtl *ctl) {
     default: FN<64>(__VA_ARGS__); break;  \
   }
 
+// Optional HIP events around each persistent expand launch (timing runs,
+// never inside a captured graph).
+struct ExpandEvents {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  ~ExpandEvents() {
+    for (auto &e : ev) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+  }
+  void record(hipStream_t s, bool start) {
+    if (start) {
+      hipEvent_t a = nullptr, b = nullptr;
+      if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+      }
+      ev.emplace_back(a, b);
+      (void)hipEventRecord(a, s);
+    } else if (!ev.empty()) {
+      (void)hipEventRecord(ev.back().second, s);
+    }
+  }
+  double total_ms() const {
+    double t = 0;
+    for (auto &e : ev) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) t += ms;
+      else (void)hipGetLastError();
+    }
+    return t;
+  }
+};
+
 template <int TW>
 static void enqueue_hop_tw(const BatchConst &c, bool shortest, uint32_t hop,
-                           uint32_t grid, hipStream_t s) {
-  if (shortest) {
-    hipLaunchKernelGGL((k_bw_expand<TW, true>), dim3(grid), dim3(kBlock), 0, s, c, hop);
-    hipLaunchKernelGGL((k_bw_epi<TW, true>), dim3(grid), dim3(kBlock), 0, s, c, hop);
-  } else {
-    hipLaunchKernelGGL((k_bw_expand<TW, false>), dim3(grid), dim3(kBlock), 0, s, c, hop);
-    hipLaunchKernelGGL((k_bw_epi<TW, false>), dim3(grid), dim3(kBlock), 0, s, c, hop);
-  }
+                           uint32_t grid, hipStream_t s, ExpandEvents *ev) {
+  if (ev) ev->record(s, true);
+  if (shortest) hipLaunchKernelGGL((k_bw_expand<TW, true>), dim3(grid), dim3(kBlock), 0, s, c, hop);
+  else hipLaunchKernelGGL((k_bw_expand<TW, false>), dim3(grid), dim3(kBlock), 0, s, c, hop);
+  if (ev) ev->record(s, false);
+  if (shortest) hipLaunchKernelGGL((k_bw_epi<TW, true>), dim3(grid), dim3(kBlock), 0, s, c, hop);
+  else hipLaunchKernelGGL((k_bw_epi<TW, false>), dim3(grid), dim3(kBlock), 0, s, c, hop);
   hipLaunchKernelGGL((k_bw_clear<TW>), dim3(grid), dim3(kBlock), 0, s, c, hop - 1);
 }
 
@@ -481,7 +540,7 @@ static void enqueue_final_tw(const BatchConst &c, uint32_t grid, hipStream_t s) 
 // Enqueues the whole launch sequence of one wave; returns the number of
 // kernel launches.
 static uint32_t enqueue_wave(const BatchConst &c, bool shortest, uint32_t grid,
-                             hipStream_t s) {
+                             hipStream_t s, ExpandEvents *ev = nullptr) {
   uint32_t n = 0;
   hipLaunchKernelGGL(k_bw_assign, dim3(1), dim3(256), 0, s, c);
   hipLaunchKernelGGL(k_bw_zero, dim3(grid), dim3(kBlock), 0, s, c);
@@ -495,7 +554,7 @@ static uint32_t enqueue_wave(const BatchConst &c, bool shortest, uint32_t grid,
     hipLaunchKernelGGL(k_bw_decide, dim3(c.nslots), dim3(64), 0, s, c, hop);
     hipLaunchKernelGGL(k_bw_items, dim3(std::max<uint32_t>(1, item_grid)), dim3(kBlock), 0, s, c, hop);
     hipLaunchKernelGGL(k_bw_plan, dim3(1), dim3(64), 0, s, c, hop);
-    VS_TW_BATCH(c.TW, enqueue_hop_tw, c, shortest, hop, grid, s);
+    VS_TW_BATCH(c.TW, enqueue_hop_tw, c, shortest, hop, grid, s, ev);
     n += 6;
   }
   VS_TW_BATCH(c.TW, enqueue_final_tw, c, grid, s);
@@ -506,8 +565,22 @@ static uint32_t enqueue_wave(const BatchConst &c, bool shortest, uint32_t grid,
   return n;
 }
 
+// Owning device buffer.  Move-only: a container that reallocates must move
+// the handle (leaving the source empty), never copy it and free the original.
 struct DevVec {
   void *p = nullptr;
+  DevVec() = default;
+  DevVec(const DevVec &) = delete;
+  DevVec &operator=(const DevVec &) = delete;
+  DevVec(DevVec &&o) noexcept : p(o.p) { o.p = nullptr; }
+  DevVec &operator=(DevVec &&o) noexcept {
+    if (this != &o) {
+      if (p) (void)hipFree(p);
+      p = o.p;
+      o.p = nullptr;
+    }
+    return *this;
+  }
   ~DevVec() { if (p) (void)hipFree(p); }
   vs_status alloc(size_t bytes) {
     if (bytes == 0) bytes = 8;
@@ -621,7 +694,7 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   VS_HIP(hipMemcpyAsync(d_tiles.p, tiles.data(), T * sizeof(TileInfo), hipMemcpyHostToDevice, s));
   // slot buffers (clean: zero once, kept clean by the kernels)
   std::vector<DevVec> bufs;
-  bufs.reserve(nslots * 10);
+  bufs.reserve(nslots * 16);  // 11 buffers per slot; moves are safe anyway
   std::vector<SlotDesc> sd(nslots);
   auto get = [&](size_t bytes, bool zero, void **out) -> vs_status {
     bufs.emplace_back();
@@ -643,6 +716,8 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
     sd[i].R = nullptr;
     if (shortest) { VS_TRY(get(mat, false, &q)); sd[i].Vis = static_cast<uint64_t *>(q); }
     if (use_r) { VS_TRY(get(mat, false, &q)); sd[i].R = static_cast<uint64_t *>(q); }
+    VS_TRY(get((H + 1) * 64 * 8ull, true, &q));
+    sd[i].front = static_cast<uint64_t *>(q);
     VS_TRY(get((H + 1) * sizeof(HopCounters), true, &q));
     sd[i].ctr = static_cast<HopCounters *>(q);
     sd[i].tile = UINT32_MAX;
@@ -694,7 +769,11 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   uint32_t launches = 0;
   double ms = 0.0;
   const uint32_t replays = std::max<uint32_t>(1, p.replays);
-  if (p.use_graph) {
+  const bool timing = (p.flags & VS_VX_TIMING) != 0;  // per-launch events: no capture
+  ExpandEvents xev;
+  double replay_ms = 0.0;
+  uint32_t replay_n = 0;
+  if (p.use_graph && !timing) {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const auto c0 = std::chrono::steady_clock::now();
@@ -724,6 +803,14 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
       for (uint32_t w = 0; w < nwaves && re == hipSuccess; ++w) re = hipGraphLaunch(exec, s);
       if (re == hipSuccess) re = hipEventRecord(e1, s);
       if (re == hipSuccess) re = hipStreamSynchronize(s);
+      float rms = 0.f;
+      if (re == hipSuccess) re = hipEventElapsedTime(&rms, e0, e1);
+      // the first replay warms caches and the instantiated graph: it counts
+      // only when it is the only one
+      if (re == hipSuccess && (rep > 0 || replays == 1)) {
+        replay_ms += rms;
+        replay_n += 1;
+      }
     }
     (void)hipGraphExecDestroy(exec);
     (void)hipGraphDestroy(graph);
@@ -738,16 +825,22 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
         VS_HIP(hipMemsetAsync(d_counts.p, 0, S * 8ull, s));
         if (want_digests) VS_HIP(hipMemsetAsync(d_dig.p, 0, S * 8ull, s));
       }
+      const bool last = rep + 1 == replays;
       VS_HIP(hipEventRecord(e0, s));
-      for (uint32_t w = 0; w < nwaves; ++w) launches = enqueue_wave(c, shortest, grid, s);
+      for (uint32_t w = 0; w < nwaves; ++w)
+        launches = enqueue_wave(c, shortest, grid, s, (timing && last) ? &xev : nullptr);
       VS_HIP(hipGetLastError());
       VS_HIP(hipEventRecord(e1, s));
       VS_HIP(hipStreamSynchronize(s));
     }
+    if (timing) {
+      st.ms_expand += xev.total_ms();
+      st.n_expand += xev.ev.size();
+    }
   }
   float fms = 0.f;
   VS_HIP(hipEventElapsedTime(&fms, e0, e1));
-  ms = fms;
+  ms = replay_n ? replay_ms / replay_n : fms;
   BatchCtl ctl{};
   VS_HIP(hipMemcpy(&ctl, d_ctl.p, sizeof(ctl), hipMemcpyDeviceToHost));
   std::vector<uint64_t> hc(S), hd;
@@ -763,8 +856,9 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   st.num_tiles += T;
   st.num_waves += nwaves;
   st.launches += launches;
-  st.push_decisions += ctl.push_dec / replays;
-  st.pull_decisions += ctl.pull_dec / replays;
+  st.push_decisions += ctl.push_dec;  // of the last replay (counters reset per replay)
+  st.pull_decisions += ctl.pull_dec;
+  st.bytes_alg += ctl.bytes;
   return VS_OK;
 }
 

@@ -294,6 +294,14 @@ __global__ __launch_bounds__(kBlock) void k_to_rowmajor(const uint64_t *__restri
   }
 }
 
+__global__ __launch_bounds__(kBlock) void k_copy_words(uint64_t *__restrict__ dst,
+                                                      const uint64_t *__restrict__ src,
+                                                      size_t n) {
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
 #define VS_TW_DISPATCH(tw, KERNEL, ...)                                      \
   switch (tw) {                                                              \
     case 1: hipLaunchKernelGGL(KERNEL<1>, __VA_ARGS__); break;               \
@@ -390,6 +398,26 @@ vs_status vs_memcpy_h2d(void *dst, const void *src, size_t bytes) {
 vs_status vs_memcpy_d2h(void *dst, const void *src, size_t bytes) {
   clear_error();
   if (bytes) VS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return VS_OK;
+}
+
+vs_status vs_device_copy(void *dst, const void *src, size_t bytes) {
+  clear_error();
+  if ((bytes % 8) || (bytes && (!dst || !src))) {
+    set_error("vs_device_copy: bytes must be a multiple of 8 and pointers non-null");
+    return VS_ERR_INVALID;
+  }
+  if (!bytes) return VS_OK;
+  int dev = 0;
+  VS_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  VS_HIP(hipGetDeviceProperties(&prop, dev));
+  const uint32_t grid = static_cast<uint32_t>(prop.multiProcessorCount) * 8;
+  hipLaunchKernelGGL(k_copy_words, dim3(grid), dim3(kBlock), 0, nullptr,
+                     static_cast<uint64_t *>(dst), static_cast<const uint64_t *>(src),
+                     bytes / 8);
+  VS_HIP(hipGetLastError());
+  VS_HIP(hipDeviceSynchronize());
   return VS_OK;
 }
 

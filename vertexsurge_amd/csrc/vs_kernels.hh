@@ -69,13 +69,23 @@ struct HopArgs {
   const uint64_t *live;      // per-word rows expanding now, or nullptr (rows < S)
   const uint64_t *live_next; // per-word rows expanding next hop, or nullptr
   const uint64_t *racc;      // per-word rows accumulating into R, or nullptr (all)
+  // Rows whose frontier F_{i-1} is non-empty anywhere (OR of all its rows,
+  // per word), or nullptr.  A row outside it cannot gain a bit this hop, so
+  // pull stops asking for it -- without this, rows of isolated sources would
+  // keep every destination's early exit from firing.
+  const uint64_t *front;
+  uint64_t *front_next;      // the same OR for L_i, accumulated here
 };
 
 struct LocalCtr {
   uint64_t pop = 0, nz = 0, act = 0, actdeg = 0, te = 0, visnew = 0, rnew = 0;
+  uint64_t rowor = 0;  // OR of the words this lane emitted (its word index j)
 };
 
-__device__ __forceinline__ void flush_ctr(LocalCtr &c, HopCounters *ctr) {
+__device__ __forceinline__ void flush_ctr(LocalCtr &c, HopCounters *ctr,
+                                          uint64_t *front_next, int j) {
+  if (front_next && c.rowor)
+    atomicOr(reinterpret_cast<ull *>(&front_next[j]), static_cast<ull>(c.rowor));
   const uint64_t pop = wave_sum(c.pop), nz = wave_sum(c.nz), act = wave_sum(c.act),
                  actdeg = wave_sum(c.actdeg), te = wave_sum(c.te),
                  visnew = wave_sum(c.visnew), rnew = wave_sum(c.rnew);
@@ -93,7 +103,7 @@ __device__ __forceinline__ void flush_ctr(LocalCtr &c, HopCounters *ctr) {
 
 // Lane masks for word j of the tile.
 struct LaneMasks {
-  uint64_t live, live_next, racc;
+  uint64_t live, live_next, racc, front;
 };
 
 __device__ __forceinline__ LaneMasks lane_masks(const HopArgs &a, int j) {
@@ -102,6 +112,7 @@ __device__ __forceinline__ LaneMasks lane_masks(const HopArgs &a, int j) {
   m.live = a.live ? a.live[j] : v;
   m.live_next = a.live_next ? a.live_next[j] : v;
   m.racc = a.racc ? a.racc[j] : ~0ull;
+  m.front = a.front ? a.front[j] : ~0ull;
   return m;
 }
 
@@ -124,6 +135,7 @@ __device__ __forceinline__ void emit_word(const HopArgs &a, size_t wi, uint64_t 
   }
   c.pop += __popcll(nb);
   c.nz += 1;
+  c.rowor |= nb;
 }
 
 // ------------------------------------------------------------------- pull
@@ -148,7 +160,7 @@ __device__ __forceinline__ void pull_item(const HopArgs &a, ull item, uint32_t *
   const size_t wi = static_cast<size_t>(v) * TW + j;
   uint64_t vis = 0;
   if (SHORTEST) vis = a.Vis[wi];
-  const uint64_t need = mk.live & ~vis;
+  const uint64_t need = mk.live & mk.front & ~vis;
   if (__ballot(need != 0) == 0) return;
   const uint64_t lb = a.poff[v], le = a.poff[v + 1];
   const bool heavy = le - lb > kPullChunk;
@@ -359,7 +371,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(HopArgs a) {
   LocalCtr c;
   for (uint64_t it = blockIdx.x * kWavesPerBlock + wid; it < a.npitems; it += nwaves)
     pull_item<TW, SHORTEST>(a, a.pitems[it], s_nbr[wid], mk, c);
-  flush_ctr(c, a.ctr);
+  flush_ctr(c, a.ctr, a.front_next, lane_id() % TW);
 }
 
 template <int TW>
@@ -380,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void k_push_epi(HopArgs a) {
   LocalCtr c;
   for (uint32_t vb = blockIdx.x * kWavesPerBlock + wid; vb < nvb; vb += nwaves)
     epi_block<TW, SHORTEST>(a, vb, mk, c);
-  flush_ctr(c, a.ctr);
+  flush_ctr(c, a.ctr, a.front_next, lane_id() % TW);
 }
 
 template <int TW>

@@ -363,6 +363,19 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
   void *d_ctr_v = nullptr;
   VS_TRY(lease.get(nctr * sizeof(HopCounters), true, &d_ctr_v));
   HopCounters *d_ctr = static_cast<HopCounters *>(d_ctr_v);
+  // per (hop, tile) per-word OR of the layer's rows; L0 holds every row
+  void *d_front_v = nullptr;
+  VS_TRY(lease.get(nctr * 64 * 8, true, &d_front_v));
+  uint64_t *d_front = static_cast<uint64_t *>(d_front_v);
+  {
+    std::vector<uint64_t> f0(static_cast<size_t>(T) * 64, 0);
+    for (uint32_t r = 0; r < S; ++r) {
+      const uint32_t w = r / 64;
+      f0[static_cast<size_t>(w / TW) * 64 + (w % TW)] |= 1ull << (r % 64);
+    }
+    VS_HIP(hipMemcpyAsync(d_front, f0.data(), f0.size() * 8, hipMemcpyHostToDevice, s));
+    VS_HIP(hipStreamSynchronize(s));  // f0 leaves scope
+  }
   std::vector<HopCounters> hctr(nctr);
   auto C = [&](uint32_t hop, uint32_t t) -> HopCounters & { return hctr[static_cast<size_t>(hop) * T + t]; };
 
@@ -461,6 +474,8 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
       a.live = nullptr;
       a.live_next = nullptr;
       a.racc = nullptr;
+      a.front = d_front + (static_cast<size_t>(hop - 1) * T + t) * 64;
+      a.front_next = d_front + (static_cast<size_t>(hop) * T + t) * 64;
       hipEvent_t k0 = timing ? evs.make() : nullptr, k1 = timing ? evs.make() : nullptr;
       if (mode == 1) {
         VS_TRY(ensure_push());
