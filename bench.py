@@ -103,7 +103,7 @@ def time_plain(vs, np, G, src, args, k_min, k_max, path, sched, tile_words):
         te = st["te"]
         res.close()
     ms = 1e3 * float(np.mean(times))
-    return {"ms": ms, "te": te, "gteps": te / (ms / 1e3) / 1e9}
+    return {"ms": ms, "te": te, "gteps": te / (ms / 1e3) / 1e9, "tile_words": st["tile_words"]}
 
 
 def load_scheduler(vs, G, path):
@@ -182,7 +182,7 @@ def main() -> int:
 
     # parity of the two legs: per-source counts and digests
     res, pst = vs.vexpand(G, src, args.k_min, args.k_max, args.path, sched=args.sched,
-                          tile_words=args.tile_words, timing=True)
+                          tile_words=plain["tile_words"], timing=True)
     pc, pd = res.row_counts(0), res.row_digests(0)
     res.close()
     legs_exact = bool(np.array_equal(pc, bc) and np.array_equal(pd, bd)

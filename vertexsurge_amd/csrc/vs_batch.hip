@@ -44,6 +44,8 @@ struct SlotDesc {
   uint64_t *front;   // [H + 1][64]: per-word OR of the rows of L_h
   uint32_t tile;     // global tile of the current wave, UINT32_MAX if idle
   int32_t mode;      // this hop: 0 skip, 1 push, 2 pull
+  uint32_t dense;    // this hop: pull without the active-bitmap lookup
+  uint32_t pad;
 };
 
 struct TileInfo {
@@ -101,7 +103,7 @@ __device__ __forceinline__ HopArgs slot_args(const BatchConst &c, const SlotDesc
   a.V = c.V;
   a.S = c.S;
   a.word0 = ti.word0;
-  a.pad = 0;
+  a.dense = sd.dense;
   a.poff = c.poff;
   a.padj = c.padj;
   a.xoff = c.xoff;
@@ -247,6 +249,8 @@ __global__ __launch_bounds__(64) void k_bw_decide(BatchConst c, uint32_t hop) {
   }
   if (lane == 0) {
     sd.mode = mode;
+    // most in-edges come from active vertices: pull skips the bitmap lookup
+    sd.dense = (mode == 2 && sd.ctr[hop - 1].actdeg * 4 > 3 * c.E) ? 1u : 0u;
     if (mode == 1) atomicAdd(&c.ctl->push_dec, 1ull);
     if (mode == 2) atomicAdd(&c.ctl->pull_dec, 1ull);
   }
@@ -294,22 +298,35 @@ __global__ __launch_bounds__(kBlock) void k_bw_expand(BatchConst c, uint32_t hop
   LaneMasks mk;
   int mode = 0;
   LocalCtr lc;
+  const int j = lane_id() % TW;
+  PullHead pre{};
+  ull pre_u = ~0ull;  // unit whose head is prefetched in `pre`
   for (ull u = blockIdx.x * kWavesPerBlock + wid; u < total; u += nwaves) {
     uint32_t s = cur == UINT32_MAX ? 0 : cur;
     while (s + 1 <= c.nslots && c.ctl->unit_prefix[s + 1] <= u) ++s;
     while (c.ctl->unit_prefix[s] > u) --s;
     if (s != cur) {
-      if (cur != UINT32_MAX && mode == 2) flush_ctr(lc, a.ctr, a.front_next, lane_id() % TW);
+      if (cur != UINT32_MAX && mode == 2) flush_ctr(lc, a.ctr, a.front_next, j);
       cur = s;
       a = slot_args(c, c.slots[s], hop);
       mode = c.slots[s].mode;
-      mk = lane_masks(a, lane_id() % TW);
+      mk = lane_masks(a, j);
     }
+    const ull slot_end = c.ctl->unit_prefix[s + 1];
     const ull local = u - c.ctl->unit_prefix[s];
-    if (mode == 2) pull_item<TW, SHORTEST>(a, a.pitems[local], s_nbr[wid], mk, lc);
-    else push_item<TW>(a, a.items[local]);
+    if (mode == 2) {
+      const PullHead h = pre_u == u ? pre : pull_head<TW, SHORTEST>(a, a.pitems[local], j);
+      const ull nu = u + nwaves;  // prefetch the next unit of the same slot
+      if (nu < slot_end) {
+        pre = pull_head<TW, SHORTEST>(a, a.pitems[nu - c.ctl->unit_prefix[s]], j);
+        pre_u = nu;
+      }
+      pull_body<TW, SHORTEST>(a, h, s_nbr[wid], mk, lc);
+    } else {
+      push_item<TW>(a, a.items[local]);
+    }
   }
-  if (cur != UINT32_MAX && mode == 2) flush_ctr(lc, a.ctr, a.front_next, lane_id() % TW);
+  if (cur != UINT32_MAX && mode == 2) flush_ctr(lc, a.ctr, a.front_next, j);
 }
 
 // ---- push epilogue and frontier clear, grid-stride over (slot, vertex block)

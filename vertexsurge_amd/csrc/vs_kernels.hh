@@ -48,7 +48,7 @@ struct HopArgs {
   uint32_t V;
   uint32_t S;      // rows of the tile set (valid-bit mask when live == nullptr)
   uint32_t word0;  // global word index of this tile's word 0
-  uint32_t pad;
+  uint32_t dense;  // frontier covers most edges: pull skips the bitmap lookup
   const uint64_t *poff;  // pull CSR: edges arriving at v
   const uint32_t *padj;
   const uint64_t *xoff;  // push CSR: edges leaving u
@@ -149,23 +149,40 @@ __device__ __forceinline__ void emit_word(const HopArgs &a, size_t wi, uint64_t 
 // A vertex with one item finishes in place (dedup / visit / union); a hub's
 // items OR their partial words into the push accumulator N and mark the
 // touched bitmap, and the push epilogue finishes it.
+// The loads that only depend on the item: issued one item ahead (software
+// pipelining), so their latency overlaps the previous item's gathers.
+struct PullHead {
+  uint32_t v, ch;
+  uint64_t vis, lb, le;
+};
+
 template <int TW, bool SHORTEST>
-__device__ __forceinline__ void pull_item(const HopArgs &a, ull item, uint32_t *nbr,
+__device__ __forceinline__ PullHead pull_head(const HopArgs &a, ull item, int j) {
+  PullHead h;
+  h.v = static_cast<uint32_t>(item);
+  h.ch = static_cast<uint32_t>(item >> 32);
+  h.vis = SHORTEST ? a.Vis[static_cast<size_t>(h.v) * TW + j] : 0;
+  h.lb = a.poff[h.v];
+  h.le = a.poff[h.v + 1];
+  return h;
+}
+
+template <int TW, bool SHORTEST>
+__device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, uint32_t *nbr,
                                           const LaneMasks &mk, LocalCtr &c) {
   constexpr int G = 64 / TW;
   const int lane = lane_id();
   const int g = lane / TW, j = lane % TW;
-  const uint32_t v = static_cast<uint32_t>(item);
-  const uint32_t ch = static_cast<uint32_t>(item >> 32);
+  const uint32_t v = h.v;
   const size_t wi = static_cast<size_t>(v) * TW + j;
-  uint64_t vis = 0;
-  if (SHORTEST) vis = a.Vis[wi];
+  const uint64_t vis = h.vis;
   const uint64_t need = mk.live & mk.front & ~vis;
   if (__ballot(need != 0) == 0) return;
-  const uint64_t lb = a.poff[v], le = a.poff[v + 1];
+  const uint64_t lb = h.lb, le = h.le;
   const bool heavy = le - lb > kPullChunk;
-  const uint64_t b = lb + static_cast<uint64_t>(ch) * kPullChunk;
+  const uint64_t b = lb + static_cast<uint64_t>(h.ch) * kPullChunk;
   const uint64_t e = min(b + kPullChunk, le);
+  const bool dense = a.dense != 0;
   uint64_t acc = 0;
   bool todo = need != 0;
   for (uint64_t base = b; base < e; base += 64) {
@@ -174,7 +191,8 @@ __device__ __forceinline__ void pull_item(const HopArgs &a, ull item, uint32_t *
     bool act = false;
     if (idx < e) {
       u = a.padj[idx];
-      act = (a.actbm[u >> 6] >> (u & 63)) & 1;
+      // a dense frontier skips the bitmap lookup: inactive rows are zero
+      act = dense || ((a.actbm[u >> 6] >> (u & 63)) & 1);
     }
     const uint64_t m = __ballot(act);
     const uint32_t n = __popcll(m);
@@ -182,13 +200,17 @@ __device__ __forceinline__ void pull_item(const HopArgs &a, ull item, uint32_t *
     wave_sync();
     if (todo) {
       uint32_t k = g;
+      for (; k + 7 * G < n; k += 8 * G) {
+        uint64_t x[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) x[q] = a.F[static_cast<size_t>(nbr[k + q * G]) * TW + j];
+        acc |= ((x[0] | x[1]) | (x[2] | x[3])) | ((x[4] | x[5]) | (x[6] | x[7]));
+      }
       for (; k + 3 * G < n; k += 4 * G) {
-        const uint32_t u0 = nbr[k], u1 = nbr[k + G], u2 = nbr[k + 2 * G],
-                       u3 = nbr[k + 3 * G];
-        const uint64_t x0 = a.F[static_cast<size_t>(u0) * TW + j];
-        const uint64_t x1 = a.F[static_cast<size_t>(u1) * TW + j];
-        const uint64_t x2 = a.F[static_cast<size_t>(u2) * TW + j];
-        const uint64_t x3 = a.F[static_cast<size_t>(u3) * TW + j];
+        const uint64_t x0 = a.F[static_cast<size_t>(nbr[k]) * TW + j];
+        const uint64_t x1 = a.F[static_cast<size_t>(nbr[k + G]) * TW + j];
+        const uint64_t x2 = a.F[static_cast<size_t>(nbr[k + 2 * G]) * TW + j];
+        const uint64_t x3 = a.F[static_cast<size_t>(nbr[k + 3 * G]) * TW + j];
         acc |= (x0 | x1) | (x2 | x3);
       }
       for (; k < n; k += G) acc |= a.F[static_cast<size_t>(nbr[k]) * TW + j];
@@ -369,9 +391,20 @@ __global__ __launch_bounds__(kBlock) void k_pull(HopArgs a) {
   const LaneMasks mk = lane_masks(a, lane_id() % TW);
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
   LocalCtr c;
-  for (uint64_t it = blockIdx.x * kWavesPerBlock + wid; it < a.npitems; it += nwaves)
-    pull_item<TW, SHORTEST>(a, a.pitems[it], s_nbr[wid], mk, c);
-  flush_ctr(c, a.ctr, a.front_next, lane_id() % TW);
+  const int j = lane_id() % TW;
+  const uint64_t n = a.npitems;
+  uint64_t it = blockIdx.x * kWavesPerBlock + wid;
+  PullHead h{};
+  if (it < n) h = pull_head<TW, SHORTEST>(a, a.pitems[it], j);
+  while (it < n) {
+    const uint64_t nx = it + nwaves;
+    PullHead hn{};
+    if (nx < n) hn = pull_head<TW, SHORTEST>(a, a.pitems[nx], j);
+    pull_body<TW, SHORTEST>(a, h, s_nbr[wid], mk, c);
+    h = hn;
+    it = nx;
+  }
+  flush_ctr(c, a.ctr, a.front_next, j);
 }
 
 template <int TW>

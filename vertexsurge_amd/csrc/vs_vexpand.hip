@@ -459,7 +459,8 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
       a.V = V;
       a.S = S;
       a.word0 = t * TW;
-      a.pad = 0;
+      // most in-edges come from active vertices: skip the bitmap lookup
+      a.dense = pc.actdeg * 4 > 3 * pull->E ? 1u : 0u;
       a.poff = pull->off;
       a.padj = pull->adj;
       a.xoff = push->off;
