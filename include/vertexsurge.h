@@ -73,6 +73,9 @@ typedef enum {
 
 /* vs_vexpand_params.flags */
 #define VS_VX_TIMING 0x1u /* record per-hop / per-launch HIP event timings */
+/* vs_batch_params.flags: pack rows in input order (default: by source degree;
+ * outputs are in input order either way) */
+#define VS_BATCH_KEEP_ORDER 0x2u
 
 /* Result export layouts. */
 typedef enum {

@@ -98,3 +98,16 @@ def test_batch_learned_scheduler_same_bits(orc, g14):
         assert st["push_decisions"] + st["pull_decisions"] > 0
     finally:
         G.set_scheduler(None)
+
+
+@pytest.mark.parametrize("keep_order", [True, False])
+def test_batch_row_packing_order(orc, g14, keep_order):
+    """Packing rows by k_max and source degree must not change any output:
+    results come back in query order either way."""
+    g, G = g14
+    qs = make_queries(g.V, 9, seed=21, kmax=(2, 6), kmins=(0, 1, 2))
+    for q in qs[1::3]:
+        q.path = "any"
+    c, d, st = vs.vexpand_batch(G, qs, tile_words=4, keep_order=keep_order)
+    ec, ed = expected(orc, g, qs)
+    assert np.array_equal(c, ec) and np.array_equal(d, ed)

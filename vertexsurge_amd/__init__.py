@@ -355,9 +355,10 @@ class BatchQuery:
 def vexpand_batch(graph: Graph, queries: list, tile_words: int = 0, sched: str = "heuristic",
                   use_graph: bool = True, replays: int = 1, mem_budget: int = 0,
                   direction: str = "forward", want_digests: bool = True,
-                  timing: bool = False):
+                  timing: bool = False, keep_order: bool = False):
     """Runs many VExpand queries packed into shared tiles.  Returns
-    (row_counts, row_digests, stats) concatenated over queries."""
+    (row_counts, row_digests, stats) concatenated over queries, in query order.
+    Rows are packed by source degree unless keep_order is set."""
     keep = [_u32(q.sources) for q in queries]
     arr = (_lib.Query * len(queries))()
     for i, q in enumerate(queries):
@@ -370,7 +371,8 @@ def vexpand_batch(graph: Graph, queries: list, tile_words: int = 0, sched: str =
     counts = np.zeros(total, dtype=np.uint64)
     digests = np.zeros(total, dtype=np.uint64) if want_digests else None
     p = _lib.BatchParams(tile_words, _SCHED[sched], int(use_graph), replays, mem_budget,
-                         _DIRS[direction], _lib.VX_TIMING if timing else 0)
+                         _DIRS[direction], (_lib.VX_TIMING if timing else 0) |
+                         (_lib.BATCH_KEEP_ORDER if keep_order else 0))
     st = _lib.BatchStats()
     _lib.call("vs_vexpand_batch", graph.handle, arr, len(queries), C.byref(p),
               _ptr(counts, C.c_uint64),

@@ -23,6 +23,7 @@ DIR_FORWARD, DIR_BACKWARD, DIR_BOTH = 0, 1, 2
 SCHED_HEURISTIC, SCHED_PUSH, SCHED_PULL, SCHED_LEARNED = 0, 1, 2, 3
 LAYOUT_NATIVE, LAYOUT_DVSM, LAYOUT_ROWMAJOR = 0, 1, 2
 VX_TIMING = 0x1
+BATCH_KEEP_ORDER = 0x2
 GRAPH_SYMMETRIC = 0x1
 MAX_HOPS = 64
 MLP_IN, MLP_HIDDEN, MLP_OUT = 8, 64, 8

@@ -91,6 +91,7 @@ struct BatchConst {
   ull *out_counts;        // [S] (group order), zeroed before the first wave
   ull *out_digests;       // [S] or nullptr
   const float *mlp;       // scheduler parameters or nullptr
+  const uint32_t *hubcnt; // hub neighbours at the head of each pull list
   const ull *pitems;      // static pull items of the pull CSR
   uint64_t npitems;
   uint32_t has_heavy;     // some in-list spans several pull items
@@ -123,6 +124,7 @@ __device__ __forceinline__ HopArgs slot_args(const BatchConst &c, const SlotDesc
   a.live_next = c.live + static_cast<size_t>(hop + 1) * c.Wpad + ti.word0;
   a.racc = c.racc ? c.racc + static_cast<size_t>(hop) * c.Wpad + ti.word0 : nullptr;
   a.pitems = c.pitems;
+  a.hubcnt = c.hubcnt;
   a.npitems = c.npitems;
   a.front = sd.front + static_cast<size_t>(hop - 1) * 64;
   a.front_next = sd.front + static_cast<size_t>(hop) * 64;
@@ -654,6 +656,40 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
       }
     }
   }
+  // Row packing: rows are independent, so they may share words in any order.
+  // Rows are grouped by k_max first (a tile runs as many hops as its deepest
+  // row), then ordered by source degree (descending, isolated sources last),
+  // which puts rows that finish at similar hops into the same words: words of
+  // isolated sources are dead after L0 (their lanes never load again), and
+  // words of central sources become fully visited early (their lanes stop
+  // early).  pos[r] is the packed position of group row r.
+  std::vector<uint32_t> pos(S);
+  {
+    std::vector<uint32_t> order(S);
+    for (uint32_t r = 0; r < S; ++r) order[r] = r;
+    if (!(p.flags & VS_BATCH_KEEP_ORDER)) {
+      const std::vector<uint64_t> &xo = push->hoff;
+      std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+        if (kmax[x] != kmax[y]) return kmax[x] > kmax[y];
+        const uint64_t dx = xo[src[x] + 1] - xo[src[x]], dy = xo[src[y] + 1] - xo[src[y]];
+        return dx > dy;
+      });
+    }
+    std::vector<uint32_t> s2(S), kn(S), kx(S);
+    std::vector<uint8_t> sb(S);
+    for (uint32_t i = 0; i < S; ++i) {
+      const uint32_t r = order[i];
+      pos[r] = i;
+      s2[i] = src[r];
+      kn[i] = kmin[r];
+      kx[i] = kmax[r];
+      sb[i] = sub[r];
+    }
+    src.swap(s2);
+    kmin.swap(kn);
+    kmax.swap(kx);
+    sub.swap(sb);
+  }
   std::vector<uint64_t> live(static_cast<size_t>(H + 2) * Wpad, 0);
   std::vector<uint64_t> racc;
   if (use_r) racc.assign(static_cast<size_t>(H + 1) * Wpad, 0);
@@ -774,6 +810,8 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   c.mlp = (p.sched == VS_SCHED_LEARNED && g->sched) ? sched_params(g->sched) : nullptr;
   VS_TRY(ensure_pull_items(*const_cast<DevCSR *>(pull), V));
   c.pitems = pull->pitems;
+  c.padj = pull->hadj;  // hubs-first lists (pull only; push uses xadj)
+  c.hubcnt = pull->hubcnt;
   c.npitems = pull->npitems;
   c.has_heavy = pull->has_heavy ? 1 : 0;
   const uint32_t grid = static_cast<uint32_t>(g->num_cus) * 8;
@@ -866,8 +904,10 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
     hd.resize(S);
     VS_HIP(hipMemcpy(hd.data(), d_dig.p, S * 8ull, hipMemcpyDeviceToHost));
   }
-  counts.insert(counts.end(), hc.begin(), hc.end());
-  if (want_digests) digests.insert(digests.end(), hd.begin(), hd.end());
+  // back from packed positions to group row order
+  for (uint32_t r = 0; r < S; ++r) counts.push_back(hc[pos[r]]);
+  if (want_digests)
+    for (uint32_t r = 0; r < S; ++r) digests.push_back(hd[pos[r]]);
   st.te += ctl.te;
   st.ms_total += ms;
   st.num_tiles += T;

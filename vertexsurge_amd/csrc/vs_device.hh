@@ -66,7 +66,16 @@ struct DevCSR {
   unsigned long long *pitems = nullptr;
   uint64_t npitems = 0;
   bool has_heavy = false;     // some list longer than kPullChunk
+  // Pull copy of the lists with the hub neighbours first (hubs = the
+  // kPullHubs vertices that appear in the most lists, i.e. whose frontier
+  // rows are gathered most often), each part ascending, and per-vertex hub
+  // counts.  Pull gathers hub rows with cached loads and the rest with
+  // streaming loads, so the rows reused across destinations stay in L2.
+  uint32_t *hadj = nullptr;
+  uint32_t *hubcnt = nullptr;
 };
+
+constexpr uint32_t kPullHubs = 8192;
 
 // Builds the pull items of a CSR once (vertex | chunk << 32 per kPullChunk
 // in-edges, vertices without edges omitted).

@@ -12,6 +12,7 @@
 // (r / 1024) * v_per_stack + c * 16 + (r % 1024) / 64).
 #include <algorithm>
 #include <cstring>
+#include <thread>
 
 #include "vs_device.hh"
 
@@ -78,7 +79,64 @@ static void free_csr(DevCSR &d) {
     (void)hipFree(d.adj);
   }
   if (d.pitems) (void)hipFree(d.pitems);
+  if (d.hadj) (void)hipFree(d.hadj);
+  if (d.hubcnt) (void)hipFree(d.hubcnt);
   d = DevCSR{};
+}
+
+// Hubs-first pull lists (see DevCSR::hadj), built on the host in parallel.
+static vs_status build_hub_lists(DevCSR &c, uint32_t V) {
+  const uint64_t E = c.E;
+  std::vector<uint32_t> adj(E);
+  if (E) VS_HIP(hipMemcpy(adj.data(), c.adj, E * 4, hipMemcpyDeviceToHost));
+  const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  // appearances of each vertex over all lists
+  std::vector<std::vector<uint32_t>> part(nt);
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nt; ++t)
+    th.emplace_back([&, t]() {
+      std::vector<uint32_t> &h = part[t];
+      h.assign(V, 0);
+      for (uint64_t k = E * t / nt; k < E * (t + 1) / nt; ++k) h[adj[k]]++;
+    });
+  for (auto &x : th) x.join();
+  th.clear();
+  std::vector<uint64_t> cnt(V, 0);
+  for (unsigned t = 0; t < nt; ++t)
+    for (uint32_t v = 0; v < V; ++v) cnt[v] += part[t][v];
+  std::vector<std::vector<uint32_t>>().swap(part);
+  std::vector<uint32_t> order(V);
+  for (uint32_t v = 0; v < V; ++v) order[v] = v;
+  const uint32_t K = std::min<uint32_t>(kPullHubs, V);
+  std::nth_element(order.begin(), order.begin() + K, order.end(), [&](uint32_t x, uint32_t y) {
+    return cnt[x] != cnt[y] ? cnt[x] > cnt[y] : x < y;
+  });
+  std::vector<uint8_t> hub(V, 0);
+  for (uint32_t i = 0; i < K; ++i) hub[order[i]] = 1;
+  std::vector<uint32_t> hcnt(V, 0);
+  for (unsigned t = 0; t < nt; ++t)
+    th.emplace_back([&, t]() {
+      for (uint64_t v = static_cast<uint64_t>(V) * t / nt; v < static_cast<uint64_t>(V) * (t + 1) / nt; ++v) {
+        uint32_t *b = adj.data() + c.hoff[v], *e = adj.data() + c.hoff[v + 1];
+        uint32_t *mid = std::stable_partition(b, e, [&](uint32_t u) { return hub[u] != 0; });
+        hcnt[v] = static_cast<uint32_t>(mid - b);
+      }
+    });
+  for (auto &x : th) x.join();
+  void *ph = nullptr, *pc = nullptr;
+  VS_HIP(hipMalloc(&ph, std::max<uint64_t>(E, 1) * 4));
+  hipError_t e = hipMalloc(&pc, std::max<uint32_t>(V, 1) * 4ull);
+  if (e == hipSuccess && E) e = hipMemcpy(ph, adj.data(), E * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(pc, hcnt.data(), V * 4ull, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(ph);
+    (void)hipFree(pc);
+    set_error("hub lists upload: %s", hipGetErrorString(e));
+    return VS_ERR_HIP;
+  }
+  c.hadj = static_cast<uint32_t *>(ph);
+  c.hubcnt = static_cast<uint32_t *>(pc);
+  return VS_OK;
 }
 
 vs_status ensure_pull_items(DevCSR &c, uint32_t V) {
@@ -105,7 +163,7 @@ vs_status ensure_pull_items(DevCSR &c, uint32_t V) {
   c.pitems = static_cast<unsigned long long *>(p);
   c.npitems = it.size();
   c.has_heavy = heavy;
-  return VS_OK;
+  return build_hub_lists(c, V);
 }
 
 vs_status graph_csrs(vs_graph_s *g, int direction, const DevCSR **push,

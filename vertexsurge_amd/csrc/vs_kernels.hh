@@ -65,6 +65,7 @@ struct HopArgs {
   const ull *nitems;
   const ull *pitems;     // pull items (static per CSR)
   uint64_t npitems;
+  const uint32_t *hubcnt; // hub neighbours at the head of each pull list, or nullptr
   HopCounters *ctr;      // counters of L_i
   const uint64_t *live;      // per-word rows expanding now, or nullptr (rows < S)
   const uint64_t *live_next; // per-word rows expanding next hop, or nullptr
@@ -154,6 +155,7 @@ __device__ __forceinline__ void emit_word(const HopArgs &a, size_t wi, uint64_t 
 struct PullHead {
   uint32_t v, ch;
   uint64_t vis, lb, le;
+  uint32_t hubs;  // hub neighbours at the head of v's pull list
 };
 
 template <int TW, bool SHORTEST>
@@ -164,7 +166,38 @@ __device__ __forceinline__ PullHead pull_head(const HopArgs &a, ull item, int j)
   h.vis = SHORTEST ? a.Vis[static_cast<size_t>(h.v) * TW + j] : 0;
   h.lb = a.poff[h.v];
   h.le = a.poff[h.v + 1];
+  h.hubs = a.hubcnt ? a.hubcnt[h.v] : 0xffffffffu;
   return h;
+}
+
+// Frontier word of neighbour u: hub rows stay cached, other rows stream.
+template <int TW, bool CACHED>
+__device__ __forceinline__ uint64_t gather(const HopArgs &a, uint32_t u, int j) {
+  const uint64_t *p = a.F + static_cast<size_t>(u) * TW + j;
+  if constexpr (CACHED) return *p;
+  else return __builtin_nontemporal_load(p);
+}
+
+template <int TW, bool CACHED>
+__device__ __forceinline__ uint64_t gather_range(const HopArgs &a, const uint32_t *nbr,
+                                                 uint32_t &k, uint32_t end, int j) {
+  constexpr int G = 64 / TW;
+  uint64_t acc = 0;
+  for (; k + 7 * G < end; k += 8 * G) {
+    uint64_t x[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) x[q] = gather<TW, CACHED>(a, nbr[k + q * G], j);
+    acc |= ((x[0] | x[1]) | (x[2] | x[3])) | ((x[4] | x[5]) | (x[6] | x[7]));
+  }
+  for (; k + 3 * G < end; k += 4 * G) {
+    const uint64_t x0 = gather<TW, CACHED>(a, nbr[k], j);
+    const uint64_t x1 = gather<TW, CACHED>(a, nbr[k + G], j);
+    const uint64_t x2 = gather<TW, CACHED>(a, nbr[k + 2 * G], j);
+    const uint64_t x3 = gather<TW, CACHED>(a, nbr[k + 3 * G], j);
+    acc |= (x0 | x1) | (x2 | x3);
+  }
+  for (; k < end; k += G) acc |= gather<TW, CACHED>(a, nbr[k], j);
+  return acc;
 }
 
 template <int TW, bool SHORTEST>
@@ -183,6 +216,7 @@ __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, u
   const uint64_t b = lb + static_cast<uint64_t>(h.ch) * kPullChunk;
   const uint64_t e = min(b + kPullChunk, le);
   const bool dense = a.dense != 0;
+  const uint64_t hub_end = h.hubs == 0xffffffffu ? le : lb + h.hubs;
   uint64_t acc = 0;
   bool todo = need != 0;
   for (uint64_t base = b; base < e; base += 64) {
@@ -196,24 +230,15 @@ __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, u
     }
     const uint64_t m = __ballot(act);
     const uint32_t n = __popcll(m);
+    // hub neighbours precede the others in the list, so after compaction the
+    // first nh slots are hubs
+    const uint32_t nh = __popcll(__ballot(act && idx < hub_end));
     if (act) nbr[lane_rank(m)] = u;
     wave_sync();
     if (todo) {
       uint32_t k = g;
-      for (; k + 7 * G < n; k += 8 * G) {
-        uint64_t x[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) x[q] = a.F[static_cast<size_t>(nbr[k + q * G]) * TW + j];
-        acc |= ((x[0] | x[1]) | (x[2] | x[3])) | ((x[4] | x[5]) | (x[6] | x[7]));
-      }
-      for (; k + 3 * G < n; k += 4 * G) {
-        const uint64_t x0 = a.F[static_cast<size_t>(nbr[k]) * TW + j];
-        const uint64_t x1 = a.F[static_cast<size_t>(nbr[k + G]) * TW + j];
-        const uint64_t x2 = a.F[static_cast<size_t>(nbr[k + 2 * G]) * TW + j];
-        const uint64_t x3 = a.F[static_cast<size_t>(nbr[k + 3 * G]) * TW + j];
-        acc |= (x0 | x1) | (x2 | x3);
-      }
-      for (; k < n; k += G) acc |= a.F[static_cast<size_t>(nbr[k]) * TW + j];
+      acc |= gather_range<TW, true>(a, nbr, k, nh, j);
+      acc |= gather_range<TW, false>(a, nbr, k, n, j);
     }
     wave_sync();
     acc = group_or<TW>(acc);

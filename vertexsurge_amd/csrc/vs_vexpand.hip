@@ -492,6 +492,8 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
       } else {
         VS_TRY(ensure_pull_items(*const_cast<DevCSR *>(pull), V));
         a.pitems = pull->pitems;
+        a.padj = pull->hadj;  // hubs-first lists
+        a.hubcnt = pull->hubcnt;
         a.npitems = pull->npitems;
         a.items = nullptr;
         a.nitems = nullptr;

@@ -124,6 +124,48 @@ TRAIN_SPECS = [{"kind": "rmat", "scale": sc, "ef": 16, "a": a, "seed": 100 + i}
                                             for a in (0.45, 0.5, 0.55, 0.6)])]
 HELDOUT_SPECS = [{"kind": "rmat", "scale": 22, "ef": 16, "a": 0.57, "seed": 7},
                  {"kind": "er", "V": 1 << 22, "pairs": 16 << 22, "seed": 7}]
+LARGE_SPECS = [{"kind": "rmat", "scale": sc, "ef": 32, "a": a, "seed": 200 + i}
+               for i, (sc, a) in enumerate([(s, a) for s in (20, 21, 22)
+                                            for a in (0.45, 0.5, 0.55, 0.6)])]
+LARGE_HELDOUT = [{"kind": "rmat", "scale": 23, "ef": 32, "a": 0.57, "seed": 77},
+                 {"kind": "er", "V": 1 << 23, "pairs": 32 << 23, "seed": 77}]
+
+
+def evaluate(s: Scheduler, specs, sources=(1024, 4096), k_max=4, reps=3, log=print):
+    """Held-out comparison on whole queries: device time with the learned
+    scheduler vs the density heuristic (median of reps), bits compared."""
+    out = []
+    for spec in specs:
+        csr = make_graph(spec)
+        G = Graph.from_csr(csr)
+        G.set_scheduler(s)
+        for qi, S in enumerate(sources):
+            src = sample_sources(csr.V, min(S, csr.V), 900 + qi)
+            t = {}
+            res = {}
+            for sched in ("heuristic", "learned", "heuristic", "learned"):
+                r, st = vexpand(G, src, 1, k_max, sched=sched)
+                res[sched] = (r.row_counts(0), r.row_digests(0), st["te"])
+                r.close()
+            for sched in ("heuristic", "learned"):
+                ms = []
+                for _ in range(reps):
+                    r, st = vexpand(G, src, 1, k_max, sched=sched)
+                    ms.append(st["ms_total"])
+                    r.close()
+                t[sched] = float(np.median(ms))
+            same = all(np.array_equal(a, b) for a, b in
+                       zip(res["heuristic"][:2], res["learned"][:2])) and \
+                res["heuristic"][2] == res["learned"][2]
+            row = {"graph": spec, "sources": int(S), "ms_heuristic": t["heuristic"],
+                   "ms_learned": t["learned"], "bit_exact": bool(same),
+                   "te": int(res["learned"][2])}
+            log(json.dumps(row))
+            out.append(row)
+        G.set_scheduler(None)
+        G.close()
+        csr.close()
+    return out
 
 
 def main(argv=None):
@@ -131,9 +173,19 @@ def main(argv=None):
     ap.add_argument("--out", default=str(WEIGHTS))
     ap.add_argument("--epochs", type=int, default=2000)
     ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--large", action="store_true",
+                    help="train on R-MAT 20-22 (EF 32), evaluate on held-out R-MAT 23 and an "
+                         "Erdos-Renyi graph of the same size")
+    ap.add_argument("--eval-only", action="store_true", help="evaluate the saved weights")
     args = ap.parse_args(argv)
     log = lambda m: print(f"[sched_train] {m}", file=sys.stderr, flush=True)
-    specs = TRAIN_SPECS[::4] if args.quick else TRAIN_SPECS
+    if args.eval_only:
+        s = load(Path(args.out))
+        held = LARGE_HELDOUT if args.large else HELDOUT_SPECS
+        rows = evaluate(s, held, log=log)
+        print(json.dumps({"heldout": rows}))
+        return
+    specs = LARGE_SPECS if args.large else (TRAIN_SPECS[::4] if args.quick else TRAIN_SPECS)
     X, Y, M = collect(specs, log=log)
     log(f"training on {len(X)} samples")
     s = train(X, Y, M, epochs=args.epochs)
@@ -142,8 +194,10 @@ def main(argv=None):
     log(f"train accuracy {acc}")
     save(s, Path(args.out), {"samples": int(len(X)), "train_acc": acc,
                              "specs": specs, "epochs": args.epochs})
+    held = LARGE_HELDOUT if args.large else HELDOUT_SPECS
+    rows = evaluate(s, held, log=log)
     print(json.dumps({"samples": int(len(X)), "final_loss": float(s.curve[-1]),
-                      "train": acc}))
+                      "train": acc, "heldout": rows}))
 
 
 if __name__ == "__main__":
