@@ -345,6 +345,90 @@ vs_status vs_sched_predict(vs_sched_t s, const float *X, uint32_t n, float *Y);
 /* Attach (or detach with NULL) a scheduler used by VS_SCHED_LEARNED. */
 vs_status vs_graph_set_scheduler(vs_graph_t g, vs_sched_t s);
 
+/* ------------------------------------------------ bit-matrix tables (ColMain) */
+/* A device-resident DenseColMainMatrix with its table maps
+ * (bitmap.hh:262-326, bitmap.cpp:580-595; ColMainMatrixTable,
+ * matrix_table.hh:85-125): `rows` x `cols` bits, column c at words
+ * [c*pitch, (c+1)*pitch), pitch = align8(ceil(rows/64)), row r at bit r%64 of
+ * word r/64 -- the reference word layout, so export is a plain copy.  Bits of
+ * rows >= `rows` are zero.  The optional row map gives the vertex id of each
+ * row, the optional column map the vertex id of each column
+ * (MatrixTable::row_map/col_map; a column lookup by id takes the first column
+ * holding that id, as set_col_hash_map's insert does, bit_matrix_table.cpp:26-37).
+ * The matrix lives on the device of `g` and is ordered on its stream. */
+typedef struct vs_colmat_s *vs_colmat_t;
+
+/* words == NULL: all zeros; else cols*pitch words in the layout above
+ * (bits of rows >= `rows` are cleared on upload). */
+vs_status vs_colmat_create(vs_graph_t g, uint32_t rows, uint32_t cols,
+                           const uint64_t *words, vs_colmat_t *out);
+/* row_map: `rows` ids or NULL (no map).  col_map: n_col_map <= cols ids or
+ * NULL (n_col_map == 0 also means no map); columns past n_col_map are not
+ * reachable by id (the reference test MintersectFlatMatMat maps 8 of 10
+ * columns, mintersect.cpp:433-457). */
+vs_status vs_colmat_set_maps(vs_colmat_t m, const uint32_t *row_map,
+                             const uint32_t *col_map, uint32_t n_col_map);
+/* *n_col_map = 0 when there is no column map */
+vs_status vs_colmat_info(vs_colmat_t m, uint32_t *rows, uint32_t *cols,
+                         uint64_t *pitch_words, int *has_row_map,
+                         uint32_t *n_col_map);
+/* row_map / col_map copies (NULL to skip; rows / n_col_map ids) */
+vs_status vs_colmat_get_maps(vs_colmat_t m, uint32_t *row_map, uint32_t *col_map);
+vs_status vs_colmat_export(vs_colmat_t m, uint64_t *words, size_t bytes);
+vs_status vs_colmat_get(vs_colmat_t m, uint32_t row, uint32_t col, int *bit);
+vs_status vs_colmat_free(vs_colmat_t m);
+
+/* TableConvert VarRowMatrix -> ColMainMatrix (TableConvert::run_internal,
+ * big_operators2.cpp:16-86) of matrix i of a VExpand result.  The maps come
+ * along (take_maps_from): the result's row map is its source list
+ * (vexpand.cpp:351-353).  transpose == 0: rows = sources, cols = vertices,
+ * column v = the rows that reach v.  transpose != 0: rows = vertices,
+ * cols = sources (row and column maps swapped), column r = the vertex set of
+ * source row r. */
+vs_status vs_table_convert(vs_result_t r, uint32_t i, int transpose,
+                           vs_colmat_t *out);
+
+/* MatrixTableSelect (big_operators2.cpp:138-213): a new single-matrix result
+ * whose column k is column col_ids[k] of matrix i (a vertex id of the
+ * result), with col_ids as its column map and the same row map. */
+vs_status vs_result_select_cols(vs_result_t r, uint32_t i, const uint32_t *col_ids,
+                                uint32_t n, vs_result_t *out);
+/* column map of a result made by vs_result_select_cols (cols ids); a VExpand
+ * result has none (VS_ERR_INVALID) */
+vs_status vs_result_col_map(vs_result_t r, uint32_t *col_ids);
+
+typedef enum { VS_ORDER_NONE = 0, VS_ORDER_ASC = 1, VS_ORDER_DESC = 2 } vs_order;
+
+/* MatrixColumnCount (matrix_column_count.cpp:19-87): for every column, its id
+ * (through the column map if any) and its number of set bits.  VS_ORDER_NONE
+ * gives columns in index order; ASC / DESC sort by (count, id) as the
+ * reference's pair sort does.  ids and counts hold `cols` entries. */
+vs_status vs_result_col_counts(vs_result_t r, uint32_t i, int32_t order,
+                               uint32_t *ids, int32_t *counts);
+vs_status vs_colmat_col_counts(vs_colmat_t m, int32_t order, uint32_t *ids,
+                               int32_t *counts);
+/* MatrixRowOrSum (matrix_row_or_sum.cpp:14-55): ids of the columns with any
+ * set bit, in column index order.  *n gets the number; ids (cap entries) may
+ * be NULL for a size query; VS_ERR_CAPACITY if cap < *n. */
+vs_status vs_result_row_or(vs_result_t r, uint32_t i, uint32_t *ids, uint32_t cap,
+                           uint32_t *n);
+vs_status vs_colmat_row_or(vs_colmat_t m, uint32_t *ids, uint32_t cap, uint32_t *n);
+
+/* Mintersect with ColMainMatrix dependencies (Mintersect::run_internal
+ * flat-base branch, mintersect.cpp:205-243, with tackle_di_for_base_vid's
+ * ColMain case, mintersect.cpp:139-156): for base tuple t,
+ *   I_t = AND_d column(deps[d], tuples[t*nd + d])
+ * where the column is found through the dependency's column map (or is the
+ * id itself when it has none), and a missing column makes I_t empty.  All
+ * dependencies have the same row count.  counts[t] = |I_t|; with lists, the
+ * rows of I_t in ascending row order, through deps[0]'s row map when it has
+ * one (mintersect.cpp:115-122, 228-241), at lists[offsets[t] ..
+ * offsets[t+1]).  Same two-phase contract as vs_mintersect. */
+vs_status vs_mintersect_colmat(const vs_colmat_t *deps, uint32_t nd,
+                               const uint32_t *tuples, uint64_t T, uint32_t *counts,
+                               uint64_t *offsets, uint32_t *lists, uint64_t list_cap,
+                               uint64_t *total, vs_mintersect_stats *stats);
+
 /* -------------------------------------------------------------- autotuner */
 /* Times candidate tile widths and push thresholds on the given sources and
  * stores the fastest in the graph's tuning. */

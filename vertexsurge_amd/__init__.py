@@ -18,6 +18,7 @@ from ._lib import (DIR_BACKWARD, DIR_BOTH, DIR_FORWARD, LAYOUT_DVSM, LAYOUT_NATI
 
 __all__ = [
     "HostCSR", "Graph", "VExpandResult", "vexpand", "find_shortest", "mintersect",
+    "ColMat", "mintersect_colmat", "pack_colmain", "unpack_colmain",
     "sample_sources", "sample_paths3", "Scheduler", "vexpand_batch", "device_count",
     "PATH_ANY", "PATH_SHORTEST", "PATH_FIND_SHORTEST", "DIR_FORWARD", "DIR_BACKWARD",
     "DIR_BOTH", "SCHED_HEURISTIC", "SCHED_PUSH", "SCHED_PULL", "SCHED_LEARNED",
@@ -225,6 +226,39 @@ class VExpandResult:
         _lib.call("vs_result_get", self._h, i, row, col, C.byref(b))
         return bool(b.value)
 
+    def col_map(self):
+        """Vertex ids of the columns of a column select; None for VExpand output."""
+        if not getattr(self, "_selected", False):
+            return None
+        out = np.zeros(self.cols, dtype=np.uint32)
+        if self.cols:
+            _lib.call("vs_result_col_map", self._h, _ptr(out, C.c_uint32))
+        return out
+
+    def to_colmain(self, i: int = 0, transpose: bool = False) -> "ColMat":
+        """TableConvert VarRow -> ColMain (maps travel; transpose swaps them)."""
+        h = C.c_void_p()
+        _lib.call("vs_table_convert", self._h, i, int(transpose), C.byref(h))
+        return ColMat(h)
+
+    def select_cols(self, col_ids, i: int = 0) -> "VExpandResult":
+        """MatrixTableSelect: a new result with columns col_ids (vertex ids)."""
+        ids = _u32(col_ids)
+        h = C.c_void_p()
+        _lib.call("vs_result_select_cols", self._h, i, _ptr(ids, C.c_uint32), len(ids),
+                  C.byref(h))
+        r = VExpandResult(h, self.row_map)
+        r._selected = True
+        return r
+
+    def col_counts(self, i: int = 0, order: str | None = None):
+        """MatrixColumnCount: (ids, counts) per column; order None/'asc'/'desc'."""
+        return _col_counts("vs_result_col_counts", (self._h, i), self.cols, order)
+
+    def row_or(self, i: int = 0) -> np.ndarray:
+        """MatrixRowOrSum: ids of the columns with any set bit."""
+        return _row_or("vs_result_row_or", (self._h, i))
+
     def close(self):
         if self._h:
             _lib.call("vs_result_free", self._h)
@@ -282,6 +316,167 @@ def mintersect(deps: list, tuples, with_lists: bool = True):
     if with_lists:
         lists = np.zeros(max(total.value, 1), dtype=np.uint32)
         _lib.call("vs_mintersect", arr, arity, _ptr(tup, C.c_uint32), T,
+                  _ptr(counts, C.c_uint32), _ptr(offsets, C.c_uint64),
+                  _ptr(lists, C.c_uint32), len(lists), C.byref(total), C.byref(st))
+        lists = lists[:total.value]
+    return counts, offsets, lists, {"ms": st.ms, "total": st.total, "bytes_alg": st.bytes_alg}
+
+
+_ORDERS = {None: _lib.ORDER_NONE, "asc": _lib.ORDER_ASC, "desc": _lib.ORDER_DESC}
+
+
+def _col_counts(fn, head, n, order):
+    ids = np.zeros(n, dtype=np.uint32)
+    counts = np.zeros(n, dtype=np.int32)
+    _lib.call(fn, *head, _ORDERS[order], _ptr(ids, C.c_uint32), _ptr(counts, C.c_int32))
+    return ids, counts
+
+
+def _row_or(fn, head):
+    n = C.c_uint32()
+    _lib.call(fn, *head, None, 0, C.byref(n))
+    ids = np.zeros(max(n.value, 1), dtype=np.uint32)
+    _lib.call(fn, *head, _ptr(ids, C.c_uint32), len(ids), C.byref(n))
+    return ids[:n.value]
+
+
+class ColMat:
+    """Device DenseColMainMatrix + row/column maps (ColMainMatrixTable)."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+        rows, cols, ncm = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        pitch = C.c_uint64()
+        hr = C.c_int()
+        _lib.call("vs_colmat_info", handle, C.byref(rows), C.byref(cols), C.byref(pitch),
+                  C.byref(hr), C.byref(ncm))
+        self.rows, self.cols, self.pitch = rows.value, cols.value, pitch.value
+        self.has_row_map, self.n_col_map = bool(hr.value), ncm.value
+
+    @classmethod
+    def create(cls, graph: Graph, rows: int, cols: int, words=None, row_map=None,
+               col_map=None) -> "ColMat":
+        """words: cols x pitch uint64 (pitch = align8(ceil(rows/64))) or None."""
+        w = None
+        if words is not None:
+            w = np.ascontiguousarray(words, dtype=np.uint64)
+            pitch = (((rows + 63) // 64) + 7) // 8 * 8
+            if w.size != cols * pitch:
+                raise ValueError(f"words: {w.size} words, expected cols x pitch = {cols * pitch}")
+        h = C.c_void_p()
+        _lib.call("vs_colmat_create", graph.handle, rows, cols,
+                  _ptr(w, C.c_uint64) if w is not None else None, C.byref(h))
+        m = cls(h)
+        if row_map is not None or col_map is not None:
+            m.set_maps(row_map, col_map)
+        return m
+
+    @classmethod
+    def from_bool(cls, graph: Graph, M, row_map=None, col_map=None) -> "ColMat":
+        """From a rows x cols boolean array."""
+        M = np.asarray(M, dtype=bool)
+        return cls.create(graph, M.shape[0], M.shape[1], pack_colmain(M), row_map, col_map)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_maps(self, row_map=None, col_map=None):
+        rm = _u32(row_map) if row_map is not None else None
+        cm = _u32(col_map) if col_map is not None else None
+        if rm is not None and len(rm) != self.rows:
+            raise ValueError("row map length differs from the row count")
+        if cm is not None and len(cm) > self.cols:
+            raise ValueError("column map longer than the column count")
+        _lib.call("vs_colmat_set_maps", self._h,
+                  _ptr(rm, C.c_uint32) if rm is not None else None,
+                  _ptr(cm, C.c_uint32) if cm is not None else None,
+                  len(cm) if cm is not None else 0)
+        self.has_row_map = rm is not None
+        self.n_col_map = len(cm) if cm is not None else 0
+
+    @property
+    def has_col_map(self) -> bool:
+        return self.n_col_map > 0
+
+    def maps(self):
+        rm = np.zeros(self.rows, dtype=np.uint32) if self.has_row_map else None
+        cm = np.zeros(self.n_col_map, dtype=np.uint32) if self.has_col_map else None
+        _lib.call("vs_colmat_get_maps", self._h,
+                  _ptr(rm, C.c_uint32) if rm is not None else None,
+                  _ptr(cm, C.c_uint32) if cm is not None else None)
+        return rm, cm
+
+    def export(self) -> np.ndarray:
+        """cols x pitch uint64 words, the reference DenseColMainMatrix layout."""
+        out = np.zeros((self.cols, self.pitch), dtype=np.uint64)
+        if out.size:
+            _lib.call("vs_colmat_export", self._h, _ptr(out, C.c_uint64), out.nbytes)
+        return out
+
+    def to_bool(self) -> np.ndarray:
+        return unpack_colmain(self.export(), self.rows)
+
+    def get(self, row: int, col: int) -> bool:
+        b = C.c_int()
+        _lib.call("vs_colmat_get", self._h, row, col, C.byref(b))
+        return bool(b.value)
+
+    def col_counts(self, order: str | None = None):
+        return _col_counts("vs_colmat_col_counts", (self._h,), self.cols, order)
+
+    def row_or(self) -> np.ndarray:
+        return _row_or("vs_colmat_row_or", (self._h,))
+
+    def close(self):
+        if self._h:
+            _lib.call("vs_colmat_free", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def pack_colmain(M) -> np.ndarray:
+    """rows x cols bool -> cols x align8(ceil(rows/64)) uint64 words."""
+    M = np.asarray(M, dtype=bool)
+    R, Cn = M.shape
+    nw = (R + 63) // 64
+    pitch = (nw + 7) // 8 * 8
+    bits = np.zeros((Cn, pitch * 64), dtype=np.uint8)
+    bits[:, :R] = M.T
+    return np.packbits(bits, axis=1, bitorder="little").view(np.uint64).reshape(Cn, pitch)
+
+
+def unpack_colmain(words, rows: int) -> np.ndarray:
+    w = np.ascontiguousarray(words, dtype=np.uint64)
+    bits = np.unpackbits(w.view(np.uint8), axis=1, bitorder="little")
+    return bits[:, :rows].T.astype(bool)
+
+
+def mintersect_colmat(deps: list, tuples, with_lists: bool = True):
+    """Mintersect over ColMain dependencies; returns (counts, offsets, lists, stats)."""
+    tup = np.ascontiguousarray(tuples, dtype=np.uint32)
+    if tup.ndim == 1:
+        tup = tup.reshape(-1, len(deps))
+    T, nd = tup.shape
+    if nd != len(deps):
+        raise ValueError("tuple width differs from the number of dependencies")
+    arr = (C.c_void_p * nd)(*[d.handle for d in deps])
+    counts = np.zeros(T, dtype=np.uint32)
+    offsets = np.zeros(T + 1, dtype=np.uint64)
+    total = C.c_uint64()
+    st = _lib.MintersectStats()
+    _lib.call("vs_mintersect_colmat", arr, nd, _ptr(tup, C.c_uint32), T,
+              _ptr(counts, C.c_uint32), _ptr(offsets, C.c_uint64), None, 0, C.byref(total),
+              C.byref(st))
+    lists = None
+    if with_lists:
+        lists = np.zeros(max(total.value, 1), dtype=np.uint32)
+        _lib.call("vs_mintersect_colmat", arr, nd, _ptr(tup, C.c_uint32), T,
                   _ptr(counts, C.c_uint32), _ptr(offsets, C.c_uint64),
                   _ptr(lists, C.c_uint32), len(lists), C.byref(total), C.byref(st))
         lists = lists[:total.value]

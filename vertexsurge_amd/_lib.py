@@ -24,6 +24,7 @@ SCHED_HEURISTIC, SCHED_PUSH, SCHED_PULL, SCHED_LEARNED = 0, 1, 2, 3
 LAYOUT_NATIVE, LAYOUT_DVSM, LAYOUT_ROWMAJOR = 0, 1, 2
 VX_TIMING = 0x1
 BATCH_KEEP_ORDER = 0x2
+ORDER_NONE, ORDER_ASC, ORDER_DESC = 0, 1, 2
 GRAPH_SYMMETRIC = 0x1
 MAX_HOPS = 64
 MLP_IN, MLP_HIDDEN, MLP_OUT = 8, 64, 8
@@ -132,6 +133,22 @@ SIGNATURES = {
                                 C.c_uint64, u64p, C.POINTER(MintersectStats)]),
     "vs_mintersect_device": (C.c_int, [C.POINTER(vp), C.c_uint32, vp, C.c_uint64, vp, vp, vp,
                                        C.c_uint64, u64p, C.POINTER(MintersectStats)]),
+    "vs_colmat_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, u64p, C.POINTER(vp)]),
+    "vs_colmat_set_maps": (C.c_int, [vp, u32p, u32p, C.c_uint32]),
+    "vs_colmat_info": (C.c_int, [vp, u32p, u32p, u64p, C.POINTER(C.c_int), u32p]),
+    "vs_colmat_get_maps": (C.c_int, [vp, u32p, u32p]),
+    "vs_colmat_export": (C.c_int, [vp, u64p, C.c_size_t]),
+    "vs_colmat_get": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_int)]),
+    "vs_colmat_free": (C.c_int, [vp]),
+    "vs_table_convert": (C.c_int, [vp, C.c_uint32, C.c_int, C.POINTER(vp)]),
+    "vs_result_select_cols": (C.c_int, [vp, C.c_uint32, u32p, C.c_uint32, C.POINTER(vp)]),
+    "vs_result_col_map": (C.c_int, [vp, u32p]),
+    "vs_result_col_counts": (C.c_int, [vp, C.c_uint32, C.c_int32, u32p, i32p]),
+    "vs_colmat_col_counts": (C.c_int, [vp, C.c_int32, u32p, i32p]),
+    "vs_result_row_or": (C.c_int, [vp, C.c_uint32, u32p, C.c_uint32, u32p]),
+    "vs_colmat_row_or": (C.c_int, [vp, u32p, C.c_uint32, u32p]),
+    "vs_mintersect_colmat": (C.c_int, [C.POINTER(vp), C.c_uint32, u32p, C.c_uint64, u32p, u64p,
+                                       u32p, C.c_uint64, u64p, C.POINTER(MintersectStats)]),
     "vs_sched_create": (C.c_int, [C.c_uint64, C.POINTER(vp)]),
     "vs_sched_free": (C.c_int, [vp]),
     "vs_sched_num_params": (C.c_int, [C.POINTER(C.c_size_t)]),

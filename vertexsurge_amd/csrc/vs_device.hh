@@ -119,8 +119,12 @@ struct Mat {
 
 struct vs_result_s {
   vs_graph_s *g = nullptr;
+  // S rows (sources); V columns: the graph's vertices, or the selected
+  // columns of a column select (then col_map holds their vertex ids)
   uint32_t S = 0, V = 0, tw = 1, ntiles = 0, k_actual_max = 0;
   std::vector<vs::Mat> mats;
+  std::vector<uint32_t> row_map;  // source vertex of each row
+  std::vector<uint32_t> col_map;  // empty: column c is vertex c
 };
 
 namespace vs {
@@ -220,9 +224,13 @@ inline uint32_t grid_for(uint64_t waves, int num_cus, int blocks_per_cu = 8) {
 }
 
 // Result-side kernels (vs_graph.hip).
-vs_status mat_row_counts(vs_graph_s *g, const Mat &m, uint32_t S, uint32_t tw,
-                         uint32_t ntiles, uint64_t *d_counts);
-vs_status mat_row_digests(vs_graph_s *g, const Mat &m, uint32_t S, uint32_t tw,
-                          uint32_t ntiles, uint64_t *d_digests);
+vs_status mat_row_counts(vs_graph_s *g, const Mat &m, uint32_t S, uint32_t V,
+                         uint32_t tw, uint32_t ntiles, uint64_t *d_counts);
+vs_status mat_row_digests(vs_graph_s *g, const Mat &m, uint32_t S, uint32_t V,
+                          uint32_t tw, uint32_t ntiles, uint64_t *d_digests);
+// native [tile][col][tw] -> one bitmap of V bits per row at out + row*pitch
+// (words nvb .. pitch-1 of each row are left as they are)
+vs_status mat_to_rowmajor(vs_graph_s *g, const Mat &m, uint32_t S, uint32_t V,
+                          uint32_t tw, uint64_t *out, uint64_t pitch);
 
 } // namespace vs

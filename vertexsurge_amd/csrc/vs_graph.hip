@@ -331,7 +331,8 @@ __global__ void k_to_dvsm(const uint64_t *__restrict__ M, uint32_t V, uint32_t W
 template <int TW>
 __global__ __launch_bounds__(kBlock) void k_to_rowmajor(const uint64_t *__restrict__ M,
                                                        uint32_t V, uint32_t S,
-                                                       uint32_t W, uint64_t *out) {
+                                                       uint32_t W, uint64_t *out,
+                                                       uint64_t pitch) {
   const int lane = lane_id(), wid = threadIdx.x >> 6;
   const uint32_t nvb = (V + 63) / 64;
   const uint64_t total = static_cast<uint64_t>(W) * nvb;
@@ -348,7 +349,7 @@ __global__ __launch_bounds__(kBlock) void k_to_rowmajor(const uint64_t *__restri
       if (lane == b) mine = y;
     }
     const uint64_t row = static_cast<uint64_t>(w) * 64 + lane;
-    if (row < S) out[row * nvb + vb] = mine;
+    if (row < S) out[row * pitch + vb] = mine;
   }
 }
 
@@ -371,25 +372,39 @@ __global__ __launch_bounds__(kBlock) void k_copy_words(uint64_t *__restrict__ ds
     default: hipLaunchKernelGGL(KERNEL<64>, __VA_ARGS__); break;             \
   }
 
-vs_status mat_row_counts(vs_graph_s *g, const Mat &m, uint32_t S, uint32_t tw,
-                         uint32_t ntiles, uint64_t *d_counts) {
+vs_status mat_row_counts(vs_graph_s *g, const Mat &m, uint32_t S, uint32_t V,
+                         uint32_t tw, uint32_t ntiles, uint64_t *d_counts) {
   VS_HIP(hipMemsetAsync(d_counts, 0, static_cast<size_t>(S) * 8, g->stream));
-  const uint64_t items = static_cast<uint64_t>(ntiles) * ((g->V + 8191) / 8192);
+  if (V == 0) return VS_OK;
+  const uint64_t items = static_cast<uint64_t>(ntiles) * ((V + 8191) / 8192);
   const uint32_t grid = grid_for(items, g->num_cus);
   VS_TW_DISPATCH(tw, k_rowcount, dim3(grid), dim3(kBlock), 0, g->stream, m.d,
-                 g->V, S, ntiles, reinterpret_cast<ull *>(d_counts));
+                 V, S, ntiles, reinterpret_cast<ull *>(d_counts));
   VS_HIP(hipGetLastError());
   return VS_OK;
 }
 
-vs_status mat_row_digests(vs_graph_s *g, const Mat &m, uint32_t S, uint32_t tw,
-                          uint32_t ntiles, uint64_t *d_digests) {
+vs_status mat_row_digests(vs_graph_s *g, const Mat &m, uint32_t S, uint32_t V,
+                          uint32_t tw, uint32_t ntiles, uint64_t *d_digests) {
   VS_HIP(hipMemsetAsync(d_digests, 0, static_cast<size_t>(S) * 8, g->stream));
+  if (V == 0) return VS_OK;
   const uint64_t items =
-      static_cast<uint64_t>(ntiles) * tw * (((g->V + 63) / 64 + 15) / 16);
+      static_cast<uint64_t>(ntiles) * tw * (((V + 63) / 64 + 15) / 16);
   const uint32_t grid = grid_for(items, g->num_cus);
   VS_TW_DISPATCH(tw, k_rowdigest, dim3(grid), dim3(kBlock), 0, g->stream, m.d,
-                 g->V, S, ntiles, reinterpret_cast<ull *>(d_digests));
+                 V, S, ntiles, reinterpret_cast<ull *>(d_digests));
+  VS_HIP(hipGetLastError());
+  return VS_OK;
+}
+
+vs_status mat_to_rowmajor(vs_graph_s *g, const Mat &m, uint32_t S, uint32_t V,
+                          uint32_t tw, uint64_t *out, uint64_t pitch) {
+  const uint32_t W = (S + 63) / 64;
+  const uint64_t items = static_cast<uint64_t>(W) * ((V + 63) / 64);
+  if (items == 0) return VS_OK;
+  const uint32_t grid = grid_for(items, g->num_cus);
+  VS_TW_DISPATCH(tw, k_to_rowmajor, dim3(grid), dim3(kBlock), 0, g->stream, m.d, V,
+                 S, W, out, pitch);
   VS_HIP(hipGetLastError());
   return VS_OK;
 }
@@ -628,9 +643,9 @@ static vs_status row_reduce(vs_result_t r, uint32_t i, uint64_t *host, bool dige
   const size_t bytes = static_cast<size_t>(r->S) * 8;
   void *d = nullptr;
   VS_TRY(g->pool.get(bytes, false, g->stream, &d));
-  vs_status s = digest ? mat_row_digests(g, r->mats[i], r->S, r->tw, r->ntiles,
+  vs_status s = digest ? mat_row_digests(g, r->mats[i], r->S, r->V, r->tw, r->ntiles,
                                          static_cast<uint64_t *>(d))
-                       : mat_row_counts(g, r->mats[i], r->S, r->tw, r->ntiles,
+                       : mat_row_counts(g, r->mats[i], r->S, r->V, r->tw, r->ntiles,
                                         static_cast<uint64_t *>(d));
   if (s == VS_OK) {
     hipError_t e = hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, g->stream);
@@ -714,10 +729,12 @@ vs_status vs_result_export(vs_result_t r, uint32_t i, int32_t layout,
     VS_TW_DISPATCH(r->tw, k_to_dvsm, dim3(grid), dim3(kBlock), 0, g->stream, m.d,
                    r->V, W, static_cast<uint64_t *>(d), nout);
   } else {
-    const uint64_t items = static_cast<uint64_t>(W) * ((r->V + 63) / 64);
-    const uint32_t grid = grid_for(items, g->num_cus);
-    VS_TW_DISPATCH(r->tw, k_to_rowmajor, dim3(grid), dim3(kBlock), 0, g->stream,
-                   m.d, r->V, r->S, W, static_cast<uint64_t *>(d));
+    const vs_status st = mat_to_rowmajor(g, m, r->S, r->V, r->tw,
+                                         static_cast<uint64_t *>(d), (r->V + 63) / 64);
+    if (st != VS_OK) {
+      g->pool.put(d, need, false);
+      return st;
+    }
   }
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(host_buf, d, need, hipMemcpyDeviceToHost, g->stream);

@@ -295,6 +295,7 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
   res->V = V;
   res->tw = TW;
   res->ntiles = T;
+  if (!find_shortest && S) res->row_map.assign(sources, sources + S);
 
   if (S == 0) {  // the reference warns and skips (vexpand.cpp:170-173)
     res->k_actual_max = 0;
