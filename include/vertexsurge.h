@@ -157,7 +157,8 @@ typedef struct {
   uint32_t tile_words;  /* storage tile width in 64-bit words: 1..64, pow2 */
   uint32_t push_alpha;  /* push while frontier-edges * alpha < E */
   uint32_t pull_beta;   /* unused when 0; pull->push when popcount*beta < rows*V */
-  uint32_t reserved;
+  uint32_t pull_hubs;   /* vertices whose frontier rows pull gathers first, with
+                           cached loads (the most-listed ones); 0 = 131072 */
 } vs_tuning;
 vs_status vs_graph_set_tuning(vs_graph_t g, const vs_tuning *t);
 vs_status vs_graph_get_tuning(vs_graph_t g, vs_tuning *t);

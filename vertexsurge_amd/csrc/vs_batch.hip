@@ -808,7 +808,7 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   c.out_counts = static_cast<ull *>(d_counts.p);
   c.out_digests = want_digests ? static_cast<ull *>(d_dig.p) : nullptr;
   c.mlp = (p.sched == VS_SCHED_LEARNED && g->sched) ? sched_params(g->sched) : nullptr;
-  VS_TRY(ensure_pull_items(*const_cast<DevCSR *>(pull), V));
+  VS_TRY(ensure_pull_items(*const_cast<DevCSR *>(pull), V, g->tuning.pull_hubs));
   c.pitems = pull->pitems;
   c.padj = pull->hadj;  // hubs-first lists (pull only; push uses xadj)
   c.hubcnt = pull->hubcnt;
@@ -995,7 +995,7 @@ vs_status vs_autotune_vexpand(vs_graph_t g, const uint32_t *sources, uint32_t S,
   double bms = 1e300;
   for (uint32_t tw = std::max<uint32_t>(1, tw_max / 4); tw <= tw_max; tw *= 2) {
     for (uint32_t alpha : {4u, 16u, 64u}) {
-      vs_tuning t{tw, alpha, 0, 0};
+      vs_tuning t{tw, alpha, saved.pull_beta, saved.pull_hubs};
       g->tuning = t;
       vs_vexpand_params pp{};
       pp.k_min = 1;

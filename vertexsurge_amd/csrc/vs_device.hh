@@ -68,18 +68,23 @@ struct DevCSR {
   bool has_heavy = false;     // some list longer than kPullChunk
   // Pull copy of the lists with the hub neighbours first (hubs = the
   // kPullHubs vertices that appear in the most lists, i.e. whose frontier
-  // rows are gathered most often), each part ascending, and per-vertex hub
-  // counts.  Pull gathers hub rows with cached loads and the rest with
+  // rows are gathered most often; most-listed first), the rest ascending,
+  // and per-vertex hub counts.  Pull gathers hub rows with cached loads and the rest with
   // streaming loads, so the rows reused across destinations stay in L2.
   uint32_t *hadj = nullptr;
   uint32_t *hubcnt = nullptr;
+  uint32_t nhubs = 0;  // hub count the lists were built for
 };
 
-constexpr uint32_t kPullHubs = 8192;
+// default hub count: 131072 rows of 512 B (tile width 64) = 64 MB, a hot set
+// the 256 MB Infinity Cache can hold next to the streaming gathers
+constexpr uint32_t kPullHubs = 131072;
 
 // Builds the pull items of a CSR once (vertex | chunk << 32 per kPullChunk
 // in-edges, vertices without edges omitted).
-vs_status ensure_pull_items(DevCSR &c, uint32_t V);
+// builds the pull items once, and the hub-first lists for `hubs` hubs
+// (0 = kPullHubs), rebuilding them when the hub count changes
+vs_status ensure_pull_items(DevCSR &c, uint32_t V, uint32_t hubs);
 
 } // namespace vs
 

@@ -85,7 +85,7 @@ static void free_csr(DevCSR &d) {
 }
 
 // Hubs-first pull lists (see DevCSR::hadj), built on the host in parallel.
-static vs_status build_hub_lists(DevCSR &c, uint32_t V) {
+static vs_status build_hub_lists(DevCSR &c, uint32_t V, uint32_t hubs) {
   const uint64_t E = c.E;
   std::vector<uint32_t> adj(E);
   if (E) VS_HIP(hipMemcpy(adj.data(), c.adj, E * 4, hipMemcpyDeviceToHost));
@@ -107,7 +107,7 @@ static vs_status build_hub_lists(DevCSR &c, uint32_t V) {
   std::vector<std::vector<uint32_t>>().swap(part);
   std::vector<uint32_t> order(V);
   for (uint32_t v = 0; v < V; ++v) order[v] = v;
-  const uint32_t K = std::min<uint32_t>(kPullHubs, V);
+  const uint32_t K = std::min<uint32_t>(hubs, V);
   std::nth_element(order.begin(), order.begin() + K, order.end(), [&](uint32_t x, uint32_t y) {
     return cnt[x] != cnt[y] ? cnt[x] > cnt[y] : x < y;
   });
@@ -119,6 +119,11 @@ static vs_status build_hub_lists(DevCSR &c, uint32_t V) {
       for (uint64_t v = static_cast<uint64_t>(V) * t / nt; v < static_cast<uint64_t>(V) * (t + 1) / nt; ++v) {
         uint32_t *b = adj.data() + c.hoff[v], *e = adj.data() + c.hoff[v + 1];
         uint32_t *mid = std::stable_partition(b, e, [&](uint32_t u) { return hub[u] != 0; });
+        // most-listed hubs first: their rows are the densest, so the pull's
+        // early exit fires after the fewest gathers
+        std::sort(b, mid, [&](uint32_t x, uint32_t y) {
+          return cnt[x] != cnt[y] ? cnt[x] > cnt[y] : x < y;
+        });
         hcnt[v] = static_cast<uint32_t>(mid - b);
       }
     });
@@ -136,11 +141,21 @@ static vs_status build_hub_lists(DevCSR &c, uint32_t V) {
   }
   c.hadj = static_cast<uint32_t *>(ph);
   c.hubcnt = static_cast<uint32_t *>(pc);
+  c.nhubs = hubs;
   return VS_OK;
 }
 
-vs_status ensure_pull_items(DevCSR &c, uint32_t V) {
-  if (c.pitems) return VS_OK;
+vs_status ensure_pull_items(DevCSR &c, uint32_t V, uint32_t hubs) {
+  if (hubs == 0) hubs = kPullHubs;
+  if (c.pitems) {
+    if (c.hadj && c.nhubs == hubs) return VS_OK;
+    VS_HIP(hipDeviceSynchronize());  // no kernel may still read the old lists
+    (void)hipFree(c.hadj);
+    (void)hipFree(c.hubcnt);
+    c.hadj = nullptr;
+    c.hubcnt = nullptr;
+    return build_hub_lists(c, V, hubs);
+  }
   std::vector<unsigned long long> it;
   it.reserve(static_cast<size_t>(V) + c.E / kPullChunk + 1);
   bool heavy = false;
@@ -163,7 +178,7 @@ vs_status ensure_pull_items(DevCSR &c, uint32_t V) {
   c.pitems = static_cast<unsigned long long *>(p);
   c.npitems = it.size();
   c.has_heavy = heavy;
-  return build_hub_lists(c, V);
+  return build_hub_lists(c, V, hubs);
 }
 
 vs_status graph_csrs(vs_graph_s *g, int direction, const DevCSR **push,

@@ -178,26 +178,36 @@ __device__ __forceinline__ uint64_t gather(const HopArgs &a, uint32_t u, int j) 
   else return __builtin_nontemporal_load(p);
 }
 
+// ORs the frontier words of nbr[k], nbr[k + G], ... below end into acc, eight
+// loads in flight, and stops as soon as this lane's partial word already
+// holds every needed row (a lane-local test: the group OR can only add bits,
+// so stopping on the partial word is exact for TW = 64 and conservative
+// below).  Lanes stop independently; the hubs at the head of the list are
+// gathered first, so most lanes that can stop do so before the streaming part.
 template <int TW, bool CACHED>
-__device__ __forceinline__ uint64_t gather_range(const HopArgs &a, const uint32_t *nbr,
-                                                 uint32_t &k, uint32_t end, int j) {
+__device__ __forceinline__ void gather_range(const HopArgs &a, const uint32_t *nbr, uint32_t &k,
+                                             uint32_t end, int j, uint64_t need,
+                                             uint64_t &acc) {
   constexpr int G = 64 / TW;
-  uint64_t acc = 0;
   for (; k + 7 * G < end; k += 8 * G) {
+    if ((acc & need) == need) return;
     uint64_t x[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) x[q] = gather<TW, CACHED>(a, nbr[k + q * G], j);
     acc |= ((x[0] | x[1]) | (x[2] | x[3])) | ((x[4] | x[5]) | (x[6] | x[7]));
   }
-  for (; k + 3 * G < end; k += 4 * G) {
+  if (k + 3 * G < end && (acc & need) != need) {
     const uint64_t x0 = gather<TW, CACHED>(a, nbr[k], j);
     const uint64_t x1 = gather<TW, CACHED>(a, nbr[k + G], j);
     const uint64_t x2 = gather<TW, CACHED>(a, nbr[k + 2 * G], j);
     const uint64_t x3 = gather<TW, CACHED>(a, nbr[k + 3 * G], j);
     acc |= (x0 | x1) | (x2 | x3);
+    k += 4 * G;
   }
-  for (; k < end; k += G) acc |= gather<TW, CACHED>(a, nbr[k], j);
-  return acc;
+  for (; k < end; k += G) {
+    if ((acc & need) == need) return;
+    acc |= gather<TW, CACHED>(a, nbr[k], j);
+  }
 }
 
 template <int TW, bool SHORTEST>
@@ -237,8 +247,8 @@ __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, u
     wave_sync();
     if (todo) {
       uint32_t k = g;
-      acc |= gather_range<TW, true>(a, nbr, k, nh, j);
-      acc |= gather_range<TW, false>(a, nbr, k, n, j);
+      gather_range<TW, true>(a, nbr, k, nh, j, need, acc);
+      gather_range<TW, false>(a, nbr, k, n, j, need, acc);
     }
     wave_sync();
     acc = group_or<TW>(acc);

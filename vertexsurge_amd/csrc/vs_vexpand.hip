@@ -491,7 +491,7 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
         launch_push(TW, shortest, push_grid, s, a);
         if (timing) VS_HIP(hipEventRecord(k1, s));
       } else {
-        VS_TRY(ensure_pull_items(*const_cast<DevCSR *>(pull), V));
+        VS_TRY(ensure_pull_items(*const_cast<DevCSR *>(pull), V, g->tuning.pull_hubs));
         a.pitems = pull->pitems;
         a.padj = pull->hadj;  // hubs-first lists
         a.hubcnt = pull->hubcnt;
