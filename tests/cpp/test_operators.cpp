@@ -101,7 +101,63 @@ int main() {
     mc.add_dependency(dep);
     mc.add_dependency(dep);
     mc.run(base);
-    std::printf("\"mintersect_kat_count\": %llu\n", (unsigned long long)mc.count());
+    std::printf("\"mintersect_kat_count\": %llu,\n", (unsigned long long)mc.count());
+  }
+
+  // MintersectFlatMatMat (mintersect.cpp:411-512): ColMain dependencies, 10x10,
+  // columns mapped through the 8 keys 1..8
+  {
+    const uint32_t keys[8] = {1, 2, 3, 4, 5, 6, 7, 8};
+    const uint32_t cs[8] = {1, 1, 1, 2, 2, 3, 3, 4};
+    std::vector<uint64_t> words(10 * 8, 0);  // 10 columns x align8(1) words
+    for (int i = 0; i < 8; ++i) words[static_cast<size_t>(i) * 8] |= 1ull << cs[i];
+    auto a = std::make_shared<ColMainMatrixTable>(
+        ColMainMatrixTable::from_words(rel->handle(), 10, 10, words));
+    auto b = std::make_shared<ColMainMatrixTable>(
+        ColMainMatrixTable::from_words(rel->handle(), 10, 10, words));
+    const std::vector<VID> kv(keys, keys + 8);
+    a->set_maps(std::nullopt, kv);
+    b->set_maps(std::nullopt, kv);
+    Mintersect mm("c", "Person", false);
+    mm.add_dependency(a);
+    mm.add_dependency(b);
+    mm.run({1, 1, 2, 1, 3, 1, 4, 2, 5, 2, 6, 3, 7, 3, 8, 4});
+    std::vector<uint64_t> flat(mm.flat_tuples().begin(), mm.flat_tuples().end());
+    std::printf("\"mintersect_matmat_tuples\": %zu,\n", mm.tuple_count());
+    print_vec("mintersect_matmat_flat", flat);
+  }
+
+  // a -[1..2]-> c <-[1..2]- b: VExpand, TableConvert (transposed), Mintersect
+  {
+    VExpand vp("Person", "Person", "knows", Direction::Forward, 1, 2,
+               VExpandPathParameters::Any(), ExpandStrategy::Dense, "a.id", std::nullopt);
+    vp.run(starts);
+    auto reach = vp.results();
+    auto cm = std::make_shared<ColMainMatrixTable>(
+        TableConvert(TableType::ColMainMatrix, true).run(reach));
+    std::printf("\"convert_rows\": %u, \"convert_cols\": %u,\n", cm->rows(), cm->cols());
+    std::vector<VID> base;
+    for (size_t i = 0; i + 1 < starts.size(); i += 2) {
+      base.push_back(starts[i]);
+      base.push_back(starts[i + 1]);
+    }
+    Mintersect pm("c", "Person", true);
+    pm.add_dependency(cm);
+    pm.add_dependency(cm);
+    pm.run(base);
+    std::printf("\"pattern_count\": %llu,\n", (unsigned long long)pm.count());
+    // MatrixColumnCount (DESC) and MatrixRowOrSum on the reach matrix
+    auto cc = MatrixColumnCount("cnt", SortOrder::DESC).run(reach);
+    std::vector<uint64_t> top;
+    for (size_t i = 0; i < 5 && i < cc.size(); ++i) {
+      top.push_back(cc[i].first);
+      top.push_back(static_cast<uint64_t>(cc[i].second));
+    }
+    print_vec("colcount_desc_top5", top);
+    std::printf("\"row_or_size\": %zu,\n", MatrixRowOrSum().run(reach).size());
+    // MatrixTableSelect of three columns, then its tuples
+    auto sel = MatrixTableSelect("c").run(reach, {starts[0], starts[1], starts[0]});
+    std::printf("\"select_tuple_count\": %zu\n", sel.tuple_count());
   }
   std::printf("}\n");
   GraphStore::instance().clear();

@@ -41,3 +41,19 @@ def test_cpp_operators_match_oracle(orc):
     assert r["unknown_schema_throws"] == 1
     assert r["mintersect_kat_tuples"] == 3 and r["mintersect_kat_count"] == 3
     assert r["mintersect_kat_flat"] == [1, 1, 1, 2, 1, 1, 3, 1, 1]
+    assert r["mintersect_matmat_tuples"] == 3
+    assert r["mintersect_matmat_flat"] == [1, 1, 1, 2, 1, 1, 3, 1, 1]
+    # the VExpand -> TableConvert -> Mintersect pattern against the oracle
+    from oracle import tables as ot
+    m, _, _, _ = orc.vexpand_rows(g.offsets, g.adj, starts, 1, 2, "any")
+    M = np.unpackbits(m[0].view(np.uint8), axis=1, bitorder="little")[:, :g.V].astype(bool)
+    E, rm, cm = ot.table_convert(M, starts, None, True)
+    assert (r["convert_rows"], r["convert_cols"]) == E.shape
+    base = starts[: len(starts) // 2 * 2].reshape(-1, 2)
+    c, _, _ = ot.mintersect_colmain([(E, rm, cm)] * 2, base)
+    assert r["pattern_count"] == int(c.sum())
+    ids, cnt = ot.column_count(M, None, "desc")
+    assert r["colcount_desc_top5"] == [int(x) for p in zip(ids[:5], cnt[:5]) for x in p]
+    assert r["row_or_size"] == len(ot.row_or_sum(M, None))
+    sel, _, _ = ot.select_cols(M, starts, [starts[0], starts[1], starts[0]])
+    assert r["select_tuple_count"] == int(sel.sum())

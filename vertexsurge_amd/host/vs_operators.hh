@@ -114,9 +114,15 @@ class GraphStore {
 class VarRowMatrixTable {
  public:
   VarRowMatrixTable(std::shared_ptr<vs_result_s> res, uint32_t index,
-                    std::vector<VID> row_map)
-      : res_(std::move(res)), index_(index), row_map_(std::move(row_map)) {}
+                    std::vector<VID> row_map,
+                    std::optional<std::vector<VID>> col_map = std::nullopt)
+      : res_(std::move(res)), index_(index), row_map_(std::move(row_map)),
+        col_map_(std::move(col_map)) {}
   const std::vector<VID> &get_row_map() const { return row_map_; }
+  // set by MatrixTableSelect; VExpand output has none (columns = vertices)
+  const std::optional<std::vector<VID>> &get_col_map() const { return col_map_; }
+  vs_result_s *handle() const { return res_.get(); }
+  uint32_t index() const { return index_; }
   // bit_matrix_table.cpp:112
   size_t tuple_count() const {
     uint64_t c = 0;
@@ -152,14 +158,168 @@ class VarRowMatrixTable {
     for (size_t r = 0; r < row_map_.size(); ++r)
       for (size_t w = 0; w < nw; ++w)
         for (uint64_t m = rm[r * nw + w]; m; m &= m - 1)
-          out.emplace_back(row_map_[r], static_cast<VID>(w * 64 + __builtin_ctzll(m)));
+          out.emplace_back(row_map_[r], col_id(static_cast<VID>(w * 64 + __builtin_ctzll(m))));
     return out;
   }
 
  private:
+  VID col_id(VID c) const { return col_map_ ? (*col_map_)[c] : c; }
   std::shared_ptr<vs_result_s> res_;
   uint32_t index_;
   std::vector<VID> row_map_;
+  std::optional<std::vector<VID>> col_map_;
+};
+
+// ColMainMatrixTable (matrix_table.hh:85-125): a DenseColMainMatrix in HBM
+// with optional row / column maps.
+class ColMainMatrixTable {
+ public:
+  explicit ColMainMatrixTable(vs_colmat_t m)
+      : m_(m, [](vs_colmat_s *x) { vs_colmat_free(x); }) {}
+  // an R x C matrix from host words in the reference layout (C x align8(ceil(R/64)))
+  static ColMainMatrixTable from_words(vs_graph_t g, uint32_t rows, uint32_t cols,
+                                       const std::vector<uint64_t> &words) {
+    vs_colmat_t m = nullptr;
+    check(vs_colmat_create(g, rows, cols, words.empty() ? nullptr : words.data(), &m),
+          "vs_colmat_create");
+    return ColMainMatrixTable(m);
+  }
+  void set_maps(const std::optional<std::vector<VID>> &row_map,
+                const std::optional<std::vector<VID>> &col_map) {
+    check(vs_colmat_set_maps(m_.get(), row_map ? row_map->data() : nullptr,
+                             col_map ? col_map->data() : nullptr,
+                             col_map ? static_cast<uint32_t>(col_map->size()) : 0),
+          "vs_colmat_set_maps");
+  }
+  uint32_t rows() const { return info().first; }
+  uint32_t cols() const { return info().second; }
+  bool get(uint32_t r, uint32_t c) const {
+    int b = 0;
+    check(vs_colmat_get(m_.get(), r, c, &b), "vs_colmat_get");
+    return b != 0;
+  }
+  std::vector<uint64_t> export_words() const {
+    uint64_t pitch = 0;
+    uint32_t r = 0, c = 0;
+    check(vs_colmat_info(m_.get(), &r, &c, &pitch, nullptr, nullptr), "vs_colmat_info");
+    std::vector<uint64_t> out(static_cast<size_t>(c) * pitch);
+    if (!out.empty())
+      check(vs_colmat_export(m_.get(), out.data(), out.size() * 8), "vs_colmat_export");
+    return out;
+  }
+  vs_colmat_s *handle() const { return m_.get(); }
+
+ private:
+  std::pair<uint32_t, uint32_t> info() const {
+    uint32_t r = 0, c = 0;
+    check(vs_colmat_info(m_.get(), &r, &c, nullptr, nullptr, nullptr), "vs_colmat_info");
+    return {r, c};
+  }
+  std::shared_ptr<vs_colmat_s> m_;
+};
+
+// src/include/execution/tuple.hh:9-18 (the kinds TableConvert handles here)
+enum class TableType { Flat, CSR, VarRowMatrix, ColMainMatrix };
+// src/include/operator/big_operators2.hh:100-103
+enum class SortOrder { ASC, DESC };
+
+// TableConvert (big_operators2.hh:27-37, big_operators2.cpp:16-86):
+// VarRowMatrix -> ColMainMatrix on the device, maps carried along and
+// swapped when transposing.
+class TableConvert {
+ public:
+  explicit TableConvert(TableType type, bool transpose = false)
+      : convert_to_(type), transpose_(transpose) {
+    if (type != TableType::ColMainMatrix)
+      throw std::runtime_error("TableConvert: only VarRowMatrix -> ColMainMatrix runs on the GPU");
+  }
+  ColMainMatrixTable run(const VarRowMatrixTable &in) const {
+    vs_colmat_t m = nullptr;
+    check(vs_table_convert(in.handle(), in.index(), transpose_ ? 1 : 0, &m), "vs_table_convert");
+    return ColMainMatrixTable(m);
+  }
+
+ private:
+  TableType convert_to_;
+  bool transpose_;
+};
+
+// MatrixTableSelect (big_operators2.hh:70-80, big_operators2.cpp:138-213):
+// keeps the columns named by the key column of a flat dependency.
+class MatrixTableSelect {
+ public:
+  explicit MatrixTableSelect(VariableName key_name) : key_name_(std::move(key_name)) {}
+  VarRowMatrixTable run(const VarRowMatrixTable &in, const std::vector<VID> &keys) const {
+    vs_result_t r = nullptr;
+    check(vs_result_select_cols(in.handle(), in.index(), keys.data(),
+                                static_cast<uint32_t>(keys.size()), &r),
+          "vs_result_select_cols");
+    return VarRowMatrixTable(std::shared_ptr<vs_result_s>(r, [](vs_result_s *x) { vs_result_free(x); }),
+                             0, in.get_row_map(), keys);
+  }
+
+ private:
+  VariableName key_name_;
+};
+
+// MatrixColumnCount (big_operators2.hh:112-129, matrix_column_count.cpp):
+// a flat table (column id, Int32 count), optionally sorted by (count, id).
+class MatrixColumnCount {
+ public:
+  explicit MatrixColumnCount(VariableName count_name, std::optional<SortOrder> ord = std::nullopt)
+      : count_name_(std::move(count_name)), ord_(ord) {}
+  std::vector<std::pair<VID, int32_t>> run(const VarRowMatrixTable &in) const {
+    uint32_t rows = 0, cols = 0;
+    check(vs_result_info(in.handle(), &rows, &cols, nullptr, nullptr), "vs_result_info");
+    std::vector<uint32_t> ids(cols);
+    std::vector<int32_t> cnt(cols);
+    if (cols)
+      check(vs_result_col_counts(in.handle(), in.index(), order(), ids.data(), cnt.data()),
+            "vs_result_col_counts");
+    return zip(ids, cnt);
+  }
+  std::vector<std::pair<VID, int32_t>> run(const ColMainMatrixTable &in) const {
+    const uint32_t cols = in.cols();
+    std::vector<uint32_t> ids(cols);
+    std::vector<int32_t> cnt(cols);
+    if (cols)
+      check(vs_colmat_col_counts(in.handle(), order(), ids.data(), cnt.data()),
+            "vs_colmat_col_counts");
+    return zip(ids, cnt);
+  }
+
+ private:
+  int32_t order() const {
+    return !ord_ ? VS_ORDER_NONE : *ord_ == SortOrder::ASC ? VS_ORDER_ASC : VS_ORDER_DESC;
+  }
+  static std::vector<std::pair<VID, int32_t>> zip(const std::vector<uint32_t> &a,
+                                                  const std::vector<int32_t> &b) {
+    std::vector<std::pair<VID, int32_t>> out(a.size());
+    for (size_t i = 0; i < a.size(); ++i) out[i] = {a[i], b[i]};
+    return out;
+  }
+  VariableName count_name_;
+  std::optional<SortOrder> ord_;
+};
+
+// MatrixRowOrSum (big_operators2.hh:131-138, matrix_row_or_sum.cpp): the ids
+// of the columns with any set bit.
+class MatrixRowOrSum {
+ public:
+  std::vector<VID> run(const VarRowMatrixTable &in) const {
+    uint32_t n = 0;
+    check(vs_result_row_or(in.handle(), in.index(), nullptr, 0, &n), "vs_result_row_or");
+    std::vector<VID> out(n);
+    if (n) check(vs_result_row_or(in.handle(), in.index(), out.data(), n, &n), "vs_result_row_or");
+    return out;
+  }
+  std::vector<VID> run(const ColMainMatrixTable &in) const {
+    uint32_t n = 0;
+    check(vs_colmat_row_or(in.handle(), nullptr, 0, &n), "vs_colmat_row_or");
+    std::vector<VID> out(n);
+    if (n) check(vs_colmat_row_or(in.handle(), out.data(), n, &n), "vs_colmat_row_or");
+    return out;
+  }
 };
 
 // MultipleMatrixTable: one matrix per distance (bit_matrix_table.cpp:291-340).
@@ -289,30 +449,42 @@ class Mintersect {
       : new_vertex_var_name_(std::move(new_vertex_var_name)),
         new_vertex_type_(std::move(new_vertex_type)), only_count_(only_count) {}
 
-  // One dependency per bound base column (addDependency order).
+  // One dependency per bound base column (addDependency order): all edge
+  // relations (sorted adjacency) or all ColMain tables (mintersect.cpp:139-156).
   void add_dependency(std::shared_ptr<EdgeRelation> rel) { deps_.push_back(std::move(rel)); }
+  void add_dependency(std::shared_ptr<ColMainMatrixTable> m) { mat_deps_.push_back(std::move(m)); }
 
   // base: T rows of arity = deps.size() vertex ids, row-major.
   void run(const std::vector<VID> &base) {
-    const uint32_t arity = static_cast<uint32_t>(deps_.size());
+    if (!deps_.empty() && !mat_deps_.empty())
+      throw std::runtime_error("Mintersect: mixed edge-relation and ColMain dependencies");
+    const bool mats = !mat_deps_.empty();
+    const uint32_t arity = static_cast<uint32_t>(mats ? mat_deps_.size() : deps_.size());
     if (arity == 0 || base.size() % arity) throw std::runtime_error("bad base tuples");
     const uint64_t T = base.size() / arity;
     std::vector<vs_graph_t> hs;
+    std::vector<vs_colmat_t> ms;
     for (auto &d : deps_) hs.push_back(d->handle());
+    for (auto &d : mat_deps_) ms.push_back(d->handle());
+    auto call = [&](uint32_t *lists, uint64_t cap, uint64_t *total) {
+      if (mats)
+        check(vs_mintersect_colmat(ms.data(), arity, base.data(), T, counts_.data(),
+                                   offsets_.data(), lists, cap, total, nullptr),
+              "vs_mintersect_colmat");
+      else
+        check(vs_mintersect(hs.data(), arity, base.data(), T, counts_.data(), offsets_.data(),
+                            lists, cap, total, nullptr),
+              "vs_mintersect");
+    };
     counts_.assign(T, 0);
     offsets_.assign(T + 1, 0);
     uint64_t total = 0;
-    check(vs_mintersect(hs.data(), arity, base.data(), T, counts_.data(), offsets_.data(),
-                        nullptr, 0, &total, nullptr),
-          "vs_mintersect");
+    call(nullptr, 0, &total);
     total_num_ = total;
     tuples_.clear();
     if (only_count_) return;
     std::vector<uint32_t> lists(total);
-    if (total)
-      check(vs_mintersect(hs.data(), arity, base.data(), T, counts_.data(), offsets_.data(),
-                          lists.data(), total, &total, nullptr),
-            "vs_mintersect");
+    if (total) call(lists.data(), total, &total);
     // base columns + new vertex, tuples in base order, new ids ascending
     for (uint64_t t = 0; t < T; ++t)
       for (uint64_t k = offsets_[t]; k < offsets_[t + 1]; ++k) {
@@ -330,6 +502,7 @@ class Mintersect {
   TypeName new_vertex_type_;
   bool only_count_;
   std::vector<std::shared_ptr<EdgeRelation>> deps_;
+  std::vector<std::shared_ptr<ColMainMatrixTable>> mat_deps_;
   std::vector<uint32_t> counts_;
   std::vector<uint64_t> offsets_;
   std::vector<VID> tuples_;

@@ -67,7 +67,7 @@ def collect(graph_specs, source_sizes=(256, 1024, 4096), k_max=5, seed=1, log=pr
         t0 = time.time()
         csr = make_graph(spec)
         G = Graph.from_csr(csr)
-        for qi, S in enumerate(source_sizes):
+        for qi, S in enumerate(spec.get("sources", source_sizes)):
             src = sample_sources(csr.V, min(S, csr.V), seed * 7919 + qi)
             x, y, m = trace_query(G, csr.V, csr.E, src, k_max)
             X += x
@@ -80,6 +80,7 @@ def collect(graph_specs, source_sizes=(256, 1024, 4096), k_max=5, seed=1, log=pr
 
 
 def make_graph(spec: dict) -> HostCSR:
+    spec = {k: v for k, v in spec.items() if k != "sources"}
     if spec["kind"] == "rmat":
         a = spec.get("a", 0.57)
         bc = (1.0 - a - 0.05) / 2
@@ -129,6 +130,18 @@ LARGE_SPECS = [{"kind": "rmat", "scale": sc, "ef": 32, "a": a, "seed": 200 + i}
                                             for a in (0.45, 0.5, 0.55, 0.6)])]
 LARGE_HELDOUT = [{"kind": "rmat", "scale": 23, "ef": 32, "a": 0.57, "seed": 77},
                  {"kind": "er", "V": 1 << 23, "pairs": 32 << 23, "seed": 77}]
+# R-MAT scales 22-24 (edge factor 32) over a in [0.45, 0.60]
+C5_SPECS = ([{"kind": "rmat", "scale": 22, "ef": 32, "a": a, "seed": 300 + i,
+              "sources": (256, 1024, 4096)} for i, a in enumerate((0.45, 0.5, 0.55, 0.6))] +
+            [{"kind": "rmat", "scale": 23, "ef": 32, "a": a, "seed": 310 + i,
+              "sources": (256, 1024, 4096)} for i, a in enumerate((0.45, 0.55))] +
+            [{"kind": "rmat", "scale": 24, "ef": 32, "a": a, "seed": 320 + i,
+              "sources": (256, 1024, 4096)} for i, a in enumerate((0.5, 0.6))])
+# held out: R-MAT scale 25 and an Erdos-Renyi graph with the same V and pairs
+C5_HELDOUT = [{"kind": "rmat", "scale": 25, "ef": 32, "a": 0.57, "seed": 777},
+              {"kind": "er", "V": 1 << 25, "pairs": 32 << 25, "seed": 777}]
+PRESETS = {"small": (TRAIN_SPECS, HELDOUT_SPECS), "large": (LARGE_SPECS, LARGE_HELDOUT),
+           "c5": (C5_SPECS, C5_HELDOUT)}
 
 
 def evaluate(s: Scheduler, specs, sources=(1024, 4096), k_max=4, reps=3, log=print):
@@ -173,28 +186,38 @@ def main(argv=None):
     ap.add_argument("--out", default=str(WEIGHTS))
     ap.add_argument("--epochs", type=int, default=2000)
     ap.add_argument("--quick", action="store_true")
-    ap.add_argument("--large", action="store_true",
-                    help="train on R-MAT 20-22 (EF 32), evaluate on held-out R-MAT 23 and an "
-                         "Erdos-Renyi graph of the same size")
+    ap.add_argument("--large", action="store_true", help="same as --preset large")
+    ap.add_argument("--preset", choices=sorted(PRESETS), default="small",
+                    help="small: R-MAT 18-21 / held-out 22 + ER; large: 20-22 / 23 + ER; "
+                         "c5: 22-24 / held-out R-MAT 25 + an Erdos-Renyi graph of its size")
     ap.add_argument("--eval-only", action="store_true", help="evaluate the saved weights")
+    ap.add_argument("--no-eval", action="store_true", help="train and save only")
+    ap.add_argument("--traces", help="also save the traces (X, Y, M) to this .npz")
     args = ap.parse_args(argv)
+    preset = "large" if args.large else args.preset
+    specs, held = PRESETS[preset]
     log = lambda m: print(f"[sched_train] {m}", file=sys.stderr, flush=True)
     if args.eval_only:
         s = load(Path(args.out))
-        held = LARGE_HELDOUT if args.large else HELDOUT_SPECS
         rows = evaluate(s, held, log=log)
         print(json.dumps({"heldout": rows}))
         return
-    specs = LARGE_SPECS if args.large else (TRAIN_SPECS[::4] if args.quick else TRAIN_SPECS)
+    if args.quick:
+        specs = specs[::4]
     X, Y, M = collect(specs, log=log)
+    if args.traces:
+        np.savez(args.traces, X=X, Y=Y, M=M)
     log(f"training on {len(X)} samples")
     s = train(X, Y, M, epochs=args.epochs)
     log(f"loss {s.curve[0]:.4f} -> {s.curve[-1]:.4f}")
     acc = accuracy(s, X, Y)
     log(f"train accuracy {acc}")
     save(s, Path(args.out), {"samples": int(len(X)), "train_acc": acc,
-                             "specs": specs, "epochs": args.epochs})
-    held = LARGE_HELDOUT if args.large else HELDOUT_SPECS
+                             "specs": specs, "epochs": args.epochs, "preset": preset})
+    if args.no_eval:
+        print(json.dumps({"samples": int(len(X)), "final_loss": float(s.curve[-1]),
+                          "train": acc}))
+        return
     rows = evaluate(s, held, log=log)
     print(json.dumps({"samples": int(len(X)), "final_loss": float(s.curve[-1]),
                       "train": acc, "heldout": rows}))
