@@ -34,6 +34,7 @@ constexpr uint32_t kMaxSlots = 64;
 
 struct SlotDesc {
   uint64_t *F[2];
+  uint64_t *Fm[2];   // word masks of F[0] / F[1] (V each)
   uint64_t *bm[2];
   uint64_t *N;
   uint64_t *touched;
@@ -45,7 +46,7 @@ struct SlotDesc {
   uint32_t tile;     // global tile of the current wave, UINT32_MAX if idle
   int32_t mode;      // this hop: 0 skip, 1 push, 2 pull
   uint32_t dense;    // this hop: pull without the active-bitmap lookup
-  uint32_t pad;
+  uint32_t masked;   // this hop: pull reads the frontier's word masks
 };
 
 struct TileInfo {
@@ -110,8 +111,10 @@ __device__ __forceinline__ HopArgs slot_args(const BatchConst &c, const SlotDesc
   a.xoff = c.xoff;
   a.xadj = c.xadj;
   a.F = sd.F[(hop - 1) & 1];
+  a.Fm = sd.masked ? sd.Fm[(hop - 1) & 1] : nullptr;
   a.actbm = sd.bm[(hop - 1) & 1];
   a.Fn = sd.F[hop & 1];
+  a.Fnm = sd.Fm[hop & 1];
   a.actbm_n = sd.bm[hop & 1];
   a.N = sd.N;
   a.touched = sd.touched;
@@ -186,7 +189,11 @@ __global__ void k_bw_l0(BatchConst c) {
     const uint64_t deg = c.xoff[v + 1] - c.xoff[v];
     HopCounters *h = &sd.ctr[0];
     const ull oldF = atomicOr(reinterpret_cast<ull *>(&sd.F[0][wi]), bit);
-    if (oldF == 0) atomicAdd(&h->nz, 1ull);
+    if (oldF == 0) {
+      atomicAdd(&h->nz, 1ull);
+      atomicAdd(&h->nzdeg, static_cast<ull>(deg));
+    }
+    atomicOr(reinterpret_cast<ull *>(&sd.Fm[0][v]), 1ull << j);
     if (sd.Vis) {
       const ull oldV = atomicOr(reinterpret_cast<ull *>(&sd.Vis[wi]), bit);
       if (oldV == 0) atomicAdd(&h->visnew, 1ull);
@@ -253,6 +260,8 @@ __global__ __launch_bounds__(64) void k_bw_decide(BatchConst c, uint32_t hop) {
     sd.mode = mode;
     // most in-edges come from active vertices: pull skips the bitmap lookup
     sd.dense = (mode == 2 && sd.ctr[hop - 1].actdeg * 4 > 3 * c.E) ? 1u : 0u;
+    // rows sparse in words: gather only the words their masks name
+    sd.masked = (sd.ctr[hop - 1].nzdeg * 2 < sd.ctr[hop - 1].actdeg * c.TW) ? 1u : 0u;
     if (mode == 1) atomicAdd(&c.ctl->push_dec, 1ull);
     if (mode == 2) atomicAdd(&c.ctl->pull_dec, 1ull);
   }
@@ -292,7 +301,8 @@ __global__ void k_bw_plan(BatchConst c, uint32_t hop) {
 template <int TW, bool SHORTEST>
 __global__ __launch_bounds__(kBlock) void k_bw_expand(BatchConst c, uint32_t hop) {
   __shared__ uint32_t s_nbr[kWavesPerBlock][64];
-  const int wid = threadIdx.x >> 6;
+  __shared__ uint64_t s_nm[kWavesPerBlock][64];
+  const int wid = wave_id();
   const ull total = c.ctl->unit_prefix[c.nslots];
   const ull nwaves = static_cast<ull>(gridDim.x) * kWavesPerBlock;
   uint32_t cur = UINT32_MAX;
@@ -323,7 +333,7 @@ __global__ __launch_bounds__(kBlock) void k_bw_expand(BatchConst c, uint32_t hop
         pre = pull_head<TW, SHORTEST>(a, a.pitems[nu - c.ctl->unit_prefix[s]], j);
         pre_u = nu;
       }
-      pull_body<TW, SHORTEST>(a, h, s_nbr[wid], mk, lc);
+      pull_body<TW, SHORTEST>(a, h, s_nbr[wid], s_nm[wid], mk, lc);
     } else {
       push_item<TW>(a, a.items[local]);
     }
@@ -334,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void k_bw_expand(BatchConst c, uint32_t hop
 // ---- push epilogue and frontier clear, grid-stride over (slot, vertex block)
 template <int TW, bool SHORTEST>
 __global__ __launch_bounds__(kBlock) void k_bw_epi(BatchConst c, uint32_t hop) {
-  const int wid = threadIdx.x >> 6;
+  const int wid = wave_id();
   const uint32_t nvb = (c.V + 63) / 64;
   const ull total = static_cast<ull>(c.nslots) * nvb;
   const ull nwaves = static_cast<ull>(gridDim.x) * kWavesPerBlock;
@@ -360,14 +370,15 @@ __global__ __launch_bounds__(kBlock) void k_bw_epi(BatchConst c, uint32_t hop) {
 
 template <int TW>
 __global__ __launch_bounds__(kBlock) void k_bw_clear(BatchConst c, uint32_t which) {
-  const int wid = threadIdx.x >> 6;
+  const int wid = wave_id();
   const uint32_t nvb = (c.V + 63) / 64;
   const ull total = static_cast<ull>(c.nslots) * nvb;
   const ull nwaves = static_cast<ull>(gridDim.x) * kWavesPerBlock;
   for (ull u = blockIdx.x * kWavesPerBlock + wid; u < total; u += nwaves) {
     const SlotDesc &sd = c.slots[u / nvb];
     if (sd.tile == UINT32_MAX) continue;
-    clear_block<TW>(sd.F[which & 1], sd.bm[which & 1], static_cast<uint32_t>(u % nvb));
+    clear_block<TW>(sd.F[which & 1], sd.Fm[which & 1], sd.bm[which & 1],
+                    static_cast<uint32_t>(u % nvb));
   }
 }
 
@@ -377,7 +388,7 @@ __global__ __launch_bounds__(kBlock) void k_bw_rows(BatchConst c) {
   constexpr int G = 64 / TW;
   constexpr uint32_t CV = 8192;
   constexpr int NB = 14;
-  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const int lane = lane_id(), wid = wave_id();
   const int g = lane / TW, j = lane % TW;
   const uint32_t nch = (c.V + CV - 1) / CV;
   const ull total = static_cast<ull>(c.nslots) * nch;
@@ -716,6 +727,7 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   const size_t mat = static_cast<size_t>(V) * TW * 8;
   const size_t items_cap = static_cast<size_t>(V) + push->E / kPushChunk + 64;
   const size_t per_slot = mat * (shortest ? 4 : 3) + (use_r ? mat : 0) + 3 * nvb * 8 +
+                          2 * static_cast<size_t>(V) * 8 +
                           items_cap * 8 + (H + 1) * sizeof(HopCounters);
   size_t freeb = 0, totalb = 0;
   VS_HIP(hipMemGetInfo(&freeb, &totalb));
@@ -747,7 +759,7 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   VS_HIP(hipMemcpyAsync(d_tiles.p, tiles.data(), T * sizeof(TileInfo), hipMemcpyHostToDevice, s));
   // slot buffers (clean: zero once, kept clean by the kernels)
   std::vector<DevVec> bufs;
-  bufs.reserve(nslots * 16);  // 11 buffers per slot; moves are safe anyway
+  bufs.reserve(nslots * 16);  // 13 buffers per slot; moves are safe anyway
   std::vector<SlotDesc> sd(nslots);
   auto get = [&](size_t bytes, bool zero, void **out) -> vs_status {
     bufs.emplace_back();
@@ -760,6 +772,8 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
     void *q;
     VS_TRY(get(mat, true, &q)); sd[i].F[0] = static_cast<uint64_t *>(q);
     VS_TRY(get(mat, true, &q)); sd[i].F[1] = static_cast<uint64_t *>(q);
+    VS_TRY(get(V * 8ull, true, &q)); sd[i].Fm[0] = static_cast<uint64_t *>(q);
+    VS_TRY(get(V * 8ull, true, &q)); sd[i].Fm[1] = static_cast<uint64_t *>(q);
     VS_TRY(get(nvb * 8ull, true, &q)); sd[i].bm[0] = static_cast<uint64_t *>(q);
     VS_TRY(get(nvb * 8ull, true, &q)); sd[i].bm[1] = static_cast<uint64_t *>(q);
     VS_TRY(get(mat, true, &q)); sd[i].N = static_cast<uint64_t *>(q);

@@ -136,6 +136,12 @@ namespace vs {
 
 // ----------------------------------------------------------- device helpers
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+// Wave index within the block.  readfirstlane tells the compiler the value is
+// wave-uniform, so the work-item indices derived from it, and what is loaded
+// through them, live in scalar registers.
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+}
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
   // number of set bits of m below this lane

@@ -241,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_rowcount(const uint64_t *__restrict_
   constexpr int G = 64 / TW;
   constexpr uint32_t CV = 8192;  // vertices per chunk
   constexpr int NB = 14;         // CV / G + 1 <= 2^14
-  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const int lane = lane_id(), wid = wave_id();
   const int g = lane / TW, j = lane % TW;
   const uint32_t nch = (V + CV - 1) / CV;
   const uint64_t total = static_cast<uint64_t>(ntiles) * nch;
@@ -284,7 +284,7 @@ template <int TW>
 __global__ __launch_bounds__(kBlock) void k_rowdigest(const uint64_t *__restrict__ M,
                                                      uint32_t V, uint32_t S,
                                                      uint32_t ntiles, ull *dig) {
-  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const int lane = lane_id(), wid = wave_id();
   const uint32_t nvb = (V + 63) / 64;
   constexpr uint32_t BPW = 16;  // vertex blocks per work item
   const uint32_t nitem_v = (nvb + BPW - 1) / BPW;
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_to_rowmajor(const uint64_t *__restri
                                                        uint32_t V, uint32_t S,
                                                        uint32_t W, uint64_t *out,
                                                        uint64_t pitch) {
-  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const int lane = lane_id(), wid = wave_id();
   const uint32_t nvb = (V + 63) / 64;
   const uint64_t total = static_cast<uint64_t>(W) * nvb;
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;

@@ -22,6 +22,13 @@
 // all-zero outside their active rows, and are cleared through the bitmap
 // after use, so no hop pays for a full-matrix memset.
 //
+// Word masks: with Fm / Fnm set, every frontier buffer carries one 64-bit
+// mask per vertex whose bit j says that word j of the vertex's row is
+// nonzero (written with the row, cleared with it).  A pull lane reads the
+// mask of each compacted neighbour and loads only the words that hold bits:
+// sparse frontier rows (hop 2 gathers rows of the sources' neighbours, a few
+// words each) then cost a few cache lines instead of TW words.
+//
 // Row masks: `live` bits are the rows that still expand at this hop (rows
 // < S, and in a batch the rows whose query has k_max >= hop); `racc` bits are
 // the rows whose result takes this hop's layer (k_min <= hop); `live_next`
@@ -42,6 +49,8 @@ struct HopCounters {
   ull visnew;  // visited words that went from zero to nonzero
   ull rnew;    // result words that went from zero to nonzero
   ull items;   // push work items built from L_i (written by the item builder)
+  ull nzdeg;   // sum over active vertices of push-degree * nonzero words: the
+               // words a full next-hop pull would gather, by frequency
 };
 
 struct HopArgs {
@@ -55,7 +64,10 @@ struct HopArgs {
   const uint32_t *xadj;
   const uint64_t *F;     // frontier L_{i-1}, V*TW words
   const uint64_t *actbm; // its active-vertex bitmap, ceil(V/64) words
+  const uint64_t *Fm;    // its word masks (V) when pull should read them (sparse
+                         // rows), or nullptr: gather every word
   uint64_t *Fn;          // L_i (all zero on entry)
+  uint64_t *Fnm;         // word masks of L_i (all zero on entry), or nullptr
   uint64_t *actbm_n;     // its bitmap (all zero on entry)
   uint64_t *N;           // push accumulator (all zero on entry and exit)
   uint64_t *touched;     // push touched bitmap (all zero on entry and exit)
@@ -79,7 +91,7 @@ struct HopArgs {
 };
 
 struct LocalCtr {
-  uint64_t pop = 0, nz = 0, act = 0, actdeg = 0, te = 0, visnew = 0, rnew = 0;
+  uint64_t pop = 0, nz = 0, act = 0, actdeg = 0, te = 0, visnew = 0, rnew = 0, nzdeg = 0;
   uint64_t rowor = 0;  // OR of the words this lane emitted (its word index j)
 };
 
@@ -89,7 +101,8 @@ __device__ __forceinline__ void flush_ctr(LocalCtr &c, HopCounters *ctr,
     atomicOr(reinterpret_cast<ull *>(&front_next[j]), static_cast<ull>(c.rowor));
   const uint64_t pop = wave_sum(c.pop), nz = wave_sum(c.nz), act = wave_sum(c.act),
                  actdeg = wave_sum(c.actdeg), te = wave_sum(c.te),
-                 visnew = wave_sum(c.visnew), rnew = wave_sum(c.rnew);
+                 visnew = wave_sum(c.visnew), rnew = wave_sum(c.rnew),
+                 nzdeg = wave_sum(c.nzdeg);
   if (lane_id() == 0) {
     if (pop) atomicAdd(&ctr->pop, static_cast<ull>(pop));
     if (nz) atomicAdd(&ctr->nz, static_cast<ull>(nz));
@@ -98,6 +111,7 @@ __device__ __forceinline__ void flush_ctr(LocalCtr &c, HopCounters *ctr,
     if (te) atomicAdd(&ctr->te, static_cast<ull>(te));
     if (visnew) atomicAdd(&ctr->visnew, static_cast<ull>(visnew));
     if (rnew) atomicAdd(&ctr->rnew, static_cast<ull>(rnew));
+    if (nzdeg) atomicAdd(&ctr->nzdeg, static_cast<ull>(nzdeg));
   }
   c = LocalCtr{};
 }
@@ -184,35 +198,48 @@ __device__ __forceinline__ uint64_t gather(const HopArgs &a, uint32_t u, int j) 
 // so stopping on the partial word is exact for TW = 64 and conservative
 // below).  Lanes stop independently; the hubs at the head of the list are
 // gathered first, so most lanes that can stop do so before the streaming part.
-template <int TW, bool CACHED>
-__device__ __forceinline__ void gather_range(const HopArgs &a, const uint32_t *nbr, uint32_t &k,
-                                             uint32_t end, int j, uint64_t need,
-                                             uint64_t &acc) {
+// Word j of neighbour slot k; with MASKED, only when the neighbour's word
+// mask says the word holds bits (read as the 32-bit half that has bit j).
+template <int TW, bool CACHED, bool MASKED>
+__device__ __forceinline__ uint64_t gather_m(const HopArgs &a, const uint32_t *nbr,
+                                             const uint64_t *nm, uint32_t k, int j) {
+  if constexpr (MASKED) {
+    const uint32_t half = reinterpret_cast<const uint32_t *>(nm)[2 * k + (j >> 5)];
+    return ((half >> (j & 31)) & 1) ? gather<TW, CACHED>(a, nbr[k], j) : 0;
+  } else {
+    return gather<TW, CACHED>(a, nbr[k], j);
+  }
+}
+
+template <int TW, bool CACHED, bool MASKED>
+__device__ __forceinline__ void gather_range(const HopArgs &a, const uint32_t *nbr,
+                                             const uint64_t *nm, uint32_t &k, uint32_t end,
+                                             int j, uint64_t need, uint64_t &acc) {
   constexpr int G = 64 / TW;
   for (; k + 7 * G < end; k += 8 * G) {
     if ((acc & need) == need) return;
     uint64_t x[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) x[q] = gather<TW, CACHED>(a, nbr[k + q * G], j);
+    for (int q = 0; q < 8; ++q) x[q] = gather_m<TW, CACHED, MASKED>(a, nbr, nm, k + q * G, j);
     acc |= ((x[0] | x[1]) | (x[2] | x[3])) | ((x[4] | x[5]) | (x[6] | x[7]));
   }
   if (k + 3 * G < end && (acc & need) != need) {
-    const uint64_t x0 = gather<TW, CACHED>(a, nbr[k], j);
-    const uint64_t x1 = gather<TW, CACHED>(a, nbr[k + G], j);
-    const uint64_t x2 = gather<TW, CACHED>(a, nbr[k + 2 * G], j);
-    const uint64_t x3 = gather<TW, CACHED>(a, nbr[k + 3 * G], j);
+    const uint64_t x0 = gather_m<TW, CACHED, MASKED>(a, nbr, nm, k, j);
+    const uint64_t x1 = gather_m<TW, CACHED, MASKED>(a, nbr, nm, k + G, j);
+    const uint64_t x2 = gather_m<TW, CACHED, MASKED>(a, nbr, nm, k + 2 * G, j);
+    const uint64_t x3 = gather_m<TW, CACHED, MASKED>(a, nbr, nm, k + 3 * G, j);
     acc |= (x0 | x1) | (x2 | x3);
     k += 4 * G;
   }
   for (; k < end; k += G) {
     if ((acc & need) == need) return;
-    acc |= gather<TW, CACHED>(a, nbr[k], j);
+    acc |= gather_m<TW, CACHED, MASKED>(a, nbr, nm, k, j);
   }
 }
 
 template <int TW, bool SHORTEST>
 __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, uint32_t *nbr,
-                                          const LaneMasks &mk, LocalCtr &c) {
+                                          uint64_t *nm, const LaneMasks &mk, LocalCtr &c) {
   constexpr int G = 64 / TW;
   const int lane = lane_id();
   const int g = lane / TW, j = lane % TW;
@@ -243,12 +270,21 @@ __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, u
     // hub neighbours precede the others in the list, so after compaction the
     // first nh slots are hubs
     const uint32_t nh = __popcll(__ballot(act && idx < hub_end));
-    if (act) nbr[lane_rank(m)] = u;
+    if (act) {
+      const uint32_t r = lane_rank(m);
+      nbr[r] = u;
+      if (a.Fm) nm[r] = a.Fm[u];
+    }
     wave_sync();
     if (todo) {
       uint32_t k = g;
-      gather_range<TW, true>(a, nbr, k, nh, j, need, acc);
-      gather_range<TW, false>(a, nbr, k, n, j, need, acc);
+      if (a.Fm) {
+        gather_range<TW, true, true>(a, nbr, nm, k, nh, j, need, acc);
+        gather_range<TW, false, true>(a, nbr, nm, k, n, j, need, acc);
+      } else {
+        gather_range<TW, true, false>(a, nbr, nm, k, nh, j, need, acc);
+        gather_range<TW, false, false>(a, nbr, nm, k, n, j, need, acc);
+      }
     }
     wave_sync();
     acc = group_or<TW>(acc);
@@ -257,7 +293,9 @@ __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, u
   }
   acc = group_or<TW>(acc);
   const uint64_t nb = acc & need;
-  const bool row_new = __ballot(g == 0 && nb != 0) != 0;
+  // bits 0..TW-1: the words of v's new row that hold bits (group 0's lanes)
+  const uint64_t wmask = __ballot(g == 0 && nb != 0);
+  const bool row_new = wmask != 0;
   if (heavy) {
     if (g == 0 && nb)
       atomicOr(reinterpret_cast<ull *>(&a.N[wi]), static_cast<ull>(nb));
@@ -272,6 +310,8 @@ __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, u
       c.act += 1;
       c.actdeg += deg;
       atomicOr(reinterpret_cast<ull *>(&a.actbm_n[v >> 6]), 1ull << (v & 63));
+      if (a.Fnm) a.Fnm[v] = wmask;
+      c.nzdeg += deg * __popcll(wmask);
     }
     if (g == 0) c.te += deg * __popcll(nb & mk.live_next);
   }
@@ -340,6 +380,8 @@ __device__ __forceinline__ void epi_block(const HopArgs &a, uint32_t vb,
         actmask |= 1ull << i;
         c.act += 1;
         c.actdeg += deg;
+        c.nzdeg += deg * __popcll(gb);
+        if (a.Fnm) a.Fnm[v] = gb;
       }
       c.te += deg * __popcll(nb & mk.live_next);
     }
@@ -352,9 +394,11 @@ __device__ __forceinline__ void epi_block(const HopArgs &a, uint32_t vb,
   }
 }
 
-// Zero the active rows of a frontier buffer and its bitmap word vb.
+// Zero the active rows of a frontier buffer, their word masks (Fm may be
+// nullptr) and its bitmap word vb.
 template <int TW>
-__device__ __forceinline__ void clear_block(uint64_t *F, uint64_t *actbm, uint32_t vb) {
+__device__ __forceinline__ void clear_block(uint64_t *F, uint64_t *Fm, uint64_t *actbm,
+                                            uint32_t vb) {
   constexpr int G = 64 / TW;
   const int lane = lane_id();
   const int g = lane / TW, j = lane % TW;
@@ -363,7 +407,10 @@ __device__ __forceinline__ void clear_block(uint64_t *F, uint64_t *actbm, uint32
   while (m) {
     const int i = nth_set_bit(m, g);
     m = drop_low_bits(m, G);
-    if (i >= 0) F[(static_cast<size_t>(vb) * 64 + i) * TW + j] = 0;
+    if (i >= 0) {
+      F[(static_cast<size_t>(vb) * 64 + i) * TW + j] = 0;
+      if (Fm && j == 0) Fm[static_cast<size_t>(vb) * 64 + i] = 0;
+    }
   }
   if (lane == 0) actbm[vb] = 0;
 }
@@ -381,7 +428,7 @@ __device__ __forceinline__ void items_slice(const uint64_t *actbm, uint32_t nvb,
                                             uint32_t base, const uint64_t *xoff,
                                             ull *items, ull *nitems, ull *s_wsum,
                                             ull *s_base) {
-  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const int lane = lane_id(), wid = wave_id();
   const uint32_t wb = base + threadIdx.x;
   const uint64_t m = wb < nvb ? actbm[wb] : 0;
   ull cnt = 0;
@@ -422,7 +469,8 @@ __device__ __forceinline__ void items_slice(const uint64_t *actbm, uint32_t nvb,
 template <int TW, bool SHORTEST>
 __global__ __launch_bounds__(kBlock) void k_pull(HopArgs a) {
   __shared__ uint32_t s_nbr[kWavesPerBlock][64];
-  const int wid = threadIdx.x >> 6;
+  __shared__ uint64_t s_nm[kWavesPerBlock][64];
+  const int wid = wave_id();
   const LaneMasks mk = lane_masks(a, lane_id() % TW);
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
   LocalCtr c;
@@ -435,7 +483,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(HopArgs a) {
     const uint64_t nx = it + nwaves;
     PullHead hn{};
     if (nx < n) hn = pull_head<TW, SHORTEST>(a, a.pitems[nx], j);
-    pull_body<TW, SHORTEST>(a, h, s_nbr[wid], mk, c);
+    pull_body<TW, SHORTEST>(a, h, s_nbr[wid], s_nm[wid], mk, c);
     h = hn;
     it = nx;
   }
@@ -444,7 +492,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(HopArgs a) {
 
 template <int TW>
 __global__ __launch_bounds__(kBlock) void k_push(HopArgs a) {
-  const int wid = threadIdx.x >> 6;
+  const int wid = wave_id();
   const uint64_t nitems = *a.nitems;
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
   for (uint64_t it = blockIdx.x * kWavesPerBlock + wid; it < nitems; it += nwaves)
@@ -453,7 +501,7 @@ __global__ __launch_bounds__(kBlock) void k_push(HopArgs a) {
 
 template <int TW, bool SHORTEST>
 __global__ __launch_bounds__(kBlock) void k_push_epi(HopArgs a) {
-  const int wid = threadIdx.x >> 6;
+  const int wid = wave_id();
   const LaneMasks mk = lane_masks(a, lane_id() % TW);
   const uint32_t nvb = (a.V + 63) / 64;
   const uint32_t nwaves = gridDim.x * kWavesPerBlock;
@@ -464,13 +512,13 @@ __global__ __launch_bounds__(kBlock) void k_push_epi(HopArgs a) {
 }
 
 template <int TW>
-__global__ __launch_bounds__(kBlock) void k_clear(uint64_t *F, uint64_t *actbm,
+__global__ __launch_bounds__(kBlock) void k_clear(uint64_t *F, uint64_t *Fm, uint64_t *actbm,
                                                  uint32_t V) {
-  const int wid = threadIdx.x >> 6;
+  const int wid = wave_id();
   const uint32_t nvb = (V + 63) / 64;
   const uint32_t nwaves = gridDim.x * kWavesPerBlock;
   for (uint32_t vb = blockIdx.x * kWavesPerBlock + wid; vb < nvb; vb += nwaves)
-    clear_block<TW>(F, actbm, vb);
+    clear_block<TW>(F, Fm, actbm, vb);
 }
 
 __global__ __launch_bounds__(kBlock) void k_items(const uint64_t *actbm, uint32_t V,
@@ -484,7 +532,7 @@ void launch_push(uint32_t tw, bool shortest, uint32_t grid, hipStream_t s,
                  const HopArgs &a);
 void launch_epi(uint32_t tw, bool shortest, uint32_t grid, hipStream_t s,
                 const HopArgs &a);
-void launch_clear(uint32_t tw, uint32_t grid, hipStream_t s, uint64_t *F,
+void launch_clear(uint32_t tw, uint32_t grid, hipStream_t s, uint64_t *F, uint64_t *Fm,
                   uint64_t *bm, uint32_t V);
 void launch_items(uint32_t grid, hipStream_t s, const uint64_t *bm, uint32_t V,
                   const uint64_t *xoff, ull *items, ull *nitems);

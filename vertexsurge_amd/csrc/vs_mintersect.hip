@@ -61,7 +61,7 @@ __device__ __forceinline__ uint64_t lower_bound_g(const uint32_t *__restrict__ a
 
 template <bool FILL>
 __global__ __launch_bounds__(kBlock) void k_mintersect(MiArgs a) {
-  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const int lane = lane_id(), wid = wave_id();
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
   for (uint64_t t = blockIdx.x * kWavesPerBlock + wid; t < a.T; t += nwaves) {
     uint64_t b[kMaxArity], e[kMaxArity];

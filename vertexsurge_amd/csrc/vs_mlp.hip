@@ -33,7 +33,7 @@ __global__ __launch_bounds__(kBlock) void k_mlp_predict(const float *P, const fl
                                                        uint32_t n, float *Y) {
   __shared__ float s_h[kWavesPerBlock][2 * kHid];
   __shared__ float s_x[kWavesPerBlock][kIn];
-  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const int lane = lane_id(), wid = wave_id();
   const uint32_t nwaves = gridDim.x * kWavesPerBlock;
   for (uint32_t s = blockIdx.x * kWavesPerBlock + wid; s < n; s += nwaves) {
     if (lane < kIn) s_x[wid][lane] = X[static_cast<size_t>(s) * kIn + lane];
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(kBlock) void k_mlp_grad(const float *P, const float
   __shared__ float s_dh2[kWavesPerBlock][kHid];
   __shared__ float s_red[kNumParams];
   __shared__ float s_loss[kWavesPerBlock];
-  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const int lane = lane_id(), wid = wave_id();
   const int k = lane;
   const uint32_t nwaves = gridDim.x * kWavesPerBlock;
   float gW1[kIn], gW2[kHid], gW3c[kOut];

@@ -107,7 +107,7 @@ __global__ __launch_bounds__(kBlock) void k_colcount_native(const uint64_t *__re
                                                             uint32_t V, uint32_t ntiles,
                                                             uint32_t *__restrict__ counts) {
   constexpr int G = 64 / TW;
-  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const int lane = lane_id(), wid = wave_id();
   const int g = lane / TW, j = lane % TW;
   const uint64_t nitems = (static_cast<uint64_t>(V) + G - 1) / G;
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void k_colcount_cm(const uint64_t *__restri
                                                         uint32_t cols, uint64_t pitch,
                                                         uint64_t used,
                                                         uint32_t *__restrict__ counts) {
-  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const int lane = lane_id(), wid = wave_id();
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
   for (uint64_t c = blockIdx.x * kWavesPerBlock + wid; c < cols; c += nwaves) {
     uint64_t s = 0;
@@ -204,7 +204,7 @@ __device__ __forceinline__ uint64_t and_word(const McArgs &a, const uint32_t *ci
 // Count pass: one wave per (tuple, chunk) item of the batch starting at t0.
 __global__ __launch_bounds__(kBlock) void k_mc_count(McArgs a, uint64_t t0, uint64_t nitems,
                                                      uint32_t *__restrict__ part) {
-  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const int lane = lane_id(), wid = wave_id();
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
   for (uint64_t it = blockIdx.x * kWavesPerBlock + wid; it < nitems; it += nwaves) {
     const uint64_t t = t0 + it / a.nchunk, ch = it % a.nchunk;
@@ -238,7 +238,7 @@ __global__ void k_mc_sum(const uint32_t *__restrict__ part, uint64_t nchunk, uin
 __global__ __launch_bounds__(kBlock) void k_mc_fill(McArgs a, uint64_t t0, uint64_t nitems,
                                                     const uint64_t *__restrict__ pscan,
                                                     uint64_t base, uint32_t *__restrict__ lists) {
-  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const int lane = lane_id(), wid = wave_id();
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
   for (uint64_t it = blockIdx.x * kWavesPerBlock + wid; it < nitems; it += nwaves) {
     const uint64_t t = t0 + it / a.nchunk, ch = it % a.nchunk;

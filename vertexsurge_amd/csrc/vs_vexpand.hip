@@ -34,7 +34,8 @@ __global__ __launch_bounds__(kBlock) void k_items(const uint64_t *actbm, uint32_
 // L0: row r has only its source vertex (ExpandStatus::init_first_m_layer,
 // src/include/operator/expand_impl.hh:250-277).
 __global__ void k_init(const uint32_t *src, uint32_t S, uint32_t V, uint32_t TW,
-                       uint64_t *F, uint64_t *actbm, uint64_t *Vis, uint64_t *R) {
+                       uint64_t *F, uint64_t *Fm, uint64_t *actbm, uint64_t *Vis,
+                       uint64_t *R) {
   const uint32_t nvb = (V + 63) / 64;
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < S;
        r += gridDim.x * blockDim.x) {
@@ -43,6 +44,7 @@ __global__ void k_init(const uint32_t *src, uint32_t S, uint32_t V, uint32_t TW,
     const ull bit = 1ull << (r % 64);
     const size_t wi = (static_cast<size_t>(t) * V + s) * TW + j;
     atomicOr(reinterpret_cast<ull *>(&F[wi]), bit);
+    atomicOr(reinterpret_cast<ull *>(&Fm[static_cast<size_t>(t) * V + s]), 1ull << j);
     if (Vis) atomicOr(reinterpret_cast<ull *>(&Vis[wi]), bit);
     if (R) atomicOr(reinterpret_cast<ull *>(&R[wi]), bit);
     atomicOr(reinterpret_cast<ull *>(&actbm[static_cast<size_t>(t) * nvb + s / 64]),
@@ -129,14 +131,14 @@ void launch_epi(uint32_t tw, bool shortest, uint32_t grid, hipStream_t s,
 }
 
 template <int TW>
-static void launch_clear_tw(uint32_t grid, hipStream_t s, uint64_t *F, uint64_t *bm,
-                            uint32_t V) {
-  hipLaunchKernelGGL((k_clear<TW>), dim3(grid), dim3(kBlock), 0, s, F, bm, V);
+static void launch_clear_tw(uint32_t grid, hipStream_t s, uint64_t *F, uint64_t *Fm,
+                            uint64_t *bm, uint32_t V) {
+  hipLaunchKernelGGL((k_clear<TW>), dim3(grid), dim3(kBlock), 0, s, F, Fm, bm, V);
 }
 
-void launch_clear(uint32_t tw, uint32_t grid, hipStream_t s, uint64_t *F,
+void launch_clear(uint32_t tw, uint32_t grid, hipStream_t s, uint64_t *F, uint64_t *Fm,
                   uint64_t *bm, uint32_t V) {
-  VS_TW_CALL(tw, launch_clear_tw, grid, s, F, bm, V);
+  VS_TW_CALL(tw, launch_clear_tw, grid, s, F, Fm, bm, V);
 }
 
 void launch_items(uint32_t grid, hipStream_t s, const uint64_t *bm, uint32_t V,
@@ -220,7 +222,10 @@ static void l0_counters(const uint32_t *src, uint32_t S, uint32_t TW, uint32_t t
     const uint64_t d = xoff[s + 1] - xoff[s];
     c.pop += 1;
     c.te += d;
-    if (words.insert((row / 64) << 32 | s).second) c.nz += 1;
+    if (words.insert((row / 64) << 32 | s).second) {
+      c.nz += 1;
+      c.nzdeg += d;
+    }
     if (verts.insert(s).second) {
       c.act += 1;
       c.actdeg += d;
@@ -324,9 +329,15 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
   // frontier buffers: ping-pong, or one per layer when layers are exported
   std::vector<uint64_t *> layer(k_max + 1, nullptr);
   uint64_t *Fbuf[2] = {nullptr, nullptr}, *BM[2] = {nullptr, nullptr};
+  uint64_t *FM[2] = {nullptr, nullptr};  // word masks of the two frontiers in use
   void *tmp = nullptr;
   VS_TRY(lease.get(mat_bytes, true, &tmp));
   Fbuf[0] = static_cast<uint64_t *>(tmp);
+  const size_t wm_bytes = static_cast<size_t>(T) * V * 8;
+  VS_TRY(lease.get(wm_bytes, true, &tmp));
+  FM[0] = static_cast<uint64_t *>(tmp);
+  VS_TRY(lease.get(wm_bytes, true, &tmp));
+  FM[1] = static_cast<uint64_t *>(tmp);
   VS_TRY(lease.get(bm_bytes, true, &tmp));
   BM[0] = static_cast<uint64_t *>(tmp);
   VS_TRY(lease.get(bm_bytes, true, &tmp));
@@ -384,7 +395,7 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
   {
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((S + 255) / 256, 1024));
     hipLaunchKernelGGL(k_init, dim3(grid), dim3(256), 0, s,
-                       static_cast<const uint32_t *>(d_src), S, V, TW, Fbuf[0],
+                       static_cast<const uint32_t *>(d_src), S, V, TW, Fbuf[0], FM[0],
                        BM[0], Vis, (R && k_min == 0) ? R : nullptr);
     VS_HIP(hipGetLastError());
     for (uint32_t t = 0; t < T; ++t)
@@ -467,8 +478,12 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
       a.xoff = push->off;
       a.xadj = push->adj;
       a.F = F + t * tile_words;
+      // read the word masks only when the rows a pull gathers (weighted by
+      // how often: degree) are mostly empty words
+      a.Fm = pc.nzdeg * 2 < pc.actdeg * TW ? FM[cur] + static_cast<size_t>(t) * V : nullptr;
       a.actbm = bmF + static_cast<size_t>(t) * nvb;
       a.Fn = Fn + t * tile_words;
+      a.Fnm = FM[cur ^ 1] + static_cast<size_t>(t) * V;
       a.actbm_n = bmN + static_cast<size_t>(t) * nvb;
       a.Vis = Vis ? Vis + t * tile_words : nullptr;
       a.R = acc_r ? R + t * tile_words : nullptr;
@@ -583,10 +598,12 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
     // frontier L_{hop-1} is consumed: zero its rows (ping-pong) or its bitmap
     if (!with_distance) {
       for (uint32_t t = 0; t < T; ++t)
-        launch_clear(TW, clear_grid, s, F + t * tile_words, bmF + static_cast<size_t>(t) * nvb, V);
+        launch_clear(TW, clear_grid, s, F + t * tile_words, FM[cur] + static_cast<size_t>(t) * V,
+                     bmF + static_cast<size_t>(t) * nvb, V);
       VS_HIP(hipGetLastError());
     } else {
       VS_HIP(hipMemsetAsync(bmF, 0, bm_bytes, s));
+      VS_HIP(hipMemsetAsync(FM[cur], 0, wm_bytes, s));
     }
     cur ^= 1;
     if (stop) break;
@@ -638,14 +655,18 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
   if (!with_distance) {
     uint64_t *F = Fbuf[cur];
     for (uint32_t t = 0; t < T; ++t)
-      launch_clear(TW, clear_grid, s, F + t * tile_words, BM[cur] + static_cast<size_t>(t) * nvb, V);
+      launch_clear(TW, clear_grid, s, F + t * tile_words, FM[cur] + static_cast<size_t>(t) * V,
+                   BM[cur] + static_cast<size_t>(t) * nvb, V);
     VS_HIP(hipGetLastError());
     lease.mark(Fbuf[0], true);
     lease.mark(Fbuf[1], true);
   } else {
     VS_HIP(hipMemsetAsync(BM[cur], 0, bm_bytes, s));
+    VS_HIP(hipMemsetAsync(FM[cur], 0, wm_bytes, s));
     // layers that are not exported go back dirty (default)
   }
+  lease.mark(FM[0], true);
+  lease.mark(FM[1], true);
   lease.mark(BM[0], true);
   lease.mark(BM[1], true);
   if (N) {
