@@ -284,6 +284,10 @@ typedef struct {
   double ms_expand;
   uint64_t n_expand;
   uint64_t bytes_alg;
+  /* host clock per replay (reset the wave counter and outputs, launch the
+   * graph for every wave, synchronize), same averaging as ms_total; 0 when
+   * the launch sequence is not captured */
+  double ms_wall;
 } vs_batch_stats;
 
 /* Runs nq independent VExpand queries packed into shared bit-matrix tiles.

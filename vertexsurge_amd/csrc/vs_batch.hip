@@ -625,6 +625,52 @@ struct DevVec {
   }
 };
 
+// key[r] = sum over u in N(src[r]) of deg(u), one wave per source
+__global__ __launch_bounds__(kBlock) void k_src_vol2(const uint64_t *__restrict__ off,
+                                                     const uint32_t *__restrict__ adj,
+                                                     const uint32_t *__restrict__ src,
+                                                     uint32_t S, ull *__restrict__ key) {
+  const int lane = lane_id(), wid = wave_id();
+  const uint32_t nwaves = gridDim.x * kWavesPerBlock;
+  for (uint32_t r = blockIdx.x * kWavesPerBlock + wid; r < S; r += nwaves) {
+    const uint32_t s = src[r];
+    uint64_t k = 0;
+    for (uint64_t e = off[s] + lane; e < off[s + 1]; e += 64) {
+      const uint32_t u = adj[e];
+      k += off[u + 1] - off[u];
+    }
+    k = wave_sum(k);
+    if (lane == 0) key[r] = k;
+  }
+}
+
+static vs_status source_volumes(vs_graph_s *g, const DevCSR *push,
+                                const std::vector<uint32_t> &src, std::vector<uint64_t> &key) {
+  const size_t S = src.size();
+  if (S == 0) return VS_OK;
+  void *ds = nullptr, *dk = nullptr;
+  VS_TRY(g->pool.get(S * 4, false, g->stream, &ds));
+  vs_status st = g->pool.get(S * 8, false, g->stream, &dk);
+  hipError_t e = hipSuccess;
+  if (st == VS_OK) e = hipMemcpyAsync(ds, src.data(), S * 4, hipMemcpyHostToDevice, g->stream);
+  if (st == VS_OK && e == hipSuccess) {
+    hipLaunchKernelGGL(k_src_vol2, dim3(grid_for(S, g->num_cus)), dim3(kBlock), 0, g->stream,
+                       push->off, push->adj, static_cast<const uint32_t *>(ds),
+                       static_cast<uint32_t>(S), static_cast<ull *>(dk));
+    e = hipGetLastError();
+  }
+  if (st == VS_OK && e == hipSuccess)
+    e = hipMemcpyAsync(key.data(), dk, S * 8, hipMemcpyDeviceToHost, g->stream);
+  if (st == VS_OK && e == hipSuccess) e = hipStreamSynchronize(g->stream);
+  if (st == VS_OK && e != hipSuccess) {
+    set_error("batch: source volumes: %s", hipGetErrorString(e));
+    st = VS_ERR_HIP;
+  }
+  g->pool.put(ds, S * 4, false);
+  if (dk) g->pool.put(dk, S * 8, false);
+  return st;
+}
+
 // One group = queries of one path type.
 static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &qs,
                            const vs_batch_params &p, bool shortest,
@@ -669,21 +715,23 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   }
   // Row packing: rows are independent, so they may share words in any order.
   // Rows are grouped by k_max first (a tile runs as many hops as its deepest
-  // row), then ordered by source degree (descending, isolated sources last),
-  // which puts rows that finish at similar hops into the same words: words of
-  // isolated sources are dead after L0 (their lanes never load again), and
-  // words of central sources become fully visited early (their lanes stop
-  // early).  pos[r] is the packed position of group row r.
+  // row), then ordered by the source's 2-hop volume sum_{u in N(s)} deg(u)
+  // (descending; isolated sources last), which puts rows that finish at
+  // similar hops into the same words: words of isolated sources are dead
+  // after L0 (their lanes never load again), and words of central sources
+  // become fully visited early (their lanes stop early).  The keys come from
+  // one small kernel over the sources' lists.  pos[r] is the packed position
+  // of group row r.
   std::vector<uint32_t> pos(S);
   {
     std::vector<uint32_t> order(S);
     for (uint32_t r = 0; r < S; ++r) order[r] = r;
     if (!(p.flags & VS_BATCH_KEEP_ORDER)) {
-      const std::vector<uint64_t> &xo = push->hoff;
+      std::vector<uint64_t> key(S);
+      VS_TRY(source_volumes(g, push, src, key));
       std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
         if (kmax[x] != kmax[y]) return kmax[x] > kmax[y];
-        const uint64_t dx = xo[src[x] + 1] - xo[src[x]], dy = xo[src[y] + 1] - xo[src[y]];
-        return dx > dy;
+        return key[x] > key[y];
       });
     }
     std::vector<uint32_t> s2(S), kn(S), kx(S);
@@ -840,7 +888,7 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   const uint32_t replays = std::max<uint32_t>(1, p.replays);
   const bool timing = (p.flags & VS_VX_TIMING) != 0;  // per-launch events: no capture
   ExpandEvents xev;
-  double replay_ms = 0.0;
+  double replay_ms = 0.0, replay_wall = 0.0;
   uint32_t replay_n = 0;
   if (p.use_graph && !timing) {
     hipGraph_t graph = nullptr;
@@ -863,6 +911,7 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
                          std::chrono::steady_clock::now() - c0).count();
     hipError_t re = hipSuccess;
     for (uint32_t rep = 0; rep < replays && re == hipSuccess; ++rep) {
+      const auto w0 = std::chrono::steady_clock::now();
       if (rep > 0) {  // timing replays: restart from wave 0 with clean outputs
         re = hipMemcpyAsync(d_ctl.p, &ctl0, sizeof(ctl0), hipMemcpyHostToDevice, s);
         if (re == hipSuccess) re = hipMemsetAsync(d_counts.p, 0, S * 8ull, s);
@@ -878,6 +927,8 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
       // only when it is the only one
       if (re == hipSuccess && (rep > 0 || replays == 1)) {
         replay_ms += rms;
+        replay_wall += std::chrono::duration<double, std::milli>(
+                           std::chrono::steady_clock::now() - w0).count();
         replay_n += 1;
       }
     }
@@ -910,6 +961,7 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   float fms = 0.f;
   VS_HIP(hipEventElapsedTime(&fms, e0, e1));
   ms = replay_n ? replay_ms / replay_n : fms;
+  st.ms_wall += replay_n ? replay_wall / replay_n : 0.0;
   BatchCtl ctl{};
   VS_HIP(hipMemcpy(&ctl, d_ctl.p, sizeof(ctl), hipMemcpyDeviceToHost));
   std::vector<uint64_t> hc(S), hd;
