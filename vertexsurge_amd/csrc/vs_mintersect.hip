@@ -10,9 +10,11 @@
 // Here each dependency is a CSR in HBM with sorted lists, so the same set is
 // the intersection of arity sorted lists.  One wave owns one tuple: the
 // shortest list drives, 64 elements per step (one per lane); each lane looks
-// its element up in every other list by binary search inside the sub-range
-// that the step's first and last elements bound; ballot + mbcnt give the
-// ascending output positions.  Two passes: count -> exclusive scan -> fill.
+// its element up in every other list by binary search, the searches of all
+// lists in lockstep (their loads are independent, so they overlap instead of
+// queueing one list after the other), each starting where the previous
+// step's largest element landed; ballot + mbcnt give the ascending output
+// positions.  Two passes: count -> exclusive scan -> fill.
 #include <algorithm>
 #include <cstring>
 
@@ -64,21 +66,26 @@ __global__ __launch_bounds__(kBlock) void k_mintersect(MiArgs a) {
   const int lane = lane_id(), wid = wave_id();
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
   for (uint64_t t = blockIdx.x * kWavesPerBlock + wid; t < a.T; t += nwaves) {
-    uint64_t b[kMaxArity], e[kMaxArity];
+    uint64_t b[kMaxArity];
+    uint32_t n[kMaxArity];
     bool empty = false;
     uint32_t drv = 0;
     uint64_t best = ~0ull;
-    for (uint32_t d = 0; d < a.arity; ++d) {
+#pragma unroll
+    for (int d = 0; d < kMaxArity; ++d) {
+      b[d] = 0;
+      n[d] = 0;
+      if (d >= static_cast<int>(a.arity)) continue;
       const uint32_t x = a.tuples[t * a.arity + d];
       if (x >= a.V[d]) {
         empty = true;
-        b[d] = e[d] = 0;
         continue;
       }
       b[d] = a.off[d][x];
-      e[d] = a.off[d][x + 1];
-      if (e[d] - b[d] < best) {
-        best = e[d] - b[d];
+      const uint64_t len = a.off[d][x + 1] - b[d];
+      n[d] = static_cast<uint32_t>(len);
+      if (len < best) {
+        best = len;
         drv = d;
       }
     }
@@ -86,27 +93,43 @@ __global__ __launch_bounds__(kBlock) void k_mintersect(MiArgs a) {
     uint64_t out = FILL ? a.offsets[t] : 0;
     if (!empty && best > 0) {
       const uint32_t *dl = a.adj[drv];
-      for (uint64_t base = b[drv]; base < e[drv]; base += 64) {
+      uint32_t from[kMaxArity];  // where this step's searches start, per list
+#pragma unroll
+      for (int d = 0; d < kMaxArity; ++d) from[d] = 0;
+      for (uint64_t base = b[drv]; base < b[drv] + best; base += 64) {
         const uint64_t idx = base + lane;
-        const bool have = idx < e[drv];
-        const uint32_t x = have ? dl[idx] : 0u;
-        const uint64_t last = min<uint64_t>(e[drv], base + 64) - 1;
-        const uint32_t xfirst = dl[base], xlast = dl[last];
-        bool keep = have;
-        for (uint32_t d = 0; d < a.arity && __ballot(keep); ++d) {
-          if (d == drv) continue;
-          // the step's elements lie in [xfirst, xlast]: narrow the list once
-          const uint32_t *ol = a.adj[d];
-          uint64_t lo = b[d], hi = e[d];
-          if (lane == 0) lo = lower_bound_g(ol, b[d], e[d], xfirst);
-          if (lane == 1) hi = upper_bound_g(ol, b[d], e[d], xlast);
-          lo = shfl64(lo, 0);
-          hi = shfl64(hi, 1);
-          if (keep) {
-            const uint64_t p = lower_bound_g(ol, lo, hi, x);
-            keep = p < hi && ol[p] == x;
+        const bool have = idx < b[drv] + best;
+        const uint32_t x = have ? dl[idx] : 0xFFFFFFFFu;
+        uint32_t lo[kMaxArity], hi[kMaxArity];
+#pragma unroll
+        for (int d = 0; d < kMaxArity; ++d) {
+          const bool on = d < static_cast<int>(a.arity) && d != static_cast<int>(drv);
+          lo[d] = on ? from[d] : 0;
+          hi[d] = on ? n[d] : 0;
+        }
+        // lockstep lower_bound of x in every other list
+        bool more = have;
+        while (more) {
+          more = false;
+#pragma unroll
+          for (int d = 0; d < kMaxArity; ++d) {
+            if (lo[d] < hi[d]) {
+              const uint32_t mid = (lo[d] + hi[d]) >> 1;
+              if (a.adj[d][b[d] + mid] < x) lo[d] = mid + 1;
+              else hi[d] = mid;
+              more |= lo[d] < hi[d];
+            }
           }
         }
+        bool keep = have;
+#pragma unroll
+        for (int d = 0; d < kMaxArity; ++d) {
+          if (d >= static_cast<int>(a.arity) || d == static_cast<int>(drv)) continue;
+          keep = keep && lo[d] < n[d] && a.adj[d][b[d] + lo[d]] == x;
+        }
+        // the next step's elements are all larger than lane 63's
+#pragma unroll
+        for (int d = 0; d < kMaxArity; ++d) from[d] = __shfl(lo[d], 63);
         const uint64_t m = __ballot(keep);
         if (FILL && keep) a.lists[out + lane_rank(m)] = x;
         out += __popcll(m);
