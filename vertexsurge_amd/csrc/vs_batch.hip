@@ -96,6 +96,8 @@ struct BatchConst {
   const ull *pitems;      // static pull items of the pull CSR
   uint64_t npitems;
   uint32_t has_heavy;     // some in-list spans several pull items
+  uint32_t nhwords;       // bitmap words holding such vertices
+  const uint32_t *hwords;
 };
 
 __device__ __forceinline__ HopArgs slot_args(const BatchConst &c, const SlotDesc &sd,
@@ -129,6 +131,8 @@ __device__ __forceinline__ HopArgs slot_args(const BatchConst &c, const SlotDesc
   a.pitems = c.pitems;
   a.hubcnt = c.hubcnt;
   a.npitems = c.npitems;
+  a.ewords = nullptr;
+  a.newords = 0;
   a.front = sd.front + static_cast<size_t>(hop - 1) * 64;
   a.front_next = sd.front + static_cast<size_t>(hop) * 64;
   return a;
@@ -354,16 +358,19 @@ __global__ __launch_bounds__(kBlock) void k_bw_epi(BatchConst c, uint32_t hop) {
   LocalCtr lc;
   for (ull u = blockIdx.x * kWavesPerBlock + wid; u < total; u += nwaves) {
     const uint32_t s = static_cast<uint32_t>(u / nvb);
-    // push slots, and pull slots whose hubs left partial words in N
+    // push slots (every bitmap word), and pull slots whose chunked vertices
+    // left partial words in N (only the words that hold such vertices)
     const int md = c.slots[s].mode;
     if (md != 1 && !(md == 2 && c.has_heavy)) continue;
+    const uint32_t i = static_cast<uint32_t>(u % nvb);
+    if (md == 2 && i >= c.nhwords) continue;
     if (s != cur) {
       if (cur != UINT32_MAX) flush_ctr(lc, a.ctr, a.front_next, lane_id() % TW);
       cur = s;
       a = slot_args(c, c.slots[s], hop);
       mk = lane_masks(a, lane_id() % TW);
     }
-    epi_block<TW, SHORTEST>(a, static_cast<uint32_t>(u % nvb), mk, lc);
+    epi_block<TW, SHORTEST>(a, md == 2 ? c.hwords[i] : i, mk, lc);
   }
   if (cur != UINT32_MAX) flush_ctr(lc, a.ctr, a.front_next, lane_id() % TW);
 }
@@ -386,7 +393,7 @@ __global__ __launch_bounds__(kBlock) void k_bw_clear(BatchConst c, uint32_t whic
 template <int TW>
 __global__ __launch_bounds__(kBlock) void k_bw_rows(BatchConst c) {
   constexpr int G = 64 / TW;
-  constexpr uint32_t CV = 8192;
+  constexpr uint32_t CV = 2048;
   constexpr int NB = 14;
   const int lane = lane_id(), wid = wave_id();
   const int g = lane / TW, j = lane % TW;
@@ -876,6 +883,8 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   c.hubcnt = pull->hubcnt;
   c.npitems = pull->npitems;
   c.has_heavy = pull->has_heavy ? 1 : 0;
+  c.nhwords = pull->nhwords;
+  c.hwords = pull->hwords;
   const uint32_t grid = static_cast<uint32_t>(g->num_cus) * 8;
 
   hipEvent_t e0, e1;

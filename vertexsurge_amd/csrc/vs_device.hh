@@ -66,6 +66,10 @@ struct DevCSR {
   unsigned long long *pitems = nullptr;
   uint64_t npitems = 0;
   bool has_heavy = false;     // some list longer than kPullChunk
+  // bitmap words (v / 64) holding a vertex with several pull items: the only
+  // words a pull hop's epilogue has to visit
+  uint32_t *hwords = nullptr;
+  uint32_t nhwords = 0;
   // Pull copy of the lists with the hub neighbours first (hubs = the
   // kPullHubs vertices that appear in the most lists, i.e. whose frontier
   // rows are gathered most often; most-listed first), the rest ascending,

@@ -79,6 +79,7 @@ static void free_csr(DevCSR &d) {
     (void)hipFree(d.adj);
   }
   if (d.pitems) (void)hipFree(d.pitems);
+  if (d.hwords) (void)hipFree(d.hwords);
   if (d.hadj) (void)hipFree(d.hadj);
   if (d.hubcnt) (void)hipFree(d.hubcnt);
   d = DevCSR{};
@@ -158,12 +159,28 @@ vs_status ensure_pull_items(DevCSR &c, uint32_t V, uint32_t hubs) {
   }
   std::vector<unsigned long long> it;
   it.reserve(static_cast<size_t>(V) + c.E / kPullChunk + 1);
+  std::vector<uint32_t> hw;
   bool heavy = false;
   for (uint32_t v = 0; v < V; ++v) {
     const uint64_t d = c.hoff[v + 1] - c.hoff[v];
     const uint64_t nch = (d + kPullChunk - 1) / kPullChunk;
-    heavy |= nch > 1;
+    if (nch > 1) {
+      heavy = true;
+      if (hw.empty() || hw.back() != v / 64) hw.push_back(v / 64);
+    }
     for (uint64_t q = 0; q < nch; ++q) it.push_back(v | (q << 32));
+  }
+  if (!hw.empty()) {
+    void *ph = nullptr;
+    VS_HIP(hipMalloc(&ph, hw.size() * 4));
+    hipError_t e = hipMemcpy(ph, hw.data(), hw.size() * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(ph);
+      set_error("heavy words upload: %s", hipGetErrorString(e));
+      return VS_ERR_HIP;
+    }
+    c.hwords = static_cast<uint32_t *>(ph);
+    c.nhwords = static_cast<uint32_t>(hw.size());
   }
   void *p = nullptr;
   VS_HIP(hipMalloc(&p, std::max<size_t>(it.size(), 1) * 8));
@@ -239,8 +256,8 @@ __global__ __launch_bounds__(kBlock) void k_rowcount(const uint64_t *__restrict_
                                                     uint32_t V, uint32_t S,
                                                     uint32_t ntiles, ull *counts) {
   constexpr int G = 64 / TW;
-  constexpr uint32_t CV = 8192;  // vertices per chunk
-  constexpr int NB = 14;         // CV / G + 1 <= 2^14
+  constexpr uint32_t CV = 2048;  // vertices per chunk (enough items to fill the GPU)
+  constexpr int NB = 12;         // CV / G + 1 <= 2^12
   const int lane = lane_id(), wid = wave_id();
   const int g = lane / TW, j = lane % TW;
   const uint32_t nch = (V + CV - 1) / CV;
@@ -391,7 +408,7 @@ vs_status mat_row_counts(vs_graph_s *g, const Mat &m, uint32_t S, uint32_t V,
                          uint32_t tw, uint32_t ntiles, uint64_t *d_counts) {
   VS_HIP(hipMemsetAsync(d_counts, 0, static_cast<size_t>(S) * 8, g->stream));
   if (V == 0) return VS_OK;
-  const uint64_t items = static_cast<uint64_t>(ntiles) * ((V + 8191) / 8192);
+  const uint64_t items = static_cast<uint64_t>(ntiles) * ((V + 2047) / 2048);
   const uint32_t grid = grid_for(items, g->num_cus);
   VS_TW_DISPATCH(tw, k_rowcount, dim3(grid), dim3(kBlock), 0, g->stream, m.d,
                  V, S, ntiles, reinterpret_cast<ull *>(d_counts));

@@ -77,6 +77,8 @@ struct HopArgs {
   const ull *nitems;
   const ull *pitems;     // pull items (static per CSR)
   uint64_t npitems;
+  const uint32_t *ewords; // bitmap words the epilogue visits, or nullptr: all
+  uint32_t newords;
   const uint32_t *hubcnt; // hub neighbours at the head of each pull list, or nullptr
   HopCounters *ctr;      // counters of L_i
   const uint64_t *live;      // per-word rows expanding now, or nullptr (rows < S)
@@ -90,6 +92,7 @@ struct HopArgs {
   uint64_t *front_next;      // the same OR for L_i, accumulated here
 };
 
+// Per-lane partial counters, summed over the wave at the end.
 struct LocalCtr {
   uint64_t pop = 0, nz = 0, act = 0, actdeg = 0, te = 0, visnew = 0, rnew = 0, nzdeg = 0;
   uint64_t rowor = 0;  // OR of the words this lane emitted (its word index j)
@@ -503,11 +506,11 @@ template <int TW, bool SHORTEST>
 __global__ __launch_bounds__(kBlock) void k_push_epi(HopArgs a) {
   const int wid = wave_id();
   const LaneMasks mk = lane_masks(a, lane_id() % TW);
-  const uint32_t nvb = (a.V + 63) / 64;
+  const uint32_t n = a.ewords ? a.newords : (a.V + 63) / 64;
   const uint32_t nwaves = gridDim.x * kWavesPerBlock;
   LocalCtr c;
-  for (uint32_t vb = blockIdx.x * kWavesPerBlock + wid; vb < nvb; vb += nwaves)
-    epi_block<TW, SHORTEST>(a, vb, mk, c);
+  for (uint32_t i = blockIdx.x * kWavesPerBlock + wid; i < n; i += nwaves)
+    epi_block<TW, SHORTEST>(a, a.ewords ? a.ewords[i] : i, mk, c);
   flush_ctr(c, a.ctr, a.front_next, lane_id() % TW);
 }
 

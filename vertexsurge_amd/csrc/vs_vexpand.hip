@@ -523,7 +523,11 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
         if (timing) VS_HIP(hipEventRecord(k0, s));
         launch_pull(TW, shortest, pull_grid, s, a);
         if (timing) VS_HIP(hipEventRecord(k1, s));
-        if (pull->has_heavy) launch_epi(TW, shortest, push_grid, s, a);
+        if (pull->has_heavy) {  // only the words that hold chunked vertices
+          a.ewords = pull->hwords;
+          a.newords = pull->nhwords;
+          launch_epi(TW, shortest, push_grid, s, a);
+        }
       }
       VS_HIP(hipGetLastError());
       if (timing && st) {
