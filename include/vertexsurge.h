@@ -279,8 +279,11 @@ typedef struct {
   uint32_t reserved;
   uint64_t push_decisions;
   uint64_t pull_decisions;
-  /* with VS_VX_TIMING in flags (runs without graph capture): device time and
-   * compulsory bytes of the persistent expand launches */
+  /* with VS_VX_TIMING in flags (runs without graph capture): device time of
+   * the persistent expand launches over dense rows (ms_expand, n_expand) and
+   * over sparse rows read through word masks (ms_expand_sparse,
+   * n_expand_sparse; one of each per hop), and the compulsory bytes of all
+   * hops (bytes_alg) and of the sparse ones (bytes_sparse) */
   double ms_expand;
   uint64_t n_expand;
   uint64_t bytes_alg;
@@ -288,6 +291,9 @@ typedef struct {
    * graph for every wave, synchronize), same averaging as ms_total; 0 when
    * the launch sequence is not captured */
   double ms_wall;
+  double ms_expand_sparse;
+  uint64_t n_expand_sparse;
+  uint64_t bytes_sparse;
 } vs_batch_stats;
 
 /* Runs nq independent VExpand queries packed into shared bit-matrix tiles.

@@ -578,4 +578,5 @@ def vexpand_batch(graph: Graph, queries: list, tile_words: int = 0, sched: str =
         "num_tiles": st.num_tiles, "num_waves": st.num_waves, "launches": st.launches,
         "push_decisions": st.push_decisions, "pull_decisions": st.pull_decisions,
         "ms_expand": st.ms_expand, "n_expand": st.n_expand, "bytes_alg": st.bytes_alg,
-        "ms_wall": st.ms_wall}
+        "ms_wall": st.ms_wall, "ms_expand_sparse": st.ms_expand_sparse,
+        "n_expand_sparse": st.n_expand_sparse, "bytes_sparse": st.bytes_sparse}

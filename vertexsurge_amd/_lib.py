@@ -79,7 +79,9 @@ class BatchStats(C.Structure):
                 ("num_tiles", C.c_uint32), ("num_waves", C.c_uint32), ("launches", C.c_uint32),
                 ("reserved", C.c_uint32), ("push_decisions", C.c_uint64),
                 ("pull_decisions", C.c_uint64), ("ms_expand", C.c_double),
-                ("n_expand", C.c_uint64), ("bytes_alg", C.c_uint64), ("ms_wall", C.c_double)]
+                ("n_expand", C.c_uint64), ("bytes_alg", C.c_uint64), ("ms_wall", C.c_double),
+                ("ms_expand_sparse", C.c_double), ("n_expand_sparse", C.c_uint64),
+                ("bytes_sparse", C.c_uint64)]
 
 
 class MintersectStats(C.Structure):

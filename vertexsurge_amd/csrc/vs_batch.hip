@@ -59,11 +59,13 @@ struct TileInfo {
 struct BatchCtl {
   uint32_t wave;
   uint32_t pad;
-  ull unit_prefix[kMaxSlots + 1];
+  ull unit_prefix[kMaxSlots + 1];    // units of push slots and dense pull slots
+  ull unit_prefix_s[kMaxSlots + 1];  // units of sparse pull slots (word masks)
   ull te;
   ull push_dec;
   ull pull_dec;
   ull bytes;
+  ull bytes_sparse;  // of the hops run by the sparse (word-mask) expansion
 };
 
 struct BatchConst {
@@ -266,6 +268,7 @@ __global__ __launch_bounds__(64) void k_bw_decide(BatchConst c, uint32_t hop) {
     sd.dense = (mode == 2 && sd.ctr[hop - 1].actdeg * 4 > 3 * c.E) ? 1u : 0u;
     // rows sparse in words: gather only the words their masks name
     sd.masked = (sd.ctr[hop - 1].nzdeg * 2 < sd.ctr[hop - 1].actdeg * c.TW) ? 1u : 0u;
+    sd.ctr[hop].how = (mode == 2 && sd.masked) ? 3u : static_cast<ull>(mode);
     if (mode == 1) atomicAdd(&c.ctl->push_dec, 1ull);
     if (mode == 2) atomicAdd(&c.ctl->pull_dec, 1ull);
   }
@@ -290,24 +293,32 @@ __global__ __launch_bounds__(kBlock) void k_bw_items(BatchConst c, uint32_t hop)
 __global__ void k_bw_plan(BatchConst c, uint32_t hop) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const uint32_t nvb = (c.V + 63) / 64;
-  ull acc = 0;
+  ull acc = 0, acc_s = 0;
   for (uint32_t s = 0; s < c.nslots; ++s) {
     c.ctl->unit_prefix[s] = acc;
+    c.ctl->unit_prefix_s[s] = acc_s;
     const SlotDesc &sd = c.slots[s];
-    if (sd.mode == 2) acc += c.npitems;
+    if (sd.mode == 2 && sd.masked) acc_s += c.npitems;
+    else if (sd.mode == 2) acc += c.npitems;
     else if (sd.mode == 1) acc += sd.ctr[hop - 1].items;
   }
   c.ctl->unit_prefix[c.nslots] = acc;
+  c.ctl->unit_prefix_s[c.nslots] = acc_s;
 }
 
 // ---- the persistent expansion: pull vertex-blocks and push items of all
-// slots; work unit u belongs to the slot whose prefix range holds it
-template <int TW, bool SHORTEST>
+// slots; work unit u belongs to the slot whose prefix range holds it.  Two
+// launches per hop: SPARSE takes the pull slots whose frontier rows are read
+// through their word masks, the other one the push and dense pull slots.
+template <int TW, bool SHORTEST, bool SPARSE>
 __global__ __launch_bounds__(kBlock) void k_bw_expand(BatchConst c, uint32_t hop) {
   __shared__ uint32_t s_nbr[kWavesPerBlock][64];
-  __shared__ uint64_t s_nm[kWavesPerBlock][64];
+  __shared__ uint64_t s_nm[kWavesPerBlock][SPARSE ? 64 : 1];
+  __shared__ uint32_t s_pairs[kWavesPerBlock][SPARSE ? kPairs : 1];
+  __shared__ uint64_t s_acc[kWavesPerBlock][SPARSE ? 64 : 1];
+  const ull *prefix = SPARSE ? c.ctl->unit_prefix_s : c.ctl->unit_prefix;
   const int wid = wave_id();
-  const ull total = c.ctl->unit_prefix[c.nslots];
+  const ull total = prefix[c.nslots];
   const ull nwaves = static_cast<ull>(gridDim.x) * kWavesPerBlock;
   uint32_t cur = UINT32_MAX;
   HopArgs a;
@@ -319,8 +330,8 @@ __global__ __launch_bounds__(kBlock) void k_bw_expand(BatchConst c, uint32_t hop
   ull pre_u = ~0ull;  // unit whose head is prefetched in `pre`
   for (ull u = blockIdx.x * kWavesPerBlock + wid; u < total; u += nwaves) {
     uint32_t s = cur == UINT32_MAX ? 0 : cur;
-    while (s + 1 <= c.nslots && c.ctl->unit_prefix[s + 1] <= u) ++s;
-    while (c.ctl->unit_prefix[s] > u) --s;
+    while (s + 1 <= c.nslots && prefix[s + 1] <= u) ++s;
+    while (prefix[s] > u) --s;
     if (s != cur) {
       if (cur != UINT32_MAX && mode == 2) flush_ctr(lc, a.ctr, a.front_next, j);
       cur = s;
@@ -328,17 +339,18 @@ __global__ __launch_bounds__(kBlock) void k_bw_expand(BatchConst c, uint32_t hop
       mode = c.slots[s].mode;
       mk = lane_masks(a, j);
     }
-    const ull slot_end = c.ctl->unit_prefix[s + 1];
-    const ull local = u - c.ctl->unit_prefix[s];
+    const ull slot_end = prefix[s + 1];
+    const ull local = u - prefix[s];
     if (mode == 2) {
       const PullHead h = pre_u == u ? pre : pull_head<TW, SHORTEST>(a, a.pitems[local], j);
       const ull nu = u + nwaves;  // prefetch the next unit of the same slot
       if (nu < slot_end) {
-        pre = pull_head<TW, SHORTEST>(a, a.pitems[nu - c.ctl->unit_prefix[s]], j);
+        pre = pull_head<TW, SHORTEST>(a, a.pitems[nu - prefix[s]], j);
         pre_u = nu;
       }
-      pull_body<TW, SHORTEST>(a, h, s_nbr[wid], s_nm[wid], mk, lc);
-    } else {
+      pull_body<TW, SHORTEST, SPARSE>(a, h, s_nbr[wid], s_nm[wid], s_pairs[wid], s_acc[wid], mk,
+                                      lc);
+    } else if (!SPARSE) {
       push_item<TW>(a, a.items[local]);
     }
   }
@@ -483,7 +495,7 @@ __global__ void k_bw_finish(BatchConst c) {
     // 8 nz(F_i) + 8 nz(L_i) + [Shortest] 8 (nz(Vis_{i-1}) + nz(Vis_i))
     // + [Any] 8 (nz(R_{i-1}) + nz(R_i)) + 4 actdeg(F_i) + 16 act(F_i),
     // plus 8 nz of the final result
-    ull te = 0, bytes = 0;
+    ull te = 0, bytes = 0, bytes_sparse = 0;
     for (uint32_t s = 0; s < c.nslots; ++s) {
       const SlotDesc &sd = c.slots[s];
       if (sd.tile == UINT32_MAX) continue;
@@ -496,14 +508,17 @@ __global__ void k_bw_finish(BatchConst c) {
         vis_nz += cc.visnew;
         r_nz += cc.rnew;
         if (pc.pop == 0 || h > hmax) continue;
-        bytes += 8 * pc.nz + 8 * cc.nz + 4 * pc.actdeg + 16 * pc.act;
-        if (sd.Vis) bytes += 8 * (vis_prev + vis_nz);
-        else bytes += 8 * (r_prev + r_nz);
+        ull b = 8 * pc.nz + 8 * cc.nz + 4 * pc.actdeg + 16 * pc.act;
+        if (sd.Vis) b += 8 * (vis_prev + vis_nz);
+        else b += 8 * (r_prev + r_nz);
+        bytes += b;
+        if (cc.how == 3) bytes_sparse += b;
       }
       bytes += 8 * (c.use_r ? r_nz : vis_nz);
     }
     c.ctl->te += te;
     c.ctl->bytes += bytes;
+    c.ctl->bytes_sparse += bytes_sparse;
   }
 }
 
@@ -522,32 +537,34 @@ __global__ void k_bw_advance(BatchCtl *ctl) {
     default: FN<64>(__VA_ARGS__); break;  \
   }
 
-// Optional HIP events around each persistent expand launch (timing runs,
-// never inside a captured graph).
+// Optional HIP events around each persistent expand launch, per variant
+// (timing runs, never inside a captured graph).
 struct ExpandEvents {
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[2];  // dense, sparse
   ~ExpandEvents() {
-    for (auto &e : ev) {
-      (void)hipEventDestroy(e.first);
-      (void)hipEventDestroy(e.second);
-    }
+    for (auto &v : ev)
+      for (auto &e : v) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+      }
   }
-  void record(hipStream_t s, bool start) {
+  void record(hipStream_t s, bool start, bool sparse) {
+    auto &v = ev[sparse ? 1 : 0];
     if (start) {
       hipEvent_t a = nullptr, b = nullptr;
       if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
         (void)hipGetLastError();
         return;
       }
-      ev.emplace_back(a, b);
+      v.emplace_back(a, b);
       (void)hipEventRecord(a, s);
-    } else if (!ev.empty()) {
-      (void)hipEventRecord(ev.back().second, s);
+    } else if (!v.empty()) {
+      (void)hipEventRecord(v.back().second, s);
     }
   }
-  double total_ms() const {
+  double total_ms(bool sparse) const {
     double t = 0;
-    for (auto &e : ev) {
+    for (auto &e : ev[sparse ? 1 : 0]) {
       float ms = 0.f;
       if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) t += ms;
       else (void)hipGetLastError();
@@ -559,10 +576,18 @@ struct ExpandEvents {
 template <int TW>
 static void enqueue_hop_tw(const BatchConst &c, bool shortest, uint32_t hop,
                            uint32_t grid, hipStream_t s, ExpandEvents *ev) {
-  if (ev) ev->record(s, true);
-  if (shortest) hipLaunchKernelGGL((k_bw_expand<TW, true>), dim3(grid), dim3(kBlock), 0, s, c, hop);
-  else hipLaunchKernelGGL((k_bw_expand<TW, false>), dim3(grid), dim3(kBlock), 0, s, c, hop);
-  if (ev) ev->record(s, false);
+  if (ev) ev->record(s, true, false);
+  if (shortest)
+    hipLaunchKernelGGL((k_bw_expand<TW, true, false>), dim3(grid), dim3(kBlock), 0, s, c, hop);
+  else
+    hipLaunchKernelGGL((k_bw_expand<TW, false, false>), dim3(grid), dim3(kBlock), 0, s, c, hop);
+  if (ev) ev->record(s, false, false);
+  if (ev) ev->record(s, true, true);
+  if (shortest)
+    hipLaunchKernelGGL((k_bw_expand<TW, true, true>), dim3(grid), dim3(kBlock), 0, s, c, hop);
+  else
+    hipLaunchKernelGGL((k_bw_expand<TW, false, true>), dim3(grid), dim3(kBlock), 0, s, c, hop);
+  if (ev) ev->record(s, false, true);
   if (shortest) hipLaunchKernelGGL((k_bw_epi<TW, true>), dim3(grid), dim3(kBlock), 0, s, c, hop);
   else hipLaunchKernelGGL((k_bw_epi<TW, false>), dim3(grid), dim3(kBlock), 0, s, c, hop);
   hipLaunchKernelGGL((k_bw_clear<TW>), dim3(grid), dim3(kBlock), 0, s, c, hop - 1);
@@ -592,7 +617,7 @@ static uint32_t enqueue_wave(const BatchConst &c, bool shortest, uint32_t grid,
     hipLaunchKernelGGL(k_bw_items, dim3(std::max<uint32_t>(1, item_grid)), dim3(kBlock), 0, s, c, hop);
     hipLaunchKernelGGL(k_bw_plan, dim3(1), dim3(64), 0, s, c, hop);
     VS_TW_BATCH(c.TW, enqueue_hop_tw, c, shortest, hop, grid, s, ev);
-    n += 6;
+    n += 7;
   }
   VS_TW_BATCH(c.TW, enqueue_final_tw, c, grid, s);
   hipLaunchKernelGGL(k_bw_finish, dim3(std::max<uint32_t>(1, (c.nslots * c.TW * 64 + 255) / 256)),
@@ -963,8 +988,10 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
       VS_HIP(hipStreamSynchronize(s));
     }
     if (timing) {
-      st.ms_expand += xev.total_ms();
-      st.n_expand += xev.ev.size();
+      st.ms_expand += xev.total_ms(false);
+      st.n_expand += xev.ev[0].size();
+      st.ms_expand_sparse += xev.total_ms(true);
+      st.n_expand_sparse += xev.ev[1].size();
     }
   }
   float fms = 0.f;
@@ -991,6 +1018,7 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   st.push_decisions += ctl.push_dec;  // of the last replay (counters reset per replay)
   st.pull_decisions += ctl.pull_dec;
   st.bytes_alg += ctl.bytes;
+  st.bytes_sparse += ctl.bytes_sparse;
   return VS_OK;
 }
 

@@ -51,6 +51,8 @@ struct HopCounters {
   ull items;   // push work items built from L_i (written by the item builder)
   ull nzdeg;   // sum over active vertices of push-degree * nonzero words: the
                // words a full next-hop pull would gather, by frequency
+  ull how;     // batched engine: how L_i was made (1 push, 2 pull, 3 pull
+               // through word masks)
 };
 
 struct HopArgs {
@@ -201,48 +203,97 @@ __device__ __forceinline__ uint64_t gather(const HopArgs &a, uint32_t u, int j) 
 // so stopping on the partial word is exact for TW = 64 and conservative
 // below).  Lanes stop independently; the hubs at the head of the list are
 // gathered first, so most lanes that can stop do so before the streaming part.
-// Word j of neighbour slot k; with MASKED, only when the neighbour's word
-// mask says the word holds bits (read as the 32-bit half that has bit j).
-template <int TW, bool CACHED, bool MASKED>
-__device__ __forceinline__ uint64_t gather_m(const HopArgs &a, const uint32_t *nbr,
-                                             const uint64_t *nm, uint32_t k, int j) {
-  if constexpr (MASKED) {
-    const uint32_t half = reinterpret_cast<const uint32_t *>(nm)[2 * k + (j >> 5)];
-    return ((half >> (j & 31)) & 1) ? gather<TW, CACHED>(a, nbr[k], j) : 0;
-  } else {
-    return gather<TW, CACHED>(a, nbr[k], j);
-  }
-}
-
-template <int TW, bool CACHED, bool MASKED>
-__device__ __forceinline__ void gather_range(const HopArgs &a, const uint32_t *nbr,
-                                             const uint64_t *nm, uint32_t &k, uint32_t end,
-                                             int j, uint64_t need, uint64_t &acc) {
+template <int TW, bool CACHED>
+__device__ __forceinline__ void gather_range(const HopArgs &a, const uint32_t *nbr, uint32_t &k,
+                                             uint32_t end, int j, uint64_t need,
+                                             uint64_t &acc) {
   constexpr int G = 64 / TW;
   for (; k + 7 * G < end; k += 8 * G) {
     if ((acc & need) == need) return;
     uint64_t x[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) x[q] = gather_m<TW, CACHED, MASKED>(a, nbr, nm, k + q * G, j);
+    for (int q = 0; q < 8; ++q) x[q] = gather<TW, CACHED>(a, nbr[k + q * G], j);
     acc |= ((x[0] | x[1]) | (x[2] | x[3])) | ((x[4] | x[5]) | (x[6] | x[7]));
   }
   if (k + 3 * G < end && (acc & need) != need) {
-    const uint64_t x0 = gather_m<TW, CACHED, MASKED>(a, nbr, nm, k, j);
-    const uint64_t x1 = gather_m<TW, CACHED, MASKED>(a, nbr, nm, k + G, j);
-    const uint64_t x2 = gather_m<TW, CACHED, MASKED>(a, nbr, nm, k + 2 * G, j);
-    const uint64_t x3 = gather_m<TW, CACHED, MASKED>(a, nbr, nm, k + 3 * G, j);
+    const uint64_t x0 = gather<TW, CACHED>(a, nbr[k], j);
+    const uint64_t x1 = gather<TW, CACHED>(a, nbr[k + G], j);
+    const uint64_t x2 = gather<TW, CACHED>(a, nbr[k + 2 * G], j);
+    const uint64_t x3 = gather<TW, CACHED>(a, nbr[k + 3 * G], j);
     acc |= (x0 | x1) | (x2 | x3);
     k += 4 * G;
   }
   for (; k < end; k += G) {
     if ((acc & need) == need) return;
-    acc |= gather_m<TW, CACHED, MASKED>(a, nbr, nm, k, j);
+    acc |= gather<TW, CACHED>(a, nbr[k], j);
   }
 }
 
-template <int TW, bool SHORTEST>
+// Sparse rows (word masks on): the (neighbour, word) pairs whose word holds
+// bits and is still needed are laid out flat over the wave's lanes, so every
+// load instruction carries up to 64 useful words instead of one per lane
+// per neighbour; the words are ORed into the wave's accumulator accw[TW] in
+// LDS.  Pairs go through LDS in rounds of kPairs (8 loads in flight per lane).
+constexpr uint32_t kPairs = 512;
+
+template <int TW>
+__device__ __forceinline__ void gather_pairs(const HopArgs &a, const uint32_t *nbr,
+                                             const uint64_t *nm, uint32_t n, uint32_t nh,
+                                             uint64_t needw, uint32_t *pairs, uint64_t *accw) {
+  const int lane = lane_id();
+  const uint64_t mine = lane < static_cast<int>(n) ? (nm[lane] & needw) : 0;
+  const uint32_t cnt = __popcll(mine);
+  uint32_t inc = cnt;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(inc, d);
+    if (lane >= d) inc += y;
+  }
+  const uint32_t off = inc - cnt;
+  const uint32_t P = __shfl(inc, 63);
+  for (uint32_t base = 0; base < P; base += kPairs) {
+    // this lane's pairs with index in [base, base + kPairs)
+    uint64_t m = mine;
+    uint32_t p = off;
+    while (m && p < base) {
+      m &= m - 1;
+      ++p;
+    }
+    while (m && p < base + kPairs) {
+      const uint32_t jj = __ffsll(static_cast<long long>(m)) - 1;
+      m &= m - 1;
+      pairs[p - base] = (static_cast<uint32_t>(lane) << 8) | jj;
+      ++p;
+    }
+    wave_sync();
+    const uint32_t np = min(kPairs, P - base);
+    uint64_t x[kPairs / 64];
+#pragma unroll
+    for (int r = 0; r < static_cast<int>(kPairs / 64); ++r) {
+      const uint32_t q = r * 64 + lane;
+      x[r] = 0;
+      if (q < np) {
+        const uint32_t pr = pairs[q];
+        const uint32_t k = pr >> 8;
+        const uint64_t *src = a.F + static_cast<size_t>(nbr[k]) * TW + (pr & 255);
+        x[r] = k < nh ? *src : __builtin_nontemporal_load(src);
+      }
+    }
+    // the word index is read back from LDS rather than held in registers
+#pragma unroll
+    for (int r = 0; r < static_cast<int>(kPairs / 64); ++r)
+      if (x[r])
+        atomicOr(reinterpret_cast<ull *>(&accw[pairs[r * 64 + lane] & 255]), static_cast<ull>(x[r]));
+    wave_sync();
+  }
+}
+
+// SPARSE (the frontier's word masks are read, a.Fm set): pair gathers;
+// otherwise every lane gathers its word of every active neighbour.
+template <int TW, bool SHORTEST, bool SPARSE>
 __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, uint32_t *nbr,
-                                          uint64_t *nm, const LaneMasks &mk, LocalCtr &c) {
+                                          uint64_t *nm, uint32_t *pairs, uint64_t *accw,
+                                          const LaneMasks &mk, LocalCtr &c) {
   constexpr int G = 64 / TW;
   const int lane = lane_id();
   const int g = lane / TW, j = lane % TW;
@@ -259,6 +310,7 @@ __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, u
   const uint64_t hub_end = h.hubs == 0xffffffffu ? le : lb + h.hubs;
   uint64_t acc = 0;
   bool todo = need != 0;
+  if (SPARSE && lane < TW) accw[lane] = 0;
   for (uint64_t base = b; base < e; base += 64) {
     const uint64_t idx = base + lane;
     uint32_t u = 0;
@@ -276,18 +328,18 @@ __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, u
     if (act) {
       const uint32_t r = lane_rank(m);
       nbr[r] = u;
-      if (a.Fm) nm[r] = a.Fm[u];
+      if (SPARSE) nm[r] = a.Fm[u];
     }
     wave_sync();
-    if (todo) {
+    if (SPARSE) {
+      // words (group 0's lanes) that still miss rows
+      const uint64_t needw = __ballot(g == 0 && todo);
+      if (needw) gather_pairs<TW>(a, nbr, nm, n, nh, needw, pairs, accw);
+      acc = accw[j];
+    } else if (todo) {
       uint32_t k = g;
-      if (a.Fm) {
-        gather_range<TW, true, true>(a, nbr, nm, k, nh, j, need, acc);
-        gather_range<TW, false, true>(a, nbr, nm, k, n, j, need, acc);
-      } else {
-        gather_range<TW, true, false>(a, nbr, nm, k, nh, j, need, acc);
-        gather_range<TW, false, false>(a, nbr, nm, k, n, j, need, acc);
-      }
+      gather_range<TW, true>(a, nbr, k, nh, j, need, acc);
+      gather_range<TW, false>(a, nbr, k, n, j, need, acc);
     }
     wave_sync();
     acc = group_or<TW>(acc);
@@ -469,10 +521,12 @@ __device__ __forceinline__ void items_slice(const uint64_t *actbm, uint32_t nvb,
 }
 
 // --------------------------------------------------------- plain kernels
-template <int TW, bool SHORTEST>
+template <int TW, bool SHORTEST, bool SPARSE>
 __global__ __launch_bounds__(kBlock) void k_pull(HopArgs a) {
   __shared__ uint32_t s_nbr[kWavesPerBlock][64];
   __shared__ uint64_t s_nm[kWavesPerBlock][64];
+  __shared__ uint32_t s_pairs[kWavesPerBlock][kPairs];
+  __shared__ uint64_t s_acc[kWavesPerBlock][64];
   const int wid = wave_id();
   const LaneMasks mk = lane_masks(a, lane_id() % TW);
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
@@ -486,7 +540,8 @@ __global__ __launch_bounds__(kBlock) void k_pull(HopArgs a) {
     const uint64_t nx = it + nwaves;
     PullHead hn{};
     if (nx < n) hn = pull_head<TW, SHORTEST>(a, a.pitems[nx], j);
-    pull_body<TW, SHORTEST>(a, h, s_nbr[wid], s_nm[wid], mk, c);
+    pull_body<TW, SHORTEST, SPARSE>(a, h, s_nbr[wid], s_nm[wid], s_pairs[wid], s_acc[wid], mk,
+                                    c);
     h = hn;
     it = nx;
   }

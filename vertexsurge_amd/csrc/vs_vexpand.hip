@@ -86,8 +86,15 @@ __global__ void k_fs_check(const uint64_t *L, const uint32_t *tgt, uint32_t S,
 template <int TW>
 static void launch_pull_tw(bool shortest, uint32_t grid, hipStream_t s,
                            const HopArgs &a) {
-  if (shortest) hipLaunchKernelGGL((k_pull<TW, true>), dim3(grid), dim3(kBlock), 0, s, a);
-  else hipLaunchKernelGGL((k_pull<TW, false>), dim3(grid), dim3(kBlock), 0, s, a);
+  const bool sparse = a.Fm != nullptr;
+  if (shortest && sparse)
+    hipLaunchKernelGGL((k_pull<TW, true, true>), dim3(grid), dim3(kBlock), 0, s, a);
+  else if (shortest)
+    hipLaunchKernelGGL((k_pull<TW, true, false>), dim3(grid), dim3(kBlock), 0, s, a);
+  else if (sparse)
+    hipLaunchKernelGGL((k_pull<TW, false, true>), dim3(grid), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_pull<TW, false, false>), dim3(grid), dim3(kBlock), 0, s, a);
 }
 
 template <int TW>
