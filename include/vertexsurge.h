@@ -156,7 +156,9 @@ vs_status vs_graph_trim(vs_graph_t g);
 typedef struct {
   uint32_t tile_words;  /* storage tile width in 64-bit words: 1..64, pow2 */
   uint32_t push_alpha;  /* push while frontier-edges * alpha < E */
-  uint32_t pull_beta;   /* unused when 0; pull->push when popcount*beta < rows*V */
+  uint32_t sparse_pct;  /* pull reads the frontier's word masks (sparse-row
+                           gathers) when its nonzero words, weighted by degree,
+                           are below this percentage of all; 0 = 50 */
   uint32_t pull_hubs;   /* vertices whose frontier rows pull gathers first, with
                            cached loads (the most-listed ones); 0 = 131072 */
 } vs_tuning;
@@ -353,7 +355,10 @@ vs_status vs_sched_train(vs_sched_t s, const float *X, const float *Y,
                          const float *M, uint32_t n, uint32_t epochs, float lr,
                          float *loss_curve);
 vs_status vs_sched_predict(vs_sched_t s, const float *X, uint32_t n, float *Y);
-/* Attach (or detach with NULL) a scheduler used by VS_SCHED_LEARNED. */
+/* Attach (or detach with NULL) a scheduler used by VS_SCHED_LEARNED.  The
+ * graph holds its own reference: vs_sched_free may be called while the
+ * scheduler is attached, and the scheduler lives until the last graph using
+ * it detaches it or is destroyed. */
 vs_status vs_graph_set_scheduler(vs_graph_t g, vs_sched_t s);
 
 /* ------------------------------------------------ bit-matrix tables (ColMain) */

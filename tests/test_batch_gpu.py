@@ -111,3 +111,23 @@ def test_batch_row_packing_order(orc, g14, keep_order):
     c, d, st = vs.vexpand_batch(G, qs, tile_words=4, keep_order=keep_order)
     ec, ed = expected(orc, g, qs)
     assert np.array_equal(c, ec) and np.array_equal(d, ed)
+
+
+def test_scheduler_handle_freed_while_attached(orc, g14):
+    """The graph keeps its own reference to an attached scheduler: freeing the
+    caller's handle must leave the learned path working (it used to leave the
+    graph with a dangling pointer)."""
+    g, G = g14
+    s = vs.Scheduler(seed=7)
+    G.set_scheduler(s)
+    s.close()
+    try:
+        qs = make_queries(g.V, 6, seed=17)
+        c, d, st = vs.vexpand_batch(G, qs, sched="learned")
+        ec, ed = expected(orc, g, qs)
+        assert np.array_equal(c, ec) and np.array_equal(d, ed)
+        r, _ = vs.vexpand(G, qs[0].sources, qs[0].k_min, qs[0].k_max, sched="learned")
+        assert np.array_equal(r.row_counts(0), ec[:len(qs[0].sources)])
+        r.close()
+    finally:
+        G.set_scheduler(None)

@@ -134,17 +134,18 @@ class Graph:
     def handle(self):
         return self._h
 
-    def set_tuning(self, tile_words=0, push_alpha=16, pull_hubs=0):
-        t = _lib.Tuning(tile_words, push_alpha, 0, pull_hubs)
+    def set_tuning(self, tile_words=0, push_alpha=16, pull_hubs=0, sparse_pct=0):
+        t = _lib.Tuning(tile_words, push_alpha, sparse_pct, pull_hubs)
         _lib.call("vs_graph_set_tuning", self._h, C.byref(t))
 
     def get_tuning(self) -> dict:
         t = _lib.Tuning()
         _lib.call("vs_graph_get_tuning", self._h, C.byref(t))
         return {"tile_words": t.tile_words, "push_alpha": t.push_alpha,
-                "pull_hubs": t.pull_hubs}
+                "pull_hubs": t.pull_hubs, "sparse_pct": t.sparse_pct}
 
     def set_scheduler(self, sched: "Scheduler | None"):
+        # the library keeps its own reference to an attached scheduler
         _lib.call("vs_graph_set_scheduler", self._h, sched.handle if sched else None)
 
     def trim(self):

@@ -60,7 +60,7 @@ class VexpandStats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("tile_words", C.c_uint32), ("push_alpha", C.c_uint32),
-                ("pull_beta", C.c_uint32), ("pull_hubs", C.c_uint32)]
+                ("sparse_pct", C.c_uint32), ("pull_hubs", C.c_uint32)]
 
 
 class Query(C.Structure):

@@ -78,6 +78,7 @@ struct BatchConst {
   uint32_t S;          // rows of the group
   int32_t sched;
   uint32_t alpha;
+  uint32_t sparse_pct;  // word-mask threshold (percent of words, by degree)
   uint32_t use_r;
   uint64_t E;
   const uint64_t *poff;
@@ -267,7 +268,8 @@ __global__ __launch_bounds__(64) void k_bw_decide(BatchConst c, uint32_t hop) {
     // most in-edges come from active vertices: pull skips the bitmap lookup
     sd.dense = (mode == 2 && sd.ctr[hop - 1].actdeg * 4 > 3 * c.E) ? 1u : 0u;
     // rows sparse in words: gather only the words their masks name
-    sd.masked = (sd.ctr[hop - 1].nzdeg * 2 < sd.ctr[hop - 1].actdeg * c.TW) ? 1u : 0u;
+    sd.masked = (sd.ctr[hop - 1].nzdeg * 100 < sd.ctr[hop - 1].actdeg * c.TW * c.sparse_pct)
+                    ? 1u : 0u;
     sd.ctr[hop].how = (mode == 2 && sd.masked) ? 3u : static_cast<ull>(mode);
     if (mode == 1) atomicAdd(&c.ctl->push_dec, 1ull);
     if (mode == 2) atomicAdd(&c.ctl->pull_dec, 1ull);
@@ -886,6 +888,7 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   c.S = S;
   c.sched = p.sched;
   c.alpha = g->tuning.push_alpha ? g->tuning.push_alpha : 16;
+  c.sparse_pct = g->tuning.sparse_pct ? g->tuning.sparse_pct : kSparsePct;
   c.use_r = use_r ? 1 : 0;
   c.E = push->E;
   c.poff = pull->off;
@@ -1098,7 +1101,7 @@ vs_status vs_autotune_vexpand(vs_graph_t g, const uint32_t *sources, uint32_t S,
   double bms = 1e300;
   for (uint32_t tw = std::max<uint32_t>(1, tw_max / 4); tw <= tw_max; tw *= 2) {
     for (uint32_t alpha : {4u, 16u, 64u}) {
-      vs_tuning t{tw, alpha, saved.pull_beta, saved.pull_hubs};
+      vs_tuning t{tw, alpha, saved.sparse_pct, saved.pull_hubs};
       g->tuning = t;
       vs_vexpand_params pp{};
       pp.k_min = 1;

@@ -83,6 +83,9 @@ struct DevCSR {
 // default hub count: 131072 rows of 512 B (tile width 64) = 64 MB, a hot set
 // the 256 MB Infinity Cache can hold next to the streaming gathers
 constexpr uint32_t kPullHubs = 131072;
+// default word-mask threshold: pull gathers sparse rows through their word
+// masks when the degree-weighted share of nonzero words is below this (%)
+constexpr uint32_t kSparsePct = 50;
 
 // Builds the pull items of a CSR once (vertex | chunk << 32 per kPullChunk
 // in-edges, vertices without edges omitted).
@@ -93,6 +96,10 @@ vs_status ensure_pull_items(DevCSR &c, uint32_t V, uint32_t hubs);
 } // namespace vs
 
 struct vs_sched_s;
+namespace vs {
+// drops one reference to a scheduler, freeing it with the last (vs_mlp.hip)
+void sched_release(vs_sched_s *s);
+}
 
 struct vs_graph_s {
   int device = 0;

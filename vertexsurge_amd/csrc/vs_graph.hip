@@ -239,6 +239,8 @@ void graph_release(vs_graph_s *g) {
   (void)hipSetDevice(g->device);
   (void)hipStreamSynchronize(g->stream);
   g->pool.trim();
+  if (g->sched) sched_release(g->sched);
+  g->sched = nullptr;
   free_csr(g->fwd);
   free_csr(g->rev);
   free_csr(g->both);

@@ -13,6 +13,7 @@
 #include "vs_sched.hh"
 
 struct vs_sched_s {
+  int refs = 1;  // the caller's handle + one per graph it is attached to
   int device = 0;
   float *d_params = nullptr;
   float *d_m = nullptr;
@@ -262,8 +263,14 @@ vs_status vs_sched_create(uint64_t seed, vs_sched_t *out) {
   return VS_OK;
 }
 
-vs_status vs_sched_free(vs_sched_t s) {
-  if (!s) return VS_OK;
+} // extern "C"
+
+namespace vs {
+
+void sched_release(vs_sched_s *s) {
+  if (!s || --s->refs > 0) return;
+  int cur = 0;
+  (void)hipGetDevice(&cur);
   (void)hipSetDevice(s->device);
   (void)hipStreamSynchronize(s->stream);
   (void)hipFree(s->d_params);
@@ -274,6 +281,16 @@ vs_status vs_sched_free(vs_sched_t s) {
   (void)hipFree(s->d_loss);
   (void)hipStreamDestroy(s->stream);
   delete s;
+  (void)hipSetDevice(cur);
+}
+
+} // namespace vs
+
+extern "C" {
+
+vs_status vs_sched_free(vs_sched_t s) {
+  clear_error();
+  sched_release(s);  // attached graphs keep their own reference
   return VS_OK;
 }
 
@@ -412,6 +429,8 @@ vs_status vs_sched_predict(vs_sched_t s, const float *X, uint32_t n, float *Y) {
 vs_status vs_graph_set_scheduler(vs_graph_t g, vs_sched_t s) {
   clear_error();
   if (!g) { set_error("vs_graph_set_scheduler: null graph"); return VS_ERR_INVALID; }
+  if (s) s->refs++;  // the graph's own reference: the caller may free its handle
+  if (g->sched) sched_release(g->sched);
   g->sched = s;
   return VS_OK;
 }

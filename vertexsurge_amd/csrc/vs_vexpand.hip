@@ -448,6 +448,7 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
     hipEvent_t eh0 = timing ? evs.make() : nullptr, eh1 = timing ? evs.make() : nullptr;
     if (timing) VS_HIP(hipEventRecord(eh0, s));
     const bool acc_r = R && hop >= k_min;
+    const uint64_t sparse_pct = g->tuning.sparse_pct ? g->tuning.sparse_pct : kSparsePct;
     for (uint32_t t = 0; t < T; ++t) {
       const HopCounters &pc = C(hop - 1, t);
       tile_mode[t] = 0;
@@ -487,7 +488,8 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
       a.F = F + t * tile_words;
       // read the word masks only when the rows a pull gathers (weighted by
       // how often: degree) are mostly empty words
-      a.Fm = pc.nzdeg * 2 < pc.actdeg * TW ? FM[cur] + static_cast<size_t>(t) * V : nullptr;
+      a.Fm = pc.nzdeg * 100 < pc.actdeg * TW * sparse_pct ? FM[cur] + static_cast<size_t>(t) * V
+                                                         : nullptr;
       a.actbm = bmF + static_cast<size_t>(t) * nvb;
       a.Fn = Fn + t * tile_words;
       a.Fnm = FM[cur ^ 1] + static_cast<size_t>(t) * V;
