@@ -70,6 +70,9 @@ struct DevCSR {
   // words a pull hop's epilogue has to visit
   uint32_t *hwords = nullptr;
   uint32_t nhwords = 0;
+  // every 64th list entry (fence[f] = adj[64 f]), built on the first
+  // intersection over this CSR
+  uint32_t *fence = nullptr;
   // Pull copy of the lists with the hub neighbours first (hubs = the
   // kPullHubs vertices that appear in the most lists, i.e. whose frontier
   // rows are gathered most often; most-listed first), the rest ascending,

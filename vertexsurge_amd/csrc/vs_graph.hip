@@ -80,6 +80,7 @@ static void free_csr(DevCSR &d) {
   }
   if (d.pitems) (void)hipFree(d.pitems);
   if (d.hwords) (void)hipFree(d.hwords);
+  if (d.fence) (void)hipFree(d.fence);
   if (d.hadj) (void)hipFree(d.hadj);
   if (d.hubcnt) (void)hipFree(d.hubcnt);
   d = DevCSR{};

@@ -14,7 +14,12 @@
 // lists in lockstep (their loads are independent, so they overlap instead of
 // queueing one list after the other), each starting where the previous
 // step's largest element landed; ballot + mbcnt give the ascending output
-// positions.  Two passes: count -> exclusive scan -> fill.
+// positions.  Each search first bisects the list's fence -- every 64th entry
+// of the adjacency array, a copy 64 times smaller that stays cached for the
+// long (hub) lists -- and then the at most 64 entries between two fence
+// posts, one or two cache lines: a lookup in a list of a million entries
+// misses the cache once or twice instead of at every one of its last eight
+// or so probes.  Two passes: count -> exclusive scan -> fill.
 #include <algorithm>
 #include <cstring>
 
@@ -33,6 +38,7 @@ struct MiArgs {
   const uint32_t *tuples;
   const uint64_t *off[kMaxArity];
   const uint32_t *adj[kMaxArity];
+  const uint32_t *fence[kMaxArity];  // fence[d][f] = adj[d][64 f]
   uint32_t V[kMaxArity];
   uint32_t *counts;
   const uint64_t *offsets;
@@ -61,22 +67,21 @@ __device__ __forceinline__ uint64_t lower_bound_g(const uint32_t *__restrict__ a
   return lo;
 }
 
-template <bool FILL>
+template <bool FILL, int N>
 __global__ __launch_bounds__(kBlock) void k_mintersect(MiArgs a) {
   const int lane = lane_id(), wid = wave_id();
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
   for (uint64_t t = blockIdx.x * kWavesPerBlock + wid; t < a.T; t += nwaves) {
-    uint64_t b[kMaxArity];
-    uint32_t n[kMaxArity];
+    uint64_t b[N];
+    uint32_t n[N];
     bool empty = false;
-    uint32_t drv = 0;
+    int drv = 0;
     uint64_t best = ~0ull;
 #pragma unroll
-    for (int d = 0; d < kMaxArity; ++d) {
+    for (int d = 0; d < N; ++d) {
       b[d] = 0;
       n[d] = 0;
-      if (d >= static_cast<int>(a.arity)) continue;
-      const uint32_t x = a.tuples[t * a.arity + d];
+      const uint32_t x = a.tuples[t * N + d];
       if (x >= a.V[d]) {
         empty = true;
         continue;
@@ -93,26 +98,58 @@ __global__ __launch_bounds__(kBlock) void k_mintersect(MiArgs a) {
     uint64_t out = FILL ? a.offsets[t] : 0;
     if (!empty && best > 0) {
       const uint32_t *dl = a.adj[drv];
-      uint32_t from[kMaxArity];  // where this step's searches start, per list
+      uint32_t from[N];  // where this step's searches start, per list
 #pragma unroll
-      for (int d = 0; d < kMaxArity; ++d) from[d] = 0;
+      for (int d = 0; d < N; ++d) from[d] = 0;
       for (uint64_t base = b[drv]; base < b[drv] + best; base += 64) {
         const uint64_t idx = base + lane;
         const bool have = idx < b[drv] + best;
         const uint32_t x = have ? dl[idx] : 0xFFFFFFFFu;
-        uint32_t lo[kMaxArity], hi[kMaxArity];
+        // positions relative to each list's start; fence posts of list d sit
+        // at the multiples of 64 in [b + lo, b + hi), indices fa + [0, fn)
+        uint32_t lo[N], hi[N], flo[N], fhi[N];
+        uint64_t fa[N];
 #pragma unroll
-        for (int d = 0; d < kMaxArity; ++d) {
-          const bool on = d < static_cast<int>(a.arity) && d != static_cast<int>(drv);
+        for (int d = 0; d < N; ++d) {
+          const bool on = have && d != drv;
           lo[d] = on ? from[d] : 0;
           hi[d] = on ? n[d] : 0;
+          fa[d] = (b[d] + lo[d] + 63) >> 6;
+          flo[d] = 0;
+          fhi[d] = on ? static_cast<uint32_t>(((b[d] + hi[d] + 63) >> 6) - fa[d]) : 0;
         }
-        // lockstep lower_bound of x in every other list
+        // phase 1, all lists in lockstep: first fence post >= x
         bool more = have;
         while (more) {
           more = false;
 #pragma unroll
-          for (int d = 0; d < kMaxArity; ++d) {
+          for (int d = 0; d < N; ++d) {
+            if (flo[d] < fhi[d]) {
+              const uint32_t mid = (flo[d] + fhi[d]) >> 1;
+              if (a.fence[d][fa[d] + mid] < x) flo[d] = mid + 1;
+              else fhi[d] = mid;
+              more |= flo[d] < fhi[d];
+            }
+          }
+        }
+#pragma unroll
+        for (int d = 0; d < N; ++d) {
+          const bool on = have && d != drv;
+          const uint32_t fnr = on ? static_cast<uint32_t>(((b[d] + hi[d] + 63) >> 6) - fa[d]) : 0;
+          if (on && fnr > 0) {
+            const uint32_t f = flo[d];
+            // post f-1 is < x: the answer lies after it
+            if (f > 0) lo[d] = max(lo[d], static_cast<uint32_t>(64 * (fa[d] + f - 1) + 1 - b[d]));
+            // post f is >= x: the answer is at or before it
+            if (f < fnr) hi[d] = static_cast<uint32_t>(64 * (fa[d] + f) - b[d]);
+          }
+        }
+        // phase 2, lockstep: lower_bound within at most 64 entries
+        more = have;
+        while (more) {
+          more = false;
+#pragma unroll
+          for (int d = 0; d < N; ++d) {
             if (lo[d] < hi[d]) {
               const uint32_t mid = (lo[d] + hi[d]) >> 1;
               if (a.adj[d][b[d] + mid] < x) lo[d] = mid + 1;
@@ -123,13 +160,13 @@ __global__ __launch_bounds__(kBlock) void k_mintersect(MiArgs a) {
         }
         bool keep = have;
 #pragma unroll
-        for (int d = 0; d < kMaxArity; ++d) {
-          if (d >= static_cast<int>(a.arity) || d == static_cast<int>(drv)) continue;
+        for (int d = 0; d < N; ++d) {
+          if (d == drv) continue;
           keep = keep && lo[d] < n[d] && a.adj[d][b[d] + lo[d]] == x;
         }
         // the next step's elements are all larger than lane 63's
 #pragma unroll
-        for (int d = 0; d < kMaxArity; ++d) from[d] = __shfl(lo[d], 63);
+        for (int d = 0; d < N; ++d) from[d] = __shfl(lo[d], 63);
         const uint64_t m = __ballot(keep);
         if (FILL && keep) a.lists[out + lane_rank(m)] = x;
         out += __popcll(m);
@@ -138,6 +175,44 @@ __global__ __launch_bounds__(kBlock) void k_mintersect(MiArgs a) {
     }
     if (!FILL && lane == 0) a.counts[t] = total;
   }
+}
+
+template <bool FILL>
+static void launch_mintersect(uint32_t arity, uint32_t grid, hipStream_t s, const MiArgs &a) {
+  switch (arity) {
+    case 1: hipLaunchKernelGGL((k_mintersect<FILL, 1>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_mintersect<FILL, 2>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_mintersect<FILL, 3>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_mintersect<FILL, 4>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 5: hipLaunchKernelGGL((k_mintersect<FILL, 5>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 6: hipLaunchKernelGGL((k_mintersect<FILL, 6>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 7: hipLaunchKernelGGL((k_mintersect<FILL, 7>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_mintersect<FILL, 8>), dim3(grid), dim3(kBlock), 0, s, a); break;
+  }
+}
+
+// fence[f] = adj[64 f]
+__global__ void k_fence(const uint32_t *__restrict__ adj, uint64_t E, uint32_t *__restrict__ fence) {
+  const uint64_t nf = (E + 63) / 64;
+  for (uint64_t f = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; f < nf;
+       f += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    fence[f] = adj[f * 64];
+}
+
+static vs_status ensure_fence(vs_graph_s *g, DevCSR &c) {
+  if (c.fence) return VS_OK;
+  const uint64_t nf = (c.E + 63) / 64;
+  void *p = nullptr;
+  VS_HIP(hipMalloc(&p, std::max<uint64_t>(nf, 1) * 4));
+  c.fence = static_cast<uint32_t *>(p);
+  if (nf) {
+    hipLaunchKernelGGL(k_fence, dim3(grid_for((nf + 63) / 64, g->num_cus)), dim3(kBlock), 0,
+                       g->stream, c.adj, c.E, c.fence);
+    VS_HIP(hipGetLastError());
+    // the intersection may run on another graph's stream
+    VS_HIP(hipStreamSynchronize(g->stream));
+  }
+  return VS_OK;
 }
 
 __global__ void k_last_offset(const uint32_t *counts, uint64_t *offsets, uint64_t T) {
@@ -204,8 +279,10 @@ static vs_status mintersect_impl(const vs_graph_t *deps, uint32_t arity,
   a.T = T;
   a.tuples = d_tuples;
   for (uint32_t d = 0; d < arity; ++d) {
+    VS_TRY(ensure_fence(deps[d], deps[d]->fwd));
     a.off[d] = deps[d]->fwd.off;
     a.adj[d] = deps[d]->fwd.adj;
+    a.fence[d] = deps[d]->fwd.fence;
     a.V[d] = deps[d]->V;
   }
   a.counts = d_counts;
@@ -215,7 +292,7 @@ static vs_status mintersect_impl(const vs_graph_t *deps, uint32_t arity,
   struct EvGuard { hipEvent_t a, b; ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); } } eg{e0, e1};
   const uint32_t grid = grid_for(T, g->num_cus);
   VS_HIP(hipEventRecord(e0, s));
-  hipLaunchKernelGGL(k_mintersect<false>, dim3(grid), dim3(kBlock), 0, s, a);
+  launch_mintersect<false>(arity, grid, s, a);
   VS_HIP(hipGetLastError());
   uint64_t tot = 0;
   if (d_offsets) {
@@ -248,7 +325,7 @@ static vs_status mintersect_impl(const vs_graph_t *deps, uint32_t arity,
       }
       a.offsets = d_offsets;
       a.lists = d_lists;
-      hipLaunchKernelGGL(k_mintersect<true>, dim3(grid), dim3(kBlock), 0, s, a);
+      launch_mintersect<true>(arity, grid, s, a);
       VS_HIP(hipGetLastError());
     }
   }
