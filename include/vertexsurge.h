@@ -36,7 +36,8 @@ typedef enum {
   VS_ERR_HIP = 3,         /* a HIP runtime call failed */
   VS_ERR_CAPACITY = 4,    /* caller buffer too small; size reported back */
   VS_ERR_UNSUPPORTED = 5, /* option not implemented */
-  VS_ERR_NODEVICE = 6     /* no GPU visible */
+  VS_ERR_NODEVICE = 6,    /* no GPU visible */
+  VS_ERR_IO = 7           /* a file could not be opened, read or written */
 } vs_status;
 
 /* Message of the last failing call on this thread ("" if none). */
@@ -164,6 +165,52 @@ typedef struct {
 } vs_tuning;
 vs_status vs_graph_set_tuning(vs_graph_t g, const vs_tuning *t);
 vs_status vs_graph_get_tuning(vs_graph_t g, vs_tuning *t);
+
+/* ------------------------------------------------------------ graph store */
+/* A graph directory in the reference's storage format, read the way
+ * Graph::load (graph_store.cpp:47-73) and Graph::get_csr_edge_refs
+ * (graph_store.cpp:75-105) read it: schema.json (GraphSchema,
+ * schema.hh:10-141) plus one DiskArray file per variable (a 64-byte header
+ * {int64 count, int64 element size, reserved}, then the elements;
+ * disk_array.hh:267-354).  The relation's source-side CSR,
+ * "<src>-<type>-<dst>_csr_src_dst[_index]" ("->" when directed;
+ * schema.cpp:11-49), is mapped read-only; lists that are not ascending are
+ * sorted and repeated ids dropped (the importer keeps multi-edges and
+ * removes only adjacent duplicates, graph_loader.hh:71-77), which changes
+ * no VExpand or Mintersect result. */
+typedef struct vs_store_s *vs_store_t;
+typedef struct {
+  uint32_t num_vertices;        /* the vertex type's statistics.count */
+  uint64_t stored_entries;      /* CSR entries in the files (edge statistics.count) */
+  uint64_t num_edges;           /* entries kept */
+  uint64_t duplicates;          /* repeated ids dropped */
+  uint64_t unsorted_lists;      /* lists that were not ascending in the files */
+  uint64_t self_loops;          /* entries u -> u in the files (kept) */
+  int32_t directed;             /* EdgeSchema::directed */
+  int32_t allow_multiple_edge;  /* EdgeSchema::allow_multiple_edge */
+} vs_store_info;
+/* edge: the relation's type name ("knows") or a simple name
+ * "Person_knows_Person" (GraphSchema::get_edge_schema, schema.cpp:222-232);
+ * NULL or "" picks the schema's only relation of a vertex type to itself.
+ * Relations between two vertex types return VS_ERR_UNSUPPORTED (a device
+ * graph is one vertex set; VExpand needs a self relation for k > 1,
+ * vexpand.cpp:88-91). */
+vs_status vs_store_open(const char *dir, const char *edge, vs_store_t *out);
+vs_status vs_store_get_info(vs_store_t s, vs_store_info *info);
+/* The CSR (num_vertices + 1 offsets, num_edges ids); valid until close. */
+vs_status vs_store_csr(vs_store_t s, const uint64_t **offsets, const uint32_t **adj);
+/* vs_graph_create over the store's CSR, VS_GRAPH_SYMMETRIC for an undirected
+ * relation (stored with both directions, graph_concepts.hh:96-103). */
+vs_status vs_store_graph(vs_store_t s, vs_graph_t *out);
+vs_status vs_store_close(vs_store_t s);
+/* Writes a CSR as a store the reference can load (gen_and_write_one_edge,
+ * graph_loader.hh:152-200, for one relation of vertex_type to itself): the
+ * forward CSR files, the reverse CSR files when directed, and schema.json
+ * with no flat edge files listed.  An undirected relation's adj must hold
+ * both directions of every edge.  dir must exist. */
+vs_status vs_store_write(const char *dir, const char *graph_name, const char *vertex_type,
+                         const char *edge_type, int32_t directed, const uint64_t *offsets,
+                         const uint32_t *adj, uint32_t num_vertices, uint64_t num_edges);
 
 /* ---------------------------------------------------------------- VExpand */
 /* Parameters of VExpand (big_operators.hh:101-129, vexpand.cpp:7-101). */

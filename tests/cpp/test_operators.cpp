@@ -15,7 +15,7 @@ static void print_vec(const char *name, const std::vector<uint64_t> &v, bool las
   std::printf("]%s\n", last ? "" : ",");
 }
 
-int main() {
+int main(int argc, char **argv) {
   vs_csr_t csr = nullptr;
   check(vs_rmat_csr(12, 16, 0.57, 0.19, 0.19, 1, &csr), "vs_rmat_csr");
   uint32_t V = 0;
@@ -157,7 +157,22 @@ int main() {
     std::printf("\"row_or_size\": %zu,\n", MatrixRowOrSum().run(reach).size());
     // MatrixTableSelect of three columns, then its tuples
     auto sel = MatrixTableSelect("c").run(reach, {starts[0], starts[1], starts[0]});
-    std::printf("\"select_tuple_count\": %zu\n", sel.tuple_count());
+    std::printf("\"select_tuple_count\": %zu,\n", sel.tuple_count());
+  }
+  // a graph directory in the reference's storage format (argv[1]), loaded in
+  // place of the R-MAT relation
+  {
+    const std::string dir = argc > 1 ? argv[1] : "tests/golden/store/knows";
+    GraphStore::instance().load(dir, "Person", "knows", "Person");
+    const std::vector<VID> all = {0, 1, 2, 3, 4};
+    VExpand vs1("Person", "Person", "knows", Direction::Both, 1, 3, ExpandPathType::Shortest,
+                ExpandStrategy::Dense, "p.id", std::nullopt);
+    vs1.run(all);
+    print_vec("store_shortest_row_counts", vs1.results().row_counts());
+    VExpand va2("Person", "Person", "knows", Direction::Both, 1, 2,
+                VExpandPathParameters::Any(), ExpandStrategy::Dense, "p.id", std::nullopt);
+    va2.run(all);
+    print_vec("store_any_row_counts", va2.results().row_counts(), true);
   }
   std::printf("}\n");
   GraphStore::instance().clear();

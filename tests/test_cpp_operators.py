@@ -15,7 +15,8 @@ DRIVER = Path(__file__).resolve().parent.parent / "vertexsurge_amd" / "_build" /
 
 
 def test_cpp_operators_match_oracle(orc):
-    out = subprocess.run([str(DRIVER)], check=True, capture_output=True, text=True,
+    store = Path(__file__).resolve().parent / "golden" / "store" / "knows"
+    out = subprocess.run([str(DRIVER), str(store)], check=True, capture_output=True, text=True,
                          timeout=120).stdout
     r = json.loads(out)
     g = vs.HostCSR.rmat(12, 16, seed=1)
@@ -57,3 +58,6 @@ def test_cpp_operators_match_oracle(orc):
     assert r["row_or_size"] == len(ot.row_or_sum(M, None))
     sel, _, _ = ot.select_cols(M, starts, [starts[0], starts[1], starts[0]])
     assert r["select_tuple_count"] == int(sel.sum())
+    # the hand-derived knows store: sets 0:{1,2} 1:{0,2} 2:{0,1} 3:{3} 4:{}
+    assert r["store_shortest_row_counts"] == [2, 2, 2, 0, 0]
+    assert r["store_any_row_counts"] == [3, 3, 3, 1, 0]

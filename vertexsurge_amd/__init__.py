@@ -17,7 +17,7 @@ from ._lib import (DIR_BACKWARD, DIR_BOTH, DIR_FORWARD, LAYOUT_DVSM, LAYOUT_NATI
                    SCHED_HEURISTIC, SCHED_LEARNED, SCHED_PULL, SCHED_PUSH, VSError)
 
 __all__ = [
-    "HostCSR", "Graph", "VExpandResult", "vexpand", "find_shortest", "mintersect",
+    "HostCSR", "Graph", "GraphStore", "store_write", "VExpandResult", "vexpand", "find_shortest", "mintersect",
     "ColMat", "mintersect_colmat", "pack_colmain", "unpack_colmain",
     "sample_sources", "sample_paths3", "Scheduler", "vexpand_batch", "device_count",
     "PATH_ANY", "PATH_SHORTEST", "PATH_FIND_SHORTEST", "DIR_FORWARD", "DIR_BACKWARD",
@@ -130,6 +130,23 @@ class Graph:
     def from_csr(cls, csr: HostCSR, symmetric: bool = True) -> "Graph":
         return cls(csr.offsets, csr.adj, symmetric)
 
+    @classmethod
+    def _wrap(cls, handle: C.c_void_p) -> "Graph":
+        g = cls.__new__(cls)
+        g._h = handle
+        V, E, dev = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        _lib.call("vs_graph_info", handle, C.byref(V), C.byref(E), C.byref(dev))
+        g.V, g.E = V.value, E.value
+        g._off = g._adj = None
+        return g
+
+    @classmethod
+    def load(cls, directory, edge: str | None = None) -> "Graph":
+        """Device graph from a directory in the reference's storage format
+        (schema.json + DiskArray files; Graph::load, graph_store.cpp:47-73)."""
+        with GraphStore(directory, edge) as st:
+            return st.graph()
+
     @property
     def handle(self):
         return self._h
@@ -161,6 +178,65 @@ class Graph:
             self.close()
         except Exception:
             pass
+
+
+class GraphStore:
+    """One edge relation of a graph directory in the reference's storage
+    format, opened on the host (vs_store_open): schema.json + the relation's
+    source-side CSR DiskArray files, with lists normalised to ascending sets."""
+
+    def __init__(self, directory, edge: str | None = None):
+        self._h = C.c_void_p()
+        _lib.call("vs_store_open", str(directory).encode(),
+                  edge.encode() if edge else None, C.byref(self._h))
+        st = _lib.StoreInfo()
+        _lib.call("vs_store_get_info", self._h, C.byref(st))
+        self.info = {k: getattr(st, k) for k, _ in _lib.StoreInfo._fields_}
+        self.V = st.num_vertices
+        self.E = st.num_edges
+
+    def csr(self):
+        """(offsets uint64[V+1], adj uint32[E]) copies of the normalised CSR."""
+        off, adj = _lib.u64p(), _lib.u32p()
+        _lib.call("vs_store_csr", self._h, C.byref(off), C.byref(adj))
+        o = np.ctypeslib.as_array(off, shape=(self.V + 1,)).copy()
+        a = (np.ctypeslib.as_array(adj, shape=(self.E,)).copy() if self.E
+             else np.zeros(0, dtype=np.uint32))
+        return o, a
+
+    def graph(self) -> "Graph":
+        h = C.c_void_p()
+        _lib.call("vs_store_graph", self._h, C.byref(h))
+        return Graph._wrap(h)
+
+    def close(self):
+        if self._h:
+            _lib.call("vs_store_close", self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def store_write(directory, offsets, adj, vertex_type: str = "Vertex", edge_type: str = "edge",
+                directed: bool = False, graph_name: str = "") -> None:
+    """Writes one relation of vertex_type to itself as a store the reference's
+    Graph::load reads (vs_store_write).  directory must exist."""
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    a = _u32(adj)
+    V = len(off) - 1
+    _lib.call("vs_store_write", str(directory).encode(), graph_name.encode(),
+              vertex_type.encode(), edge_type.encode(), int(directed), _ptr(off, C.c_uint64),
+              _ptr(a, C.c_uint32), V, int(off[-1]) if V >= 0 else 0)
 
 
 def stats_dict(st: _lib.VexpandStats) -> dict:

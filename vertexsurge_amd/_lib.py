@@ -17,6 +17,7 @@ VS_ERR_HIP = 3
 VS_ERR_CAPACITY = 4
 VS_ERR_UNSUPPORTED = 5
 VS_ERR_NODEVICE = 6
+VS_ERR_IO = 7
 
 PATH_ANY, PATH_SHORTEST, PATH_FIND_SHORTEST = 0, 1, 2
 DIR_FORWARD, DIR_BACKWARD, DIR_BOTH = 0, 1, 2
@@ -84,6 +85,13 @@ class BatchStats(C.Structure):
                 ("bytes_sparse", C.c_uint64)]
 
 
+class StoreInfo(C.Structure):
+    _fields_ = [("num_vertices", C.c_uint32), ("stored_entries", C.c_uint64),
+                ("num_edges", C.c_uint64), ("duplicates", C.c_uint64),
+                ("unsorted_lists", C.c_uint64), ("self_loops", C.c_uint64),
+                ("directed", C.c_int32), ("allow_multiple_edge", C.c_int32)]
+
+
 class MintersectStats(C.Structure):
     _fields_ = [("ms", C.c_double), ("total", C.c_uint64), ("bytes_alg", C.c_uint64)]
 
@@ -116,6 +124,13 @@ SIGNATURES = {
     "vs_graph_trim": (C.c_int, [vp]),
     "vs_graph_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
     "vs_graph_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+    "vs_store_open": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(vp)]),
+    "vs_store_get_info": (C.c_int, [vp, C.POINTER(StoreInfo)]),
+    "vs_store_csr": (C.c_int, [vp, C.POINTER(u64p), C.POINTER(u32p)]),
+    "vs_store_graph": (C.c_int, [vp, C.POINTER(vp)]),
+    "vs_store_close": (C.c_int, [vp]),
+    "vs_store_write": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int32,
+                                 u64p, u32p, C.c_uint32, C.c_uint64]),
     "vs_vexpand": (C.c_int, [vp, u32p, C.c_uint32, C.POINTER(VexpandParams), C.POINTER(vp),
                              C.POINTER(VexpandStats)]),
     "vs_find_shortest": (C.c_int, [vp, u32p, u32p, C.c_uint32, C.c_int32, C.c_uint32, i32p,

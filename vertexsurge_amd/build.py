@@ -23,7 +23,7 @@ INCLUDE = ROOT / "include"
 
 HIP_SOURCES = ["vs_graph.hip", "vs_vexpand.hip", "vs_mintersect.hip", "vs_mlp.hip",
                "vs_batch.hip", "vs_tables.hip"]
-HOST_SOURCES = ["vs_host.cpp"]
+HOST_SOURCES = ["vs_host.cpp", "vs_store.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 GXX = os.environ.get("CXX_HOST", "g++")

@@ -68,6 +68,8 @@ class EdgeRelation {
                           symmetric ? VS_GRAPH_SYMMETRIC : 0u, &g_),
           "vs_graph_create");
   }
+  // adopts a graph handle (vs_store_graph)
+  explicit EdgeRelation(vs_graph_t g) : g_(g) {}
   ~EdgeRelation() { vs_graph_destroy(g_); }
   EdgeRelation(const EdgeRelation &) = delete;
   EdgeRelation &operator=(const EdgeRelation &) = delete;
@@ -99,6 +101,21 @@ class GraphStore {
     if (it == rels_.end())
       throw std::runtime_error("no edge schema " + key(src, edge, dst));  // schema.cpp:217
     return it->second;
+  }
+  // Graph::load (graph_store.cpp:47-73) of one relation src -edge-> dst from
+  // a graph directory in the reference's storage format (schema.json +
+  // DiskArray files); the relation is registered as add_edge does.
+  std::shared_ptr<EdgeRelation> load(const std::string &dir, const TypeName &src,
+                                     const TypeName &edge, const TypeName &dst) {
+    vs_store_t st = nullptr;
+    check(vs_store_open(dir.c_str(), key(src, edge, dst).c_str(), &st), "vs_store_open");
+    vs_graph_t g = nullptr;
+    const vs_status s = vs_store_graph(st, &g);
+    vs_store_close(st);
+    check(s, "vs_store_graph");
+    auto rel = std::make_shared<EdgeRelation>(g);
+    add_edge(src, edge, dst, rel);
+    return rel;
   }
   void clear() { rels_.clear(); }
 
