@@ -57,3 +57,20 @@ def test_counts_only_and_empty():
     assert l is None and st["total"] == int(c.sum())
     c, o, l, st = vs.mintersect([G, G, G], np.zeros((0, 3), dtype=np.uint32))
     assert len(c) == 0 and o.tolist() == [0]
+
+
+def test_lists_must_be_sets():
+    off = np.array([0, 3, 5, 6], dtype=np.uint64)
+    tup = np.array([[0, 1]], dtype=np.uint32)
+    # a repeated id, then a misordered list, in list 0
+    for adj in ([1, 1, 2, 0, 2, 0], [2, 1, 0, 0, 2, 0]):
+        G = vs.Graph(off, np.array(adj, dtype=np.uint32), symmetric=False)
+        for _ in range(2):  # the check is remembered per CSR
+            with pytest.raises(vs.VSError) as e:
+                vs.mintersect([G, G], tup)
+            assert e.value.code == vs._lib.VS_ERR_INVALID
+        G.close()
+    G = vs.Graph(off, np.array([0, 1, 2, 0, 2, 0], dtype=np.uint32), symmetric=False)
+    c, o, l, _ = vs.mintersect([G, G], tup)
+    assert c.tolist() == [2] and l.tolist() == [0, 2]
+    G.close()

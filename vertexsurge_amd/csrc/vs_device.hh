@@ -73,6 +73,7 @@ struct DevCSR {
   // every 64th list entry (fence[f] = adj[64 f]), built on the first
   // intersection over this CSR
   uint32_t *fence = nullptr;
+  int sets = -1;  // lists strictly ascending: 1 yes, 0 no, -1 not checked
   // Pull copy of the lists with the hub neighbours first (hubs = the
   // kPullHubs vertices that appear in the most lists, i.e. whose frontier
   // rows are gathered most often; most-listed first), the rest ascending,

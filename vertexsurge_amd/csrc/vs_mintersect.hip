@@ -199,18 +199,50 @@ __global__ void k_fence(const uint32_t *__restrict__ adj, uint64_t E, uint32_t *
     fence[f] = adj[f * 64];
 }
 
+// *bad = 1 if some list is not strictly ascending; one wave per vertex
+__global__ __launch_bounds__(kBlock) void k_check_sets(const uint64_t *__restrict__ off,
+                                                       const uint32_t *__restrict__ adj,
+                                                       uint32_t V, int *bad) {
+  const int lane = lane_id(), wid = wave_id();
+  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
+  bool ok = true;
+  for (uint64_t u = blockIdx.x * kWavesPerBlock + wid; u < V; u += nwaves) {
+    const uint64_t e = off[u + 1];
+    for (uint64_t i = off[u] + lane; i + 1 < e; i += 64) ok = ok && adj[i] < adj[i + 1];
+  }
+  if (!ok) *bad = 1;
+}
+
+// Builds the fence of a CSR and checks, once, that its lists are sets: the
+// searches need strictly ascending lists (a repeated id in the driving list
+// would be emitted twice, where the reference's bitmap AND yields it once).
 static vs_status ensure_fence(vs_graph_s *g, DevCSR &c) {
-  if (c.fence) return VS_OK;
-  const uint64_t nf = (c.E + 63) / 64;
-  void *p = nullptr;
-  VS_HIP(hipMalloc(&p, std::max<uint64_t>(nf, 1) * 4));
-  c.fence = static_cast<uint32_t *>(p);
-  if (nf) {
-    hipLaunchKernelGGL(k_fence, dim3(grid_for((nf + 63) / 64, g->num_cus)), dim3(kBlock), 0,
-                       g->stream, c.adj, c.E, c.fence);
-    VS_HIP(hipGetLastError());
+  if (!c.fence) {
+    const uint64_t nf = (c.E + 63) / 64;
+    void *p = nullptr, *flag = nullptr;
+    VS_HIP(hipMalloc(&flag, sizeof(int)));
+    struct Free { void *p; ~Free() { (void)hipFree(p); } } fr{flag};
+    VS_HIP(hipMalloc(&p, std::max<uint64_t>(nf, 1) * 4));
+    c.fence = static_cast<uint32_t *>(p);
+    VS_HIP(hipMemsetAsync(flag, 0, sizeof(int), g->stream));
+    if (nf) {
+      hipLaunchKernelGGL(k_fence, dim3(grid_for((nf + 63) / 64, g->num_cus)), dim3(kBlock), 0,
+                         g->stream, c.adj, c.E, c.fence);
+      VS_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_check_sets, dim3(grid_for(g->V, g->num_cus)), dim3(kBlock), 0,
+                         g->stream, c.off, c.adj, g->V, static_cast<int *>(flag));
+      VS_HIP(hipGetLastError());
+    }
+    int bad = 0;
+    VS_HIP(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, g->stream));
     // the intersection may run on another graph's stream
     VS_HIP(hipStreamSynchronize(g->stream));
+    c.sets = bad ? 0 : 1;
+  }
+  if (!c.sets) {
+    set_error("vs_mintersect: an adjacency list of a dependency repeats or misorders ids; "
+              "intersections need ascending sets (vs_store_open builds them from a store)");
+    return VS_ERR_INVALID;
   }
   return VS_OK;
 }
