@@ -32,6 +32,16 @@ namespace vs {
 
 constexpr uint32_t kMaxSlots = 64;
 
+// Counters and front words are accumulated by every wave of the expand and
+// epilogue kernels.  Thousands of waves adding into one 80-byte HopCounters
+// (one L2 line) and 64 front words serialise on that line, so each wave
+// adds into one of kParts partial copies (picked by its block), 128-byte
+// apart; the next hop's decision kernel (or the finishing kernel for the
+// last hop) folds the partials into the primary copy and zeroes them.
+constexpr uint32_t kParts = 16;
+constexpr size_t kPartStride = 128;  // bytes between partial HopCounters
+static_assert(sizeof(HopCounters) <= kPartStride, "HopCounters outgrew its partial slot");
+
 struct SlotDesc {
   uint64_t *F[2];
   uint64_t *Fm[2];   // word masks of F[0] / F[1] (V each)
@@ -43,6 +53,8 @@ struct SlotDesc {
   uint64_t *R;
   HopCounters *ctr;  // [H + 1]
   uint64_t *front;   // [H + 1][64]: per-word OR of the rows of L_h
+  char *part;        // [H + 1][kParts] partial HopCounters, kPartStride apart
+  uint64_t *fpart;   // [H + 1][kParts][64] partial front words
   uint32_t tile;     // global tile of the current wave, UINT32_MAX if idle
   int32_t mode;      // this hop: 0 skip, 1 push, 2 pull
   uint32_t dense;    // this hop: pull without the active-bitmap lookup
@@ -127,7 +139,8 @@ __device__ __forceinline__ HopArgs slot_args(const BatchConst &c, const SlotDesc
   a.R = c.use_r ? sd.R : nullptr;
   a.items = sd.items;
   a.nitems = &sd.ctr[hop - 1].items;
-  a.ctr = &sd.ctr[hop];
+  const size_t part = static_cast<size_t>(hop) * kParts + blockIdx.x % kParts;
+  a.ctr = reinterpret_cast<HopCounters *>(sd.part + part * kPartStride);
   a.live = c.live + static_cast<size_t>(hop) * c.Wpad + ti.word0;
   a.live_next = c.live + static_cast<size_t>(hop + 1) * c.Wpad + ti.word0;
   a.racc = c.racc ? c.racc + static_cast<size_t>(hop) * c.Wpad + ti.word0 : nullptr;
@@ -137,8 +150,60 @@ __device__ __forceinline__ HopArgs slot_args(const BatchConst &c, const SlotDesc
   a.ewords = nullptr;
   a.newords = 0;
   a.front = sd.front + static_cast<size_t>(hop - 1) * 64;
-  a.front_next = sd.front + static_cast<size_t>(hop) * 64;
+  a.front_next = sd.fpart + part * 64;
   return a;
+}
+
+// Adds the accumulated fields of b into a (not items / how).
+__device__ __forceinline__ void add_ctr(HopCounters &a, const HopCounters &b) {
+  a.pop += b.pop;
+  a.nz += b.nz;
+  a.act += b.act;
+  a.actdeg += b.actdeg;
+  a.te += b.te;
+  a.visnew += b.visnew;
+  a.rnew += b.rnew;
+  a.nzdeg += b.nzdeg;
+}
+
+__device__ __forceinline__ HopCounters *part_ctr(const SlotDesc &sd, uint32_t h, uint32_t p) {
+  return reinterpret_cast<HopCounters *>(sd.part + (static_cast<size_t>(h) * kParts + p) *
+                                                       kPartStride);
+}
+
+// One wave: folds hop h's partial counters and front words into the primary
+// copy, zeroes the partials, and returns the folded counters to every lane.
+__device__ HopCounters fold_parts(const SlotDesc &sd, uint32_t h) {
+  const int lane = lane_id();
+  HopCounters q{};
+  if (lane < static_cast<int>(kParts)) {
+    HopCounters *pp = part_ctr(sd, h, lane);
+    q = *pp;
+    *pp = HopCounters{};
+  }
+  HopCounters t = sd.ctr[h];
+  HopCounters w{};
+  w.pop = wave_sum(q.pop);
+  w.nz = wave_sum(q.nz);
+  w.act = wave_sum(q.act);
+  w.actdeg = wave_sum(q.actdeg);
+  w.te = wave_sum(q.te);
+  w.visnew = wave_sum(q.visnew);
+  w.rnew = wave_sum(q.rnew);
+  w.nzdeg = wave_sum(q.nzdeg);
+  add_ctr(t, w);
+  uint64_t f = 0;
+  uint64_t *fp = sd.fpart + static_cast<size_t>(h) * kParts * 64 + lane;
+  for (uint32_t p = 0; p < kParts; ++p) {
+    const uint64_t x = fp[p * 64];
+    if (x) {
+      f |= x;
+      fp[p * 64] = 0;
+    }
+  }
+  if (f) sd.front[static_cast<size_t>(h) * 64 + lane] |= f;
+  if (lane == 0) add_ctr(sd.ctr[h], w);
+  return t;
 }
 
 // ---- wave init: slot -> tile, zero counters (one block)
@@ -161,17 +226,19 @@ __global__ void k_bw_assign(BatchConst c) {
 // memset node inside the captured graph)
 __global__ __launch_bounds__(kBlock) void k_bw_zero(BatchConst c) {
   const size_t words = static_cast<size_t>(c.V) * c.TW;
-  const size_t per = words * (c.use_r ? 2 : 1);
-  const size_t total = per * c.nslots;
   const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
-  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < total;
-       i += stride) {
-    const uint32_t s = static_cast<uint32_t>(i / per);
+  const size_t tid = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  for (uint32_t s = 0; s < c.nslots; ++s) {
     const SlotDesc &sd = c.slots[s];
     if (sd.tile == UINT32_MAX) continue;
-    const size_t k = i % per;
-    uint64_t *M = k < words ? sd.Vis : sd.R;
-    if (M) M[k % words] = 0;
+    for (int m = 0; m < 2; ++m) {
+      uint64_t *M = m == 0 ? sd.Vis : (c.use_r ? sd.R : nullptr);
+      if (!M) continue;
+      // 16-byte stores (the matrices are hipMalloc'd, 256-byte aligned)
+      uint4 *M4 = reinterpret_cast<uint4 *>(M);
+      for (size_t i = tid; i < words / 2; i += stride) M4[i] = make_uint4(0, 0, 0, 0);
+      if ((words & 1) && tid == 0) M[words - 1] = 0;
+    }
   }
 }
 
@@ -229,9 +296,9 @@ __global__ __launch_bounds__(64) void k_bw_decide(BatchConst c, uint32_t hop) {
   SlotDesc &sd = c.slots[s];
   const int lane = lane_id();
   int mode = 0;
+  const HopCounters pc = fold_parts(sd, hop - 1);
   if (sd.tile != UINT32_MAX) {
     const TileInfo ti = c.tiles[sd.tile];
-    const HopCounters pc = sd.ctr[hop - 1];
     if (pc.pop > 0 && hop <= ti.hmax) {
       if (c.sched == VS_SCHED_PUSH) mode = 1;
       else if (c.sched == VS_SCHED_PULL) mode = 2;
@@ -239,8 +306,8 @@ __global__ __launch_bounds__(64) void k_bw_decide(BatchConst c, uint32_t hop) {
         const int heur = (pc.actdeg * static_cast<uint64_t>(c.alpha) < c.E) ? 1 : 2;
         mode = heur;
         if (c.sched == VS_SCHED_LEARNED && c.mlp) {
-          uint64_t visnz = 0;
-          for (uint32_t h = 0; h < hop; ++h) visnz += sd.ctr[h].visnew;
+          uint64_t visnz = pc.visnew;
+          for (uint32_t h = 0; h + 1 < hop; ++h) visnz += sd.ctr[h].visnew;
           if (lane == 0) {
             s_x[0] = log2f(1.f + static_cast<float>(pc.pop)) / 32.f;
             s_x[1] = log2f(1.f + static_cast<float>(pc.act)) / 32.f;
@@ -266,10 +333,9 @@ __global__ __launch_bounds__(64) void k_bw_decide(BatchConst c, uint32_t hop) {
   if (lane == 0) {
     sd.mode = mode;
     // most in-edges come from active vertices: pull skips the bitmap lookup
-    sd.dense = (mode == 2 && sd.ctr[hop - 1].actdeg * 4 > 3 * c.E) ? 1u : 0u;
+    sd.dense = (mode == 2 && pc.actdeg * 4 > 3 * c.E) ? 1u : 0u;
     // rows sparse in words: gather only the words their masks name
-    sd.masked = (sd.ctr[hop - 1].nzdeg * 100 < sd.ctr[hop - 1].actdeg * c.TW * c.sparse_pct)
-                    ? 1u : 0u;
+    sd.masked = (pc.nzdeg * 100 < pc.actdeg * c.TW * c.sparse_pct) ? 1u : 0u;
     sd.ctr[hop].how = (mode == 2 && sd.masked) ? 3u : static_cast<ull>(mode);
     if (mode == 1) atomicAdd(&c.ctl->push_dec, 1ull);
     if (mode == 2) atomicAdd(&c.ctl->pull_dec, 1ull);
@@ -398,52 +464,126 @@ __global__ __launch_bounds__(kBlock) void k_bw_clear(BatchConst c, uint32_t whic
   for (ull u = blockIdx.x * kWavesPerBlock + wid; u < total; u += nwaves) {
     const SlotDesc &sd = c.slots[u / nvb];
     if (sd.tile == UINT32_MAX) continue;
-    clear_block<TW>(sd.F[which & 1], sd.Fm[which & 1], sd.bm[which & 1],
-                    static_cast<uint32_t>(u % nvb));
+    clear_block_masked<TW>(sd.F[which & 1], sd.Fm[which & 1], sd.bm[which & 1],
+                           static_cast<uint32_t>(u % nvb));
+  }
+}
+
+// Carry-save adder: h:l = a + b + c, bitwise.
+__device__ __forceinline__ void csa(uint64_t &h, uint64_t &l, uint64_t a, uint64_t b,
+                                    uint64_t c) {
+  const uint64_t u = a ^ b;
+  h = (a & b) | (u & c);
+  l = u ^ c;
+}
+
+// Adds x, weighted 2^K, into a bit-sliced counter (plane k = bit k of every
+// bit position's count).
+template <int NB, int K>
+__device__ __forceinline__ void vadd(uint64_t (&cnt)[NB], uint64_t x) {
+#pragma unroll
+  for (int k = K; k < NB; ++k) {
+    const uint64_t t = cnt[k] & x;
+    cnt[k] ^= x;
+    x = t;
   }
 }
 
 // ---- per-row popcount and digest of each active slot's result
+// Each lane counts, per bit position, the set bits of its word over the
+// chunk's vertices: words are combined 16 at a time by a carry-save tree
+// (ones / twos / fours / eights carried across groups, one sixteens word per
+// group added into the bit-sliced counter), about 6 logic ops per word
+// instead of 3 per counter plane.  The four waves of a block work on chunks
+// of one slot at a time and add their per-row counts into LDS; the block
+// adds the LDS counts to the output once per slot, not once per chunk.
 template <int TW>
 __global__ __launch_bounds__(kBlock) void k_bw_rows(BatchConst c) {
   constexpr int G = 64 / TW;
   constexpr uint32_t CV = 2048;
-  constexpr int NB = 14;
+  constexpr int NB = 12;  // counts up to CV = 2^11
+  // counts of the block's current slot: row j * 64 + b at [b * 64 + j], so
+  // the lanes of one bit b add to consecutive banks
+  __shared__ uint32_t s_cnt[64 * 64];
   const int lane = lane_id(), wid = wave_id();
   const int g = lane / TW, j = lane % TW;
   const uint32_t nch = (c.V + CV - 1) / CV;
-  const ull total = static_cast<ull>(c.nslots) * nch;
-  const ull nwaves = static_cast<ull>(gridDim.x) * kWavesPerBlock;
-  for (ull it = blockIdx.x * kWavesPerBlock + wid; it < total; it += nwaves) {
-    const SlotDesc &sd = c.slots[it / nch];
-    if (sd.tile == UINT32_MAX) continue;
+  const uint32_t nchp = (nch + kWavesPerBlock - 1) / kWavesPerBlock * kWavesPerBlock;
+  const ull total = static_cast<ull>(c.nslots) * nchp;  // chunks, padded per slot
+  for (uint32_t i = threadIdx.x; i < 64 * 64; i += blockDim.x) s_cnt[i] = 0;
+  __syncthreads();
+  uint32_t cur = UINT32_MAX;  // slot whose counts s_cnt holds (block-uniform)
+  auto flush = [&](uint32_t s) {
+    __syncthreads();
+    const SlotDesc &sd = c.slots[s];
     const TileInfo ti = c.tiles[sd.tile];
+    for (uint32_t i = threadIdx.x; i < 64 * 64; i += blockDim.x) {
+      const uint32_t n = s_cnt[i];
+      if (n) {
+        const uint32_t lr = (i % 64) * 64 + i / 64;
+        if (lr < ti.rows) atomicAdd(&c.out_counts[ti.row0 + lr], static_cast<ull>(n));
+        s_cnt[i] = 0;
+      }
+    }
+    __syncthreads();
+  };
+  for (ull q = static_cast<ull>(blockIdx.x) * kWavesPerBlock; q < total;
+       q += static_cast<ull>(gridDim.x) * kWavesPerBlock) {
+    const uint32_t s = static_cast<uint32_t>(q / nchp);  // same slot for the block
+    const SlotDesc &sd = c.slots[s];
+    if (sd.tile == UINT32_MAX) continue;
+    if (s != cur) {
+      if (cur != UINT32_MAX) flush(cur);
+      cur = s;
+    }
+    const uint32_t ch = static_cast<uint32_t>((q + wid) % nchp);
+    if (ch >= nch) continue;
     const uint64_t *M = c.use_r ? sd.R : sd.Vis;
-    const uint32_t ch = static_cast<uint32_t>(it % nch);
     const uint32_t v0 = ch * CV, v1 = min(c.V, v0 + CV);
     uint64_t cnt[NB];
 #pragma unroll
     for (int k = 0; k < NB; ++k) cnt[k] = 0;
-    for (uint32_t v = v0 + g; v < v1; v += G) {
-      uint64_t carry = M[static_cast<size_t>(v) * TW + j];
+    uint64_t ones = 0, twos = 0, fours = 0, eights = 0;
+    uint32_t v = v0 + g;
+    for (; v + 15 * G < v1; v += 16 * G) {
+      uint64_t d[16];
 #pragma unroll
-      for (int k = 0; k < NB; ++k) {
-        const uint64_t t2 = cnt[k] & carry;
-        cnt[k] ^= carry;
-        carry = t2;
-      }
+      for (int q = 0; q < 16; ++q) d[q] = M[static_cast<size_t>(v + q * G) * TW + j];
+      uint64_t twosA, twosB, foursA, foursB, eightsA, eightsB, sixteens;
+      csa(twosA, ones, ones, d[0], d[1]);
+      csa(twosB, ones, ones, d[2], d[3]);
+      csa(foursA, twos, twos, twosA, twosB);
+      csa(twosA, ones, ones, d[4], d[5]);
+      csa(twosB, ones, ones, d[6], d[7]);
+      csa(foursB, twos, twos, twosA, twosB);
+      csa(eightsA, fours, fours, foursA, foursB);
+      csa(twosA, ones, ones, d[8], d[9]);
+      csa(twosB, ones, ones, d[10], d[11]);
+      csa(foursA, twos, twos, twosA, twosB);
+      csa(twosA, ones, ones, d[12], d[13]);
+      csa(twosB, ones, ones, d[14], d[15]);
+      csa(foursB, twos, twos, twosA, twosB);
+      csa(eightsB, fours, fours, foursA, foursB);
+      csa(sixteens, eights, eights, eightsA, eightsB);
+      vadd<NB, 4>(cnt, sixteens);
     }
+    for (; v < v1; v += G) vadd<NB, 0>(cnt, M[static_cast<size_t>(v) * TW + j]);
+    vadd<NB, 0>(cnt, ones);
+    vadd<NB, 1>(cnt, twos);
+    vadd<NB, 2>(cnt, fours);
+    vadd<NB, 3>(cnt, eights);
     for (int b = 0; b < 64; ++b) {
       uint32_t n = 0;
 #pragma unroll
       for (int k = 0; k < NB; ++k) n |= static_cast<uint32_t>((cnt[k] >> b) & 1) << k;
 #pragma unroll
       for (int m = TW; m < 64; m <<= 1) n += __shfl_xor(n, m);
-      const uint32_t lr = j * 64 + b;
-      if (g == 0 && n && lr < ti.rows) atomicAdd(&c.out_counts[ti.row0 + lr], static_cast<ull>(n));
+      if (g == 0 && n) atomicAdd(&s_cnt[b * 64 + j], n);
     }
   }
+  if (cur != UINT32_MAX) flush(cur);
   if (!c.out_digests) return;
+  const ull nwaves = static_cast<ull>(gridDim.x) * kWavesPerBlock;
   // digests: lane = row bit, walk 64-vertex blocks of each word
   const uint32_t nvb = (c.V + 63) / 64;
   constexpr uint32_t BPW = 16;
@@ -477,6 +617,27 @@ __global__ __launch_bounds__(kBlock) void k_bw_rows(BatchConst c) {
 
 // ---- visited-trick correction, TE accumulation, advance the wave
 __global__ void k_bw_finish(BatchConst c) {
+  if (blockIdx.x == 0) {
+    // fold the last hop's partials (earlier hops were folded by decide)
+    for (uint32_t i = threadIdx.x; i < c.nslots * 64; i += blockDim.x) {
+      const SlotDesc &sd = c.slots[i / 64];
+      uint64_t *fp = sd.fpart + static_cast<size_t>(c.H) * kParts * 64 + i % 64;
+      uint64_t f = 0;
+      for (uint32_t p = 0; p < kParts; ++p) {
+        f |= fp[p * 64];
+        fp[p * 64] = 0;
+      }
+      sd.front[static_cast<size_t>(c.H) * 64 + i % 64] |= f;
+    }
+    if (threadIdx.x == 0)
+      for (uint32_t s = 0; s < c.nslots; ++s)
+        for (uint32_t p = 0; p < kParts; ++p) {
+          HopCounters *pp = part_ctr(c.slots[s], c.H, p);
+          add_ctr(c.slots[s].ctr[c.H], *pp);
+          *pp = HopCounters{};
+        }
+    __syncthreads();
+  }
   const uint32_t rows_per_tile = c.TW * 64;
   const uint32_t total = c.nslots * rows_per_tile;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total;
@@ -598,7 +759,9 @@ static void enqueue_hop_tw(const BatchConst &c, bool shortest, uint32_t hop,
 template <int TW>
 static void enqueue_final_tw(const BatchConst &c, uint32_t grid, hipStream_t s) {
   hipLaunchKernelGGL((k_bw_clear<TW>), dim3(grid), dim3(kBlock), 0, s, c, c.H);
-  hipLaunchKernelGGL((k_bw_rows<TW>), dim3(grid), dim3(kBlock), 0, s, c);
+  // half the usual grid: each block accumulates two chunk groups in LDS
+  hipLaunchKernelGGL((k_bw_rows<TW>), dim3(std::max<uint32_t>(1, grid / 2)), dim3(kBlock), 0, s,
+                     c);
 }
 
 // Enqueues the whole launch sequence of one wave; returns the number of
@@ -810,7 +973,8 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   const size_t items_cap = static_cast<size_t>(V) + push->E / kPushChunk + 64;
   const size_t per_slot = mat * (shortest ? 4 : 3) + (use_r ? mat : 0) + 3 * nvb * 8 +
                           2 * static_cast<size_t>(V) * 8 +
-                          items_cap * 8 + (H + 1) * sizeof(HopCounters);
+                          items_cap * 8 + (H + 1) * sizeof(HopCounters) +
+                          (H + 1) * kParts * (kPartStride + 64 * 8);
   size_t freeb = 0, totalb = 0;
   VS_HIP(hipMemGetInfo(&freeb, &totalb));
   const size_t budget = p.mem_budget ? p.mem_budget : freeb / 10 * 7;
@@ -869,6 +1033,10 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
     sd[i].front = static_cast<uint64_t *>(q);
     VS_TRY(get((H + 1) * sizeof(HopCounters), true, &q));
     sd[i].ctr = static_cast<HopCounters *>(q);
+    VS_TRY(get((H + 1) * kParts * kPartStride, true, &q));
+    sd[i].part = static_cast<char *>(q);
+    VS_TRY(get((H + 1) * kParts * 64 * 8ull, true, &q));
+    sd[i].fpart = static_cast<uint64_t *>(q);
     sd[i].tile = UINT32_MAX;
     sd[i].mode = 0;
   }

@@ -470,6 +470,31 @@ __device__ __forceinline__ void clear_block(uint64_t *F, uint64_t *Fm, uint64_t 
   if (lane == 0) actbm[vb] = 0;
 }
 
+// As clear_block, but writes only the words each row's word mask names.
+// Every writer of a frontier row writes its mask too (emits in the pull and
+// the epilogue, the L0 initialiser), so the mask covers all nonzero words,
+// and a row that holds bits in a few words costs a few sector writes.
+template <int TW>
+__device__ __forceinline__ void clear_block_masked(uint64_t *F, uint64_t *Fm, uint64_t *actbm,
+                                                   uint32_t vb) {
+  constexpr int G = 64 / TW;
+  const int lane = lane_id();
+  const int g = lane / TW, j = lane % TW;
+  uint64_t m = actbm[vb];
+  if (!m) return;
+  const size_t v0 = static_cast<size_t>(vb) * 64;
+  // lane i reads (and zeroes) the mask of vertex v0 + i if it is active
+  const uint64_t mine = ((m >> lane) & 1) ? Fm[v0 + lane] : 0;
+  if (mine) Fm[v0 + lane] = 0;
+  while (m) {
+    const int i = nth_set_bit(m, g);
+    m = drop_low_bits(m, G);
+    const uint64_t wm = shfl64(mine, i >= 0 ? i : 0);
+    if (i >= 0 && ((wm >> j) & 1)) F[(v0 + i) * TW + j] = 0;
+  }
+  if (lane == 0) actbm[vb] = 0;
+}
+
 // Item builder for one 256-word slice of an active bitmap: per-thread counts
 // of kPushChunk-neighbour chunks, block-level exclusive scan, one atomicAdd
 // per block.  Must be called by the whole block (uses __syncthreads).
