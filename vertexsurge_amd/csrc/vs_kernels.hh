@@ -408,37 +408,55 @@ template <int TW, bool SHORTEST>
 __device__ __forceinline__ void epi_block(const HopArgs &a, uint32_t vb,
                                           const LaneMasks &mk, LocalCtr &c) {
   constexpr int G = 64 / TW;
+  constexpr int U = 4;  // vertices per group of lanes whose loads are in flight together
   const int lane = lane_id();
   const int g = lane / TW, j = lane % TW;
   uint64_t m = a.touched[vb];
   if (!m) return;
+  // degrees of the block's touched vertices, one coalesced read (lane = vertex)
+  const uint32_t vl = vb * 64 + lane;
+  const uint64_t degl = ((m >> lane) & 1) ? a.xoff[vl + 1] - a.xoff[vl] : 0;
   uint64_t actmask = 0;
   while (m) {
-    const int i = nth_set_bit(m, g);
-    m = drop_low_bits(m, G);
-    uint64_t nb = 0;
-    uint32_t v = 0;
-    if (i >= 0) {
-      v = vb * 64 + i;
-      const size_t wi = static_cast<size_t>(v) * TW + j;
-      const uint64_t n = a.N[wi];
-      if (n) a.N[wi] = 0;
-      uint64_t vis = 0;
-      if (SHORTEST) vis = a.Vis[wi];
-      nb = n & mk.live & ~vis;
-      if (nb) emit_word<SHORTEST>(a, wi, nb, vis, mk.racc, c);
-    }
-    const uint64_t gb = group_bits<TW>(__ballot(nb != 0), g);
-    if (i >= 0 && gb) {
-      const uint64_t deg = a.xoff[v + 1] - a.xoff[v];
-      if (j == 0) {
-        actmask |= 1ull << i;
-        c.act += 1;
-        c.actdeg += deg;
-        c.nzdeg += deg * __popcll(gb);
-        if (a.Fnm) a.Fnm[v] = gb;
+    int iv[U];
+    uint64_t nv[U], visv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      iv[u] = nth_set_bit(m, g + u * G);
+      nv[u] = 0;
+      visv[u] = 0;
+      if (iv[u] >= 0) {
+        const size_t wi = (static_cast<size_t>(vb) * 64 + iv[u]) * TW + j;
+        nv[u] = a.N[wi];
+        if (SHORTEST) visv[u] = a.Vis[wi];
       }
-      c.te += deg * __popcll(nb & mk.live_next);
+    }
+    m = drop_low_bits(m, U * G);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = iv[u];
+      const uint32_t v = vb * 64 + (i >= 0 ? i : 0);
+      uint64_t nb = 0;
+      if (i >= 0) {
+        const size_t wi = static_cast<size_t>(v) * TW + j;
+        const uint64_t n = nv[u];
+        if (n) a.N[wi] = 0;
+        const uint64_t vis = visv[u];
+        nb = n & mk.live & ~vis;
+        if (nb) emit_word<SHORTEST>(a, wi, nb, vis, mk.racc, c);
+      }
+      const uint64_t gb = group_bits<TW>(__ballot(nb != 0), g);
+      const uint64_t deg = shfl64(degl, i >= 0 ? i : 0);
+      if (i >= 0 && gb) {
+        if (j == 0) {
+          actmask |= 1ull << i;
+          c.act += 1;
+          c.actdeg += deg;
+          c.nzdeg += deg * __popcll(gb);
+          if (a.Fnm) a.Fnm[v] = gb;
+        }
+        c.te += deg * __popcll(nb & mk.live_next);
+      }
     }
   }
   actmask = wave_or(actmask);
