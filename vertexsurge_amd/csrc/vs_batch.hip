@@ -130,9 +130,12 @@ __device__ __forceinline__ HopArgs slot_args(const BatchConst &c, const SlotDesc
   a.F = sd.F[(hop - 1) & 1];
   a.Fm = sd.masked ? sd.Fm[(hop - 1) & 1] : nullptr;
   a.actbm = sd.bm[(hop - 1) & 1];
-  a.Fn = sd.F[hop & 1];
-  a.Fnm = sd.Fm[hop & 1];
-  a.actbm_n = sd.bm[hop & 1];
+  // the tile's last hop: its frontier is never read, so it is not written
+  // (and needs no clearing)
+  const bool last = hop >= ti.hmax;
+  a.Fn = last ? nullptr : sd.F[hop & 1];
+  a.Fnm = last ? nullptr : sd.Fm[hop & 1];
+  a.actbm_n = last ? nullptr : sd.bm[hop & 1];
   a.N = sd.N;
   a.touched = sd.touched;
   a.Vis = sd.Vis;
@@ -382,7 +385,7 @@ template <int TW, bool SHORTEST, bool SPARSE>
 __global__ __launch_bounds__(kBlock) void k_bw_expand(BatchConst c, uint32_t hop) {
   __shared__ uint32_t s_nbr[kWavesPerBlock][64];
   __shared__ uint64_t s_nm[kWavesPerBlock][SPARSE ? 64 : 1];
-  __shared__ uint32_t s_pairs[kWavesPerBlock][SPARSE ? kPairs : 1];
+  __shared__ uint16_t s_pairs[kWavesPerBlock][SPARSE ? kPairCap : 1];
   __shared__ uint64_t s_acc[kWavesPerBlock][SPARSE ? 64 : 1];
   const ull *prefix = SPARSE ? c.ctl->unit_prefix_s : c.ctl->unit_prefix;
   const int wid = wave_id();

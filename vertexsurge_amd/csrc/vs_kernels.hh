@@ -70,7 +70,9 @@ struct HopArgs {
                          // rows), or nullptr: gather every word
   uint64_t *Fn;          // L_i (all zero on entry)
   uint64_t *Fnm;         // word masks of L_i (all zero on entry), or nullptr
-  uint64_t *actbm_n;     // its bitmap (all zero on entry)
+  uint64_t *actbm_n;     // its bitmap (all zero on entry); Fn, Fnm and actbm_n are
+                         // nullptr on a hop whose frontier nothing reads (the
+                         // last): the new bits still reach Vis / R and the counters
   uint64_t *N;           // push accumulator (all zero on entry and exit)
   uint64_t *touched;     // push touched bitmap (all zero on entry and exit)
   uint64_t *Vis;         // visited (Shortest) or nullptr (Any)
@@ -140,7 +142,7 @@ __device__ __forceinline__ LaneMasks lane_masks(const HopArgs &a, int j) {
 template <bool SHORTEST>
 __device__ __forceinline__ void emit_word(const HopArgs &a, size_t wi, uint64_t nb,
                                           uint64_t vis, uint64_t racc, LocalCtr &c) {
-  a.Fn[wi] = nb;
+  if (a.Fn) a.Fn[wi] = nb;
   if (SHORTEST) {
     a.Vis[wi] = vis | nb;
     c.visnew += (vis == 0);
@@ -233,13 +235,17 @@ __device__ __forceinline__ void gather_range(const HopArgs &a, const uint32_t *n
 // bits and is still needed are laid out flat over the wave's lanes, so every
 // load instruction carries up to 64 useful words instead of one per lane
 // per neighbour; the words are ORed into the wave's accumulator accw[TW] in
-// LDS.  Pairs go through LDS in rounds of kPairs (8 loads in flight per lane).
+// LDS.  Each lane first writes all of its neighbour's pairs (16-bit codes
+// neighbour << 6 | word) into the wave's LDS list, which holds every pair of
+// a 64-neighbour group (64 x 64); the gathers then read the list in rounds
+// of kPairs (8 loads in flight per lane).
 constexpr uint32_t kPairs = 512;
+constexpr uint32_t kPairCap = 64 * 64;
 
 template <int TW>
 __device__ __forceinline__ void gather_pairs(const HopArgs &a, const uint32_t *nbr,
                                              const uint64_t *nm, uint32_t n, uint32_t nh,
-                                             uint64_t needw, uint32_t *pairs, uint64_t *accw) {
+                                             uint64_t needw, uint16_t *pairs, uint64_t *accw) {
   const int lane = lane_id();
   const uint64_t mine = lane < static_cast<int>(n) ? (nm[lane] & needw) : 0;
   const uint32_t cnt = __popcll(mine);
@@ -249,23 +255,19 @@ __device__ __forceinline__ void gather_pairs(const HopArgs &a, const uint32_t *n
     const uint32_t y = __shfl_up(inc, d);
     if (lane >= d) inc += y;
   }
-  const uint32_t off = inc - cnt;
   const uint32_t P = __shfl(inc, 63);
-  for (uint32_t base = 0; base < P; base += kPairs) {
-    // this lane's pairs with index in [base, base + kPairs)
+  {
+    // every lane writes its pairs at once: as many steps as the longest list
     uint64_t m = mine;
-    uint32_t p = off;
-    while (m && p < base) {
-      m &= m - 1;
-      ++p;
-    }
-    while (m && p < base + kPairs) {
+    uint32_t p = inc - cnt;
+    while (m) {
       const uint32_t jj = __ffsll(static_cast<long long>(m)) - 1;
       m &= m - 1;
-      pairs[p - base] = (static_cast<uint32_t>(lane) << 8) | jj;
-      ++p;
+      pairs[p++] = static_cast<uint16_t>((lane << 6) | jj);
     }
-    wave_sync();
+  }
+  wave_sync();
+  for (uint32_t base = 0; base < P; base += kPairs) {
     const uint32_t np = min(kPairs, P - base);
     uint64_t x[kPairs / 64];
 #pragma unroll
@@ -273,9 +275,9 @@ __device__ __forceinline__ void gather_pairs(const HopArgs &a, const uint32_t *n
       const uint32_t q = r * 64 + lane;
       x[r] = 0;
       if (q < np) {
-        const uint32_t pr = pairs[q];
-        const uint32_t k = pr >> 8;
-        const uint64_t *src = a.F + static_cast<size_t>(nbr[k]) * TW + (pr & 255);
+        const uint32_t pr = pairs[base + q];
+        const uint32_t k = pr >> 6;
+        const uint64_t *src = a.F + static_cast<size_t>(nbr[k]) * TW + (pr & 63);
         x[r] = k < nh ? *src : __builtin_nontemporal_load(src);
       }
     }
@@ -283,7 +285,8 @@ __device__ __forceinline__ void gather_pairs(const HopArgs &a, const uint32_t *n
 #pragma unroll
     for (int r = 0; r < static_cast<int>(kPairs / 64); ++r)
       if (x[r])
-        atomicOr(reinterpret_cast<ull *>(&accw[pairs[r * 64 + lane] & 255]), static_cast<ull>(x[r]));
+        atomicOr(reinterpret_cast<ull *>(&accw[pairs[base + r * 64 + lane] & 63]),
+                 static_cast<ull>(x[r]));
     wave_sync();
   }
 }
@@ -292,7 +295,7 @@ __device__ __forceinline__ void gather_pairs(const HopArgs &a, const uint32_t *n
 // otherwise every lane gathers its word of every active neighbour.
 template <int TW, bool SHORTEST, bool SPARSE>
 __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, uint32_t *nbr,
-                                          uint64_t *nm, uint32_t *pairs, uint64_t *accw,
+                                          uint64_t *nm, uint16_t *pairs, uint64_t *accw,
                                           const LaneMasks &mk, LocalCtr &c) {
   constexpr int G = 64 / TW;
   const int lane = lane_id();
@@ -364,7 +367,7 @@ __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, u
     if (lane == 0) {
       c.act += 1;
       c.actdeg += deg;
-      atomicOr(reinterpret_cast<ull *>(&a.actbm_n[v >> 6]), 1ull << (v & 63));
+      if (a.actbm_n) atomicOr(reinterpret_cast<ull *>(&a.actbm_n[v >> 6]), 1ull << (v & 63));
       if (a.Fnm) a.Fnm[v] = wmask;
       c.nzdeg += deg * __popcll(wmask);
     }
@@ -462,7 +465,8 @@ __device__ __forceinline__ void epi_block(const HopArgs &a, uint32_t vb,
   actmask = wave_or(actmask);
   if (lane == 0) {
     // pull items of light vertices set bits of this word too
-    if (actmask) atomicOr(reinterpret_cast<ull *>(&a.actbm_n[vb]), static_cast<ull>(actmask));
+    if (actmask && a.actbm_n)
+      atomicOr(reinterpret_cast<ull *>(&a.actbm_n[vb]), static_cast<ull>(actmask));
     a.touched[vb] = 0;
   }
 }
@@ -567,9 +571,9 @@ __device__ __forceinline__ void items_slice(const uint64_t *actbm, uint32_t nvb,
 template <int TW, bool SHORTEST, bool SPARSE>
 __global__ __launch_bounds__(kBlock) void k_pull(HopArgs a) {
   __shared__ uint32_t s_nbr[kWavesPerBlock][64];
-  __shared__ uint64_t s_nm[kWavesPerBlock][64];
-  __shared__ uint32_t s_pairs[kWavesPerBlock][kPairs];
-  __shared__ uint64_t s_acc[kWavesPerBlock][64];
+  __shared__ uint64_t s_nm[kWavesPerBlock][SPARSE ? 64 : 1];
+  __shared__ uint16_t s_pairs[kWavesPerBlock][SPARSE ? kPairCap : 1];
+  __shared__ uint64_t s_acc[kWavesPerBlock][SPARSE ? 64 : 1];
   const int wid = wave_id();
   const LaneMasks mk = lane_masks(a, lane_id() % TW);
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
