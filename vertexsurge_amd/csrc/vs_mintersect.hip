@@ -43,7 +43,13 @@ struct MiArgs {
   uint32_t *counts;
   const uint64_t *offsets;
   uint32_t *lists;
+  // tuples split by the length of their shortest list: [0] short (at most
+  // kShortLen entries, four tuples per wave), [1] the rest (one per wave)
+  const uint32_t *order[2];
+  const ull *norder;  // device counts of the two lists
 };
+
+constexpr uint32_t kShortLen = 16;
 
 // first index in [lo, hi) with a[idx] > x
 __device__ __forceinline__ uint64_t upper_bound_g(const uint32_t *__restrict__ a,
@@ -67,103 +73,133 @@ __device__ __forceinline__ uint64_t lower_bound_g(const uint32_t *__restrict__ a
   return lo;
 }
 
+// A tuple's lists: start and length of each, and the shortest (the driver).
+// Returns false if an id is outside its dependency (empty intersection).
+template <int N>
+__device__ __forceinline__ bool tuple_lists(const MiArgs &a, uint64_t t, uint64_t (&b)[N],
+                                            uint32_t (&n)[N], int &drv, uint64_t &best) {
+  bool ok = true;
+  drv = 0;
+  best = ~0ull;
+#pragma unroll
+  for (int d = 0; d < N; ++d) {
+    b[d] = 0;
+    n[d] = 0;
+    const uint32_t x = a.tuples[t * N + d];
+    if (x >= a.V[d]) {
+      ok = false;
+      continue;
+    }
+    b[d] = a.off[d][x];
+    const uint64_t len = a.off[d][x + 1] - b[d];
+    n[d] = static_cast<uint32_t>(len);
+    if (len < best) {
+      best = len;
+      drv = d;
+    }
+  }
+  return ok;
+}
+
+// Per lane: is x in every list but the driver's?  lo[d] starts at from[d]
+// (positions relative to each list's start; everything before it is < x)
+// and ends at x's lower bound.  Two phases, all lists in lockstep: the
+// fence (every 64th adjacency entry), then at most 64 entries between two
+// fence posts.
+template <int N>
+__device__ __forceinline__ bool member_all(const MiArgs &a, const uint64_t (&b)[N],
+                                           const uint32_t (&n)[N], int drv,
+                                           const uint32_t (&from)[N], uint32_t x, bool have,
+                                           uint32_t (&lo)[N]) {
+  // fence posts of list d sit at the multiples of 64 in [b + lo, b + hi),
+  // fence indices fa + [0, fhi)
+  uint32_t hi[N], flo[N], fhi[N];
+  uint64_t fa[N];
+#pragma unroll
+  for (int d = 0; d < N; ++d) {
+    const bool on = have && d != drv;
+    lo[d] = on ? from[d] : 0;
+    hi[d] = on ? n[d] : 0;
+    fa[d] = (b[d] + lo[d] + 63) >> 6;
+    flo[d] = 0;
+    fhi[d] = on ? static_cast<uint32_t>(((b[d] + hi[d] + 63) >> 6) - fa[d]) : 0;
+  }
+  // phase 1: first fence post >= x
+  bool more = have;
+  while (more) {
+    more = false;
+#pragma unroll
+    for (int d = 0; d < N; ++d) {
+      if (flo[d] < fhi[d]) {
+        const uint32_t mid = (flo[d] + fhi[d]) >> 1;
+        if (a.fence[d][fa[d] + mid] < x) flo[d] = mid + 1;
+        else fhi[d] = mid;
+        more |= flo[d] < fhi[d];
+      }
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < N; ++d) {
+    const bool on = have && d != drv;
+    const uint32_t fnr = on ? static_cast<uint32_t>(((b[d] + hi[d] + 63) >> 6) - fa[d]) : 0;
+    if (on && fnr > 0) {
+      const uint32_t f = flo[d];
+      // post f-1 is < x: the answer lies after it
+      if (f > 0) lo[d] = max(lo[d], static_cast<uint32_t>(64 * (fa[d] + f - 1) + 1 - b[d]));
+      // post f is >= x: the answer is at or before it
+      if (f < fnr) hi[d] = static_cast<uint32_t>(64 * (fa[d] + f) - b[d]);
+    }
+  }
+  // phase 2: lower_bound within at most 64 entries
+  more = have;
+  while (more) {
+    more = false;
+#pragma unroll
+    for (int d = 0; d < N; ++d) {
+      if (lo[d] < hi[d]) {
+        const uint32_t mid = (lo[d] + hi[d]) >> 1;
+        if (a.adj[d][b[d] + mid] < x) lo[d] = mid + 1;
+        else hi[d] = mid;
+        more |= lo[d] < hi[d];
+      }
+    }
+  }
+  bool keep = have;
+#pragma unroll
+  for (int d = 0; d < N; ++d) {
+    if (d == drv) continue;
+    keep = keep && lo[d] < n[d] && a.adj[d][b[d] + lo[d]] == x;
+  }
+  return keep;
+}
+
+// Tuples with a long shortest list: one wave per tuple, 64 driver elements
+// per step; each step's searches start where lane 63's landed.
 template <bool FILL, int N>
 __global__ __launch_bounds__(kBlock) void k_mintersect(MiArgs a) {
   const int lane = lane_id(), wid = wave_id();
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
-  for (uint64_t t = blockIdx.x * kWavesPerBlock + wid; t < a.T; t += nwaves) {
+  const uint64_t cnt = a.norder[1];
+  for (uint64_t i = blockIdx.x * kWavesPerBlock + wid; i < cnt; i += nwaves) {
+    const uint64_t t = a.order[1][i];
     uint64_t b[N];
     uint32_t n[N];
-    bool empty = false;
-    int drv = 0;
-    uint64_t best = ~0ull;
-#pragma unroll
-    for (int d = 0; d < N; ++d) {
-      b[d] = 0;
-      n[d] = 0;
-      const uint32_t x = a.tuples[t * N + d];
-      if (x >= a.V[d]) {
-        empty = true;
-        continue;
-      }
-      b[d] = a.off[d][x];
-      const uint64_t len = a.off[d][x + 1] - b[d];
-      n[d] = static_cast<uint32_t>(len);
-      if (len < best) {
-        best = len;
-        drv = d;
-      }
-    }
+    int drv;
+    uint64_t best;
+    const bool ok = tuple_lists<N>(a, t, b, n, drv, best);
     uint32_t total = 0;
     uint64_t out = FILL ? a.offsets[t] : 0;
-    if (!empty && best > 0) {
+    if (ok && best > 0) {
       const uint32_t *dl = a.adj[drv];
-      uint32_t from[N];  // where this step's searches start, per list
+      uint32_t from[N];
 #pragma unroll
       for (int d = 0; d < N; ++d) from[d] = 0;
       for (uint64_t base = b[drv]; base < b[drv] + best; base += 64) {
         const uint64_t idx = base + lane;
         const bool have = idx < b[drv] + best;
         const uint32_t x = have ? dl[idx] : 0xFFFFFFFFu;
-        // positions relative to each list's start; fence posts of list d sit
-        // at the multiples of 64 in [b + lo, b + hi), indices fa + [0, fn)
-        uint32_t lo[N], hi[N], flo[N], fhi[N];
-        uint64_t fa[N];
-#pragma unroll
-        for (int d = 0; d < N; ++d) {
-          const bool on = have && d != drv;
-          lo[d] = on ? from[d] : 0;
-          hi[d] = on ? n[d] : 0;
-          fa[d] = (b[d] + lo[d] + 63) >> 6;
-          flo[d] = 0;
-          fhi[d] = on ? static_cast<uint32_t>(((b[d] + hi[d] + 63) >> 6) - fa[d]) : 0;
-        }
-        // phase 1, all lists in lockstep: first fence post >= x
-        bool more = have;
-        while (more) {
-          more = false;
-#pragma unroll
-          for (int d = 0; d < N; ++d) {
-            if (flo[d] < fhi[d]) {
-              const uint32_t mid = (flo[d] + fhi[d]) >> 1;
-              if (a.fence[d][fa[d] + mid] < x) flo[d] = mid + 1;
-              else fhi[d] = mid;
-              more |= flo[d] < fhi[d];
-            }
-          }
-        }
-#pragma unroll
-        for (int d = 0; d < N; ++d) {
-          const bool on = have && d != drv;
-          const uint32_t fnr = on ? static_cast<uint32_t>(((b[d] + hi[d] + 63) >> 6) - fa[d]) : 0;
-          if (on && fnr > 0) {
-            const uint32_t f = flo[d];
-            // post f-1 is < x: the answer lies after it
-            if (f > 0) lo[d] = max(lo[d], static_cast<uint32_t>(64 * (fa[d] + f - 1) + 1 - b[d]));
-            // post f is >= x: the answer is at or before it
-            if (f < fnr) hi[d] = static_cast<uint32_t>(64 * (fa[d] + f) - b[d]);
-          }
-        }
-        // phase 2, lockstep: lower_bound within at most 64 entries
-        more = have;
-        while (more) {
-          more = false;
-#pragma unroll
-          for (int d = 0; d < N; ++d) {
-            if (lo[d] < hi[d]) {
-              const uint32_t mid = (lo[d] + hi[d]) >> 1;
-              if (a.adj[d][b[d] + mid] < x) lo[d] = mid + 1;
-              else hi[d] = mid;
-              more |= lo[d] < hi[d];
-            }
-          }
-        }
-        bool keep = have;
-#pragma unroll
-        for (int d = 0; d < N; ++d) {
-          if (d == drv) continue;
-          keep = keep && lo[d] < n[d] && a.adj[d][b[d] + lo[d]] == x;
-        }
+        uint32_t lo[N];
+        const bool keep = member_all<N>(a, b, n, drv, from, x, have, lo);
         // the next step's elements are all larger than lane 63's
 #pragma unroll
         for (int d = 0; d < N; ++d) from[d] = __shfl(lo[d], 63);
@@ -177,17 +213,113 @@ __global__ __launch_bounds__(kBlock) void k_mintersect(MiArgs a) {
   }
 }
 
+// Tuples whose shortest list has at most kShortLen entries: four per wave,
+// sixteen lanes each, one step.
+template <bool FILL, int N>
+__global__ __launch_bounds__(kBlock) void k_mintersect_short(MiArgs a) {
+  constexpr int GS = kShortLen, NG = 64 / GS;
+  const int lane = lane_id(), wid = wave_id();
+  const int g = lane / GS, gl = lane % GS;
+  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
+  const uint64_t cnt = a.norder[0];
+  for (uint64_t w = blockIdx.x * kWavesPerBlock + wid; w * NG < cnt; w += nwaves) {
+    const uint64_t i = w * NG + g;
+    const bool mine = i < cnt;
+    const uint64_t t = mine ? a.order[0][i] : 0;
+    uint64_t b[N] = {};
+    uint32_t n[N] = {};
+    int drv = 0;
+    uint64_t best = 0;
+    const bool ok = mine && tuple_lists<N>(a, t, b, n, drv, best);
+    const bool have = ok && static_cast<uint64_t>(gl) < best;
+    const uint32_t x = have ? a.adj[drv][b[drv] + gl] : 0xFFFFFFFFu;
+    uint32_t from[N], lo[N];
+#pragma unroll
+    for (int d = 0; d < N; ++d) from[d] = 0;
+    const bool keep = member_all<N>(a, b, n, drv, from, x, have, lo);
+    const uint64_t m = __ballot(keep);
+    const uint64_t gm = (m >> (g * GS)) & ((1ull << GS) - 1);
+    if (FILL && keep)
+      a.lists[a.offsets[t] + __popcll(gm & ((1ull << gl) - 1))] = x;
+    if (!FILL && mine && gl == 0) a.counts[t] = static_cast<uint32_t>(__popcll(gm));
+  }
+}
+
+// Splits the tuples by the length of their shortest list (one thread per
+// tuple; positions taken with one atomic per wave and list).
+template <int N>
+__global__ __launch_bounds__(kBlock) void k_mi_split(MiArgs a, uint32_t *shorts,
+                                                     uint32_t *longs, ull *norder) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  const uint64_t lim = (a.T + 63) / 64 * 64;  // whole waves stay in the loop
+  for (uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; t < lim;
+       t += stride) {
+    bool isshort = false, valid = t < a.T;
+    if (valid) {
+      uint64_t b[N];
+      uint32_t n[N];
+      int drv;
+      uint64_t best;
+      const bool ok = tuple_lists<N>(a, t, b, n, drv, best);
+      isshort = !ok || best <= kShortLen;
+    }
+    const uint64_t ms = __ballot(valid && isshort), ml = __ballot(valid && !isshort);
+    ull bs = 0, bl = 0;
+    if (lane_id() == 0) {
+      if (ms) bs = atomicAdd(&norder[0], static_cast<ull>(__popcll(ms)));
+      if (ml) bl = atomicAdd(&norder[1], static_cast<ull>(__popcll(ml)));
+    }
+    bs = shfl64(bs, 0);
+    bl = shfl64(bl, 0);
+    if (valid && isshort) shorts[bs + lane_rank(ms)] = static_cast<uint32_t>(t);
+    if (valid && !isshort) longs[bl + lane_rank(ml)] = static_cast<uint32_t>(t);
+  }
+}
+
+template <int N>
+static void launch_n(bool fill, uint32_t grid, hipStream_t s, const MiArgs &a) {
+  if (fill) {
+    hipLaunchKernelGGL((k_mintersect<true, N>), dim3(grid), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_mintersect_short<true, N>), dim3(grid), dim3(kBlock), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((k_mintersect<false, N>), dim3(grid), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_mintersect_short<false, N>), dim3(grid), dim3(kBlock), 0, s, a);
+  }
+}
+
+template <int N>
+static void launch_split(uint32_t grid, hipStream_t s, const MiArgs &a, uint32_t *shorts,
+                         uint32_t *longs, ull *norder) {
+  hipLaunchKernelGGL((k_mi_split<N>), dim3(grid), dim3(kBlock), 0, s, a, shorts, longs, norder);
+}
+
+// arity as a template parameter keeps each variant's per-list state in
+// registers
 template <bool FILL>
 static void launch_mintersect(uint32_t arity, uint32_t grid, hipStream_t s, const MiArgs &a) {
   switch (arity) {
-    case 1: hipLaunchKernelGGL((k_mintersect<FILL, 1>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((k_mintersect<FILL, 2>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    case 3: hipLaunchKernelGGL((k_mintersect<FILL, 3>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    case 4: hipLaunchKernelGGL((k_mintersect<FILL, 4>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    case 5: hipLaunchKernelGGL((k_mintersect<FILL, 5>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    case 6: hipLaunchKernelGGL((k_mintersect<FILL, 6>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    case 7: hipLaunchKernelGGL((k_mintersect<FILL, 7>), dim3(grid), dim3(kBlock), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_mintersect<FILL, 8>), dim3(grid), dim3(kBlock), 0, s, a); break;
+    case 1: launch_n<1>(FILL, grid, s, a); break;
+    case 2: launch_n<2>(FILL, grid, s, a); break;
+    case 3: launch_n<3>(FILL, grid, s, a); break;
+    case 4: launch_n<4>(FILL, grid, s, a); break;
+    case 5: launch_n<5>(FILL, grid, s, a); break;
+    case 6: launch_n<6>(FILL, grid, s, a); break;
+    case 7: launch_n<7>(FILL, grid, s, a); break;
+    default: launch_n<8>(FILL, grid, s, a); break;
+  }
+}
+
+static void launch_mi_split(uint32_t arity, uint32_t grid, hipStream_t s, const MiArgs &a,
+                            uint32_t *shorts, uint32_t *longs, ull *norder) {
+  switch (arity) {
+    case 1: launch_split<1>(grid, s, a, shorts, longs, norder); break;
+    case 2: launch_split<2>(grid, s, a, shorts, longs, norder); break;
+    case 3: launch_split<3>(grid, s, a, shorts, longs, norder); break;
+    case 4: launch_split<4>(grid, s, a, shorts, longs, norder); break;
+    case 5: launch_split<5>(grid, s, a, shorts, longs, norder); break;
+    case 6: launch_split<6>(grid, s, a, shorts, longs, norder); break;
+    case 7: launch_split<7>(grid, s, a, shorts, longs, norder); break;
+    default: launch_split<8>(grid, s, a, shorts, longs, norder); break;
   }
 }
 
@@ -322,8 +454,31 @@ static vs_status mintersect_impl(const vs_graph_t *deps, uint32_t arity,
   VS_HIP(hipEventCreate(&e0));
   VS_HIP(hipEventCreate(&e1));
   struct EvGuard { hipEvent_t a, b; ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); } } eg{e0, e1};
+  if (T >= 0xFFFFFFFFull) {
+    set_error("vs_mintersect: %llu tuples in one call (at most 2^32 - 2)",
+              static_cast<unsigned long long>(T));
+    return VS_ERR_INVALID;
+  }
   const uint32_t grid = grid_for(T, g->num_cus);
+  // tuple lists by driver length: T ids each, two counters
+  void *sp = nullptr, *lp = nullptr, *np = nullptr;
+  VS_TRY(g->pool.get(T * 4, false, s, &sp));
+  struct Back {
+    Pool &p; void *a, *b, *c; size_t n;
+    ~Back() { if (a) p.put(a, n, false); if (b) p.put(b, n, false); if (c) p.put(c, 16, false); }
+  } back{g->pool, sp, nullptr, nullptr, T * 4};
+  VS_TRY(g->pool.get(T * 4, false, s, &lp));
+  back.b = lp;
+  VS_TRY(g->pool.get(16, true, s, &np));
+  back.c = np;
+  VS_HIP(hipMemsetAsync(np, 0, 16, s));
+  a.order[0] = static_cast<const uint32_t *>(sp);
+  a.order[1] = static_cast<const uint32_t *>(lp);
+  a.norder = static_cast<const ull *>(np);
   VS_HIP(hipEventRecord(e0, s));
+  launch_mi_split(arity, grid_for((T + 63) / 64, g->num_cus), s, a, static_cast<uint32_t *>(sp),
+                  static_cast<uint32_t *>(lp), static_cast<ull *>(np));
+  VS_HIP(hipGetLastError());
   launch_mintersect<false>(arity, grid, s, a);
   VS_HIP(hipGetLastError());
   uint64_t tot = 0;
