@@ -43,13 +43,14 @@ struct MiArgs {
   uint32_t *counts;
   const uint64_t *offsets;
   uint32_t *lists;
-  // tuples split by the length of their shortest list: [0] short (at most
-  // kShortLen entries, four tuples per wave), [1] the rest (one per wave)
-  const uint32_t *order[2];
-  const ull *norder;  // device counts of the two lists
+  // tuples split by the length of their shortest list: at most 8, 16, 32
+  // entries (eight, four, two tuples per wave), and the rest (one per wave)
+  const uint32_t *order[4];
+  const ull *norder;  // device counts of the four lists
 };
 
-constexpr uint32_t kShortLen = 16;
+constexpr int kBuckets = 4;
+__host__ __device__ constexpr uint32_t bucket_len(int k) { return 8u << k; }  // 8, 16, 32
 
 // first index in [lo, hi) with a[idx] > x
 __device__ __forceinline__ uint64_t upper_bound_g(const uint32_t *__restrict__ a,
@@ -179,9 +180,9 @@ template <bool FILL, int N>
 __global__ __launch_bounds__(kBlock) void k_mintersect(MiArgs a) {
   const int lane = lane_id(), wid = wave_id();
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
-  const uint64_t cnt = a.norder[1];
+  const uint64_t cnt = a.norder[kBuckets - 1];
   for (uint64_t i = blockIdx.x * kWavesPerBlock + wid; i < cnt; i += nwaves) {
-    const uint64_t t = a.order[1][i];
+    const uint64_t t = a.order[kBuckets - 1][i];
     uint64_t b[N];
     uint32_t n[N];
     int drv;
@@ -213,19 +214,19 @@ __global__ __launch_bounds__(kBlock) void k_mintersect(MiArgs a) {
   }
 }
 
-// Tuples whose shortest list has at most kShortLen entries: four per wave,
-// sixteen lanes each, one step.
-template <bool FILL, int N>
+// Tuples of bucket K, whose shortest list has at most GS = bucket_len(K)
+// entries: 64 / GS tuples per wave, GS lanes each, one step.
+template <bool FILL, int N, int K>
 __global__ __launch_bounds__(kBlock) void k_mintersect_short(MiArgs a) {
-  constexpr int GS = kShortLen, NG = 64 / GS;
+  constexpr int GS = bucket_len(K), NG = 64 / GS;
   const int lane = lane_id(), wid = wave_id();
   const int g = lane / GS, gl = lane % GS;
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
-  const uint64_t cnt = a.norder[0];
+  const uint64_t cnt = a.norder[K];
   for (uint64_t w = blockIdx.x * kWavesPerBlock + wid; w * NG < cnt; w += nwaves) {
     const uint64_t i = w * NG + g;
     const bool mine = i < cnt;
-    const uint64_t t = mine ? a.order[0][i] : 0;
+    const uint64_t t = mine ? a.order[K][i] : 0;
     uint64_t b[N] = {};
     uint32_t n[N] = {};
     int drv = 0;
@@ -245,52 +246,56 @@ __global__ __launch_bounds__(kBlock) void k_mintersect_short(MiArgs a) {
   }
 }
 
-// Splits the tuples by the length of their shortest list (one thread per
-// tuple; positions taken with one atomic per wave and list).
+// Splits the tuples by the length of their shortest list into the buckets'
+// lists (one thread per tuple; positions taken with one atomic per wave and
+// bucket).  lists[k] has room for every tuple.
 template <int N>
-__global__ __launch_bounds__(kBlock) void k_mi_split(MiArgs a, uint32_t *shorts,
-                                                     uint32_t *longs, ull *norder) {
+__global__ __launch_bounds__(kBlock) void k_mi_split(MiArgs a, uint32_t *lists, ull *norder) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   const uint64_t lim = (a.T + 63) / 64 * 64;  // whole waves stay in the loop
   for (uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; t < lim;
        t += stride) {
-    bool isshort = false, valid = t < a.T;
+    const bool valid = t < a.T;
+    int k = kBuckets - 1;
     if (valid) {
       uint64_t b[N];
       uint32_t n[N];
       int drv;
       uint64_t best;
       const bool ok = tuple_lists<N>(a, t, b, n, drv, best);
-      isshort = !ok || best <= kShortLen;
+      if (!ok) best = 0;
+      for (int q = kBuckets - 2; q >= 0; --q)
+        if (best <= bucket_len(q)) k = q;
     }
-    const uint64_t ms = __ballot(valid && isshort), ml = __ballot(valid && !isshort);
-    ull bs = 0, bl = 0;
-    if (lane_id() == 0) {
-      if (ms) bs = atomicAdd(&norder[0], static_cast<ull>(__popcll(ms)));
-      if (ml) bl = atomicAdd(&norder[1], static_cast<ull>(__popcll(ml)));
+#pragma unroll
+    for (int q = 0; q < kBuckets; ++q) {
+      const uint64_t m = __ballot(valid && k == q);
+      ull base = 0;
+      if (lane_id() == 0 && m) base = atomicAdd(&norder[q], static_cast<ull>(__popcll(m)));
+      base = shfl64(base, 0);
+      if (valid && k == q) lists[q * a.T + base + lane_rank(m)] = static_cast<uint32_t>(t);
     }
-    bs = shfl64(bs, 0);
-    bl = shfl64(bl, 0);
-    if (valid && isshort) shorts[bs + lane_rank(ms)] = static_cast<uint32_t>(t);
-    if (valid && !isshort) longs[bl + lane_rank(ml)] = static_cast<uint32_t>(t);
   }
+}
+
+template <bool FILL, int N>
+static void launch_fn(uint32_t grid, hipStream_t s, const MiArgs &a) {
+  hipLaunchKernelGGL((k_mintersect<FILL, N>), dim3(grid), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((k_mintersect_short<FILL, N, 0>), dim3(grid), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((k_mintersect_short<FILL, N, 1>), dim3(grid), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((k_mintersect_short<FILL, N, 2>), dim3(grid), dim3(kBlock), 0, s, a);
 }
 
 template <int N>
 static void launch_n(bool fill, uint32_t grid, hipStream_t s, const MiArgs &a) {
-  if (fill) {
-    hipLaunchKernelGGL((k_mintersect<true, N>), dim3(grid), dim3(kBlock), 0, s, a);
-    hipLaunchKernelGGL((k_mintersect_short<true, N>), dim3(grid), dim3(kBlock), 0, s, a);
-  } else {
-    hipLaunchKernelGGL((k_mintersect<false, N>), dim3(grid), dim3(kBlock), 0, s, a);
-    hipLaunchKernelGGL((k_mintersect_short<false, N>), dim3(grid), dim3(kBlock), 0, s, a);
-  }
+  if (fill) launch_fn<true, N>(grid, s, a);
+  else launch_fn<false, N>(grid, s, a);
 }
 
 template <int N>
-static void launch_split(uint32_t grid, hipStream_t s, const MiArgs &a, uint32_t *shorts,
-                         uint32_t *longs, ull *norder) {
-  hipLaunchKernelGGL((k_mi_split<N>), dim3(grid), dim3(kBlock), 0, s, a, shorts, longs, norder);
+static void launch_split(uint32_t grid, hipStream_t s, const MiArgs &a, uint32_t *lists,
+                         ull *norder) {
+  hipLaunchKernelGGL((k_mi_split<N>), dim3(grid), dim3(kBlock), 0, s, a, lists, norder);
 }
 
 // arity as a template parameter keeps each variant's per-list state in
@@ -310,16 +315,16 @@ static void launch_mintersect(uint32_t arity, uint32_t grid, hipStream_t s, cons
 }
 
 static void launch_mi_split(uint32_t arity, uint32_t grid, hipStream_t s, const MiArgs &a,
-                            uint32_t *shorts, uint32_t *longs, ull *norder) {
+                            uint32_t *lists, ull *norder) {
   switch (arity) {
-    case 1: launch_split<1>(grid, s, a, shorts, longs, norder); break;
-    case 2: launch_split<2>(grid, s, a, shorts, longs, norder); break;
-    case 3: launch_split<3>(grid, s, a, shorts, longs, norder); break;
-    case 4: launch_split<4>(grid, s, a, shorts, longs, norder); break;
-    case 5: launch_split<5>(grid, s, a, shorts, longs, norder); break;
-    case 6: launch_split<6>(grid, s, a, shorts, longs, norder); break;
-    case 7: launch_split<7>(grid, s, a, shorts, longs, norder); break;
-    default: launch_split<8>(grid, s, a, shorts, longs, norder); break;
+    case 1: launch_split<1>(grid, s, a, lists, norder); break;
+    case 2: launch_split<2>(grid, s, a, lists, norder); break;
+    case 3: launch_split<3>(grid, s, a, lists, norder); break;
+    case 4: launch_split<4>(grid, s, a, lists, norder); break;
+    case 5: launch_split<5>(grid, s, a, lists, norder); break;
+    case 6: launch_split<6>(grid, s, a, lists, norder); break;
+    case 7: launch_split<7>(grid, s, a, lists, norder); break;
+    default: launch_split<8>(grid, s, a, lists, norder); break;
   }
 }
 
@@ -460,24 +465,22 @@ static vs_status mintersect_impl(const vs_graph_t *deps, uint32_t arity,
     return VS_ERR_INVALID;
   }
   const uint32_t grid = grid_for(T, g->num_cus);
-  // tuple lists by driver length: T ids each, two counters
-  void *sp = nullptr, *lp = nullptr, *np = nullptr;
-  VS_TRY(g->pool.get(T * 4, false, s, &sp));
+  // tuple lists by driver length: room for T ids in each bucket, counters
+  void *lp = nullptr, *np = nullptr;
+  const size_t lbytes = kBuckets * T * 4, nbytes = kBuckets * 8;
+  VS_TRY(g->pool.get(lbytes, false, s, &lp));
   struct Back {
-    Pool &p; void *a, *b, *c; size_t n;
-    ~Back() { if (a) p.put(a, n, false); if (b) p.put(b, n, false); if (c) p.put(c, 16, false); }
-  } back{g->pool, sp, nullptr, nullptr, T * 4};
-  VS_TRY(g->pool.get(T * 4, false, s, &lp));
-  back.b = lp;
-  VS_TRY(g->pool.get(16, true, s, &np));
-  back.c = np;
-  VS_HIP(hipMemsetAsync(np, 0, 16, s));
-  a.order[0] = static_cast<const uint32_t *>(sp);
-  a.order[1] = static_cast<const uint32_t *>(lp);
+    Pool &p; void *l, *n; size_t lb, nb;
+    ~Back() { if (l) p.put(l, lb, false); if (n) p.put(n, nb, false); }
+  } back{g->pool, lp, nullptr, lbytes, nbytes};
+  VS_TRY(g->pool.get(nbytes, false, s, &np));
+  back.n = np;
+  VS_HIP(hipMemsetAsync(np, 0, nbytes, s));
+  for (int k = 0; k < kBuckets; ++k) a.order[k] = static_cast<const uint32_t *>(lp) + k * T;
   a.norder = static_cast<const ull *>(np);
   VS_HIP(hipEventRecord(e0, s));
-  launch_mi_split(arity, grid_for((T + 63) / 64, g->num_cus), s, a, static_cast<uint32_t *>(sp),
-                  static_cast<uint32_t *>(lp), static_cast<ull *>(np));
+  launch_mi_split(arity, grid_for((T + 63) / 64, g->num_cus), s, a, static_cast<uint32_t *>(lp),
+                  static_cast<ull *>(np));
   VS_HIP(hipGetLastError());
   launch_mintersect<false>(arity, grid, s, a);
   VS_HIP(hipGetLastError());
