@@ -336,16 +336,28 @@ __global__ void k_fence(const uint32_t *__restrict__ adj, uint64_t E, uint32_t *
     fence[f] = adj[f * 64];
 }
 
-// *bad = 1 if some list is not strictly ascending; one wave per vertex
-__global__ __launch_bounds__(kBlock) void k_check_sets(const uint64_t *__restrict__ off,
-                                                       const uint32_t *__restrict__ adj,
-                                                       uint32_t V, int *bad) {
-  const int lane = lane_id(), wid = wave_id();
-  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
+// starts: bit i set if entry i begins a list (of a vertex u >= 1; entry 0
+// needs no bit), one thread per vertex
+__global__ __launch_bounds__(kBlock) void k_list_starts(const uint64_t *__restrict__ off,
+                                                        uint32_t V, uint64_t E,
+                                                        uint64_t *starts) {
+  for (uint64_t u = 1 + blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; u < V;
+       u += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t i = off[u];
+    if (i < E) atomicOr(reinterpret_cast<ull *>(&starts[i >> 6]), 1ull << (i & 63));
+  }
+}
+
+// *bad = 1 if two neighbouring entries of one list are not strictly
+// ascending; one thread per entry (hub lists spread over the whole grid)
+__global__ __launch_bounds__(kBlock) void k_check_sets(const uint32_t *__restrict__ adj,
+                                                       uint64_t E,
+                                                       const uint64_t *__restrict__ starts,
+                                                       int *bad) {
   bool ok = true;
-  for (uint64_t u = blockIdx.x * kWavesPerBlock + wid; u < V; u += nwaves) {
-    const uint64_t e = off[u + 1];
-    for (uint64_t i = off[u] + lane; i + 1 < e; i += 64) ok = ok && adj[i] < adj[i + 1];
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i + 1 < E;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    if (adj[i] >= adj[i + 1] && !((starts[(i + 1) >> 6] >> ((i + 1) & 63)) & 1)) ok = false;
   }
   if (!ok) *bad = 1;
 }
@@ -362,18 +374,25 @@ static vs_status ensure_fence(vs_graph_s *g, DevCSR &c) {
     VS_HIP(hipMalloc(&p, std::max<uint64_t>(nf, 1) * 4));
     c.fence = static_cast<uint32_t *>(p);
     VS_HIP(hipMemsetAsync(flag, 0, sizeof(int), g->stream));
+    void *starts = nullptr;
+    const size_t sbytes = ((c.E + 63) / 64) * 8;
     if (nf) {
       hipLaunchKernelGGL(k_fence, dim3(grid_for((nf + 63) / 64, g->num_cus)), dim3(kBlock), 0,
                          g->stream, c.adj, c.E, c.fence);
       VS_HIP(hipGetLastError());
-      hipLaunchKernelGGL(k_check_sets, dim3(grid_for(g->V, g->num_cus)), dim3(kBlock), 0,
-                         g->stream, c.off, c.adj, g->V, static_cast<int *>(flag));
+      VS_TRY(g->pool.get(sbytes, true, g->stream, &starts));
+      hipLaunchKernelGGL(k_list_starts, dim3(grid_for((g->V + 63) / 64, g->num_cus)), dim3(kBlock),
+                         0, g->stream, c.off, g->V, c.E, static_cast<uint64_t *>(starts));
+      hipLaunchKernelGGL(k_check_sets, dim3(grid_for((c.E + 63) / 64, g->num_cus)), dim3(kBlock),
+                         0, g->stream, c.adj, c.E, static_cast<const uint64_t *>(starts),
+                         static_cast<int *>(flag));
       VS_HIP(hipGetLastError());
     }
     int bad = 0;
     VS_HIP(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, g->stream));
     // the intersection may run on another graph's stream
     VS_HIP(hipStreamSynchronize(g->stream));
+    if (starts) g->pool.put(starts, sbytes, false);
     c.sets = bad ? 0 : 1;
   }
   if (!c.sets) {
