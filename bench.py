@@ -71,7 +71,8 @@ def parse():
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="first run a plain copy kernel over this many MiB (counter scale)")
     ap.add_argument("--profile-only", action="store_true",
-                    help="graph + timed legs only (for rocprofv3 runs)")
+                    help="graph + timed legs only (for rocprofv3 runs); with --workload batch, "
+                         "the batched leg alone")
     return ap.parse_args()
 
 
@@ -342,6 +343,13 @@ def run_batch_workload(vs, np, csr, G, args):
     # one untimed call builds the graph's pull work items and hub lists
     res, _ = vs.vexpand(G, qs[0].sources, qs[0].k_min, qs[0].k_max, qs[0].path)
     res.close()
+    if args.profile_only:
+        sched = load_scheduler(vs, G, args.weights)
+        _, _, st = vs.vexpand_batch(G, qs, sched="learned" if sched is not None else "heuristic",
+                                    replays=1, want_digests=False)
+        log(f"batched {st['ms_total'] / 1e3:.2f}s device, "
+            f"{st['te'] / (st['ms_total'] / 1e3) / 1e9:.1f} GTEPS, {st['num_waves']} waves")
+        return 0
     # plain: every query alone, one after another; device time (HIP events of
     # each call) is what the batched leg is compared on, wall time is logged
     t = time.perf_counter()
