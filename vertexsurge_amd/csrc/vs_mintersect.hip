@@ -52,28 +52,6 @@ struct MiArgs {
 constexpr int kBuckets = 4;
 __host__ __device__ constexpr uint32_t bucket_len(int k) { return 8u << k; }  // 8, 16, 32
 
-// first index in [lo, hi) with a[idx] > x
-__device__ __forceinline__ uint64_t upper_bound_g(const uint32_t *__restrict__ a,
-                                                  uint64_t lo, uint64_t hi, uint32_t x) {
-  while (lo < hi) {
-    const uint64_t mid = (lo + hi) >> 1;
-    if (a[mid] <= x) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
-// first index in [lo, hi) with a[idx] >= x
-__device__ __forceinline__ uint64_t lower_bound_g(const uint32_t *__restrict__ a,
-                                                  uint64_t lo, uint64_t hi, uint32_t x) {
-  while (lo < hi) {
-    const uint64_t mid = (lo + hi) >> 1;
-    if (a[mid] < x) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
 // A tuple's lists: start and length of each, and the shortest (the driver).
 // Returns false if an id is outside its dependency (empty intersection).
 template <int N>
