@@ -386,7 +386,8 @@ __global__ __launch_bounds__(kBlock) void k_bw_expand(BatchConst c, uint32_t hop
   __shared__ uint32_t s_nbr[kWavesPerBlock][64];
   __shared__ uint64_t s_nm[kWavesPerBlock][SPARSE ? 64 : 1];
   __shared__ uint16_t s_pairs[kWavesPerBlock][SPARSE ? kPairCap : 1];
-  __shared__ uint64_t s_acc[kWavesPerBlock][SPARSE ? 64 : 1];
+  __shared__ uint64_t s_acc[kWavesPerBlock][64];
+  __shared__ uint8_t s_wl[kWavesPerBlock][64];
   const ull *prefix = SPARSE ? c.ctl->unit_prefix_s : c.ctl->unit_prefix;
   const int wid = wave_id();
   const ull total = prefix[c.nslots];
@@ -419,8 +420,8 @@ __global__ __launch_bounds__(kBlock) void k_bw_expand(BatchConst c, uint32_t hop
         pre = pull_head<TW, SHORTEST>(a, a.pitems[nu - prefix[s]], j);
         pre_u = nu;
       }
-      pull_body<TW, SHORTEST, SPARSE>(a, h, s_nbr[wid], s_nm[wid], s_pairs[wid], s_acc[wid], mk,
-                                      lc);
+      pull_body<TW, SHORTEST, SPARSE>(a, h, s_nbr[wid], s_nm[wid], s_pairs[wid], s_acc[wid],
+                                      s_wl[wid], mk, lc);
     } else if (!SPARSE) {
       push_item<TW>(a, a.items[local]);
     }

@@ -291,12 +291,57 @@ __device__ __forceinline__ void gather_pairs(const HopArgs &a, const uint32_t *n
   }
 }
 
+// Dense rows, few words still needed: the pairs (neighbour k, needed word)
+// of all n active neighbours are laid flat over the lanes without reading
+// word masks (neighbour-major, so a neighbour's words are neighbouring
+// lanes), at most kCross of them: one round of loads instead of one load
+// instruction per neighbour with only the needed words' lanes active.  The
+// same words are read as by the per-neighbour gather.  wl is the wave's LDS
+// list of the needed words.
+constexpr uint32_t kCross = 256;
+
+template <int TW>
+__device__ __forceinline__ void gather_cross(const HopArgs &a, const uint32_t *nbr, uint32_t n,
+                                             uint32_t nh, uint64_t needw, uint8_t *wl,
+                                             uint64_t *accw) {
+  const int lane = lane_id();
+  if (lane < TW && ((needw >> lane) & 1))
+    wl[__popcll(needw & ((1ull << lane) - 1))] = static_cast<uint8_t>(lane);
+  wave_sync();
+  const uint32_t nw = __popcll(needw);
+  const uint32_t P = n * nw;           // <= kCross (checked by the caller)
+  const uint32_t inv = (65536u + nw - 1) / nw;  // q / nw == (q * inv) >> 16 for q < 512
+  uint64_t x[kCross / 64];
+#pragma unroll
+  for (int r = 0; r < static_cast<int>(kCross / 64); ++r) {
+    const uint32_t q = r * 64 + lane;
+    x[r] = 0;
+    if (q < P) {
+      const uint32_t k = (q * inv) >> 16;
+      const uint64_t *src = a.F + static_cast<size_t>(nbr[k]) * TW + wl[q - k * nw];
+      x[r] = k < nh ? *src : __builtin_nontemporal_load(src);
+    }
+  }
+  // the word index is recomputed rather than held in registers
+#pragma unroll
+  for (int r = 0; r < static_cast<int>(kCross / 64); ++r) {
+    const uint32_t q = r * 64 + lane;
+    if (x[r]) {
+      const uint32_t k = (q * inv) >> 16;
+      atomicOr(reinterpret_cast<ull *>(&accw[wl[q - k * nw]]), static_cast<ull>(x[r]));
+    }
+  }
+  wave_sync();
+}
+
 // SPARSE (the frontier's word masks are read, a.Fm set): pair gathers;
-// otherwise every lane gathers its word of every active neighbour.
+// otherwise every lane gathers its word of every active neighbour, or, when
+// few words are still needed, the (neighbour, needed word) cross product is
+// gathered flat over the lanes.
 template <int TW, bool SHORTEST, bool SPARSE>
 __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, uint32_t *nbr,
                                           uint64_t *nm, uint16_t *pairs, uint64_t *accw,
-                                          const LaneMasks &mk, LocalCtr &c) {
+                                          uint8_t *wl, const LaneMasks &mk, LocalCtr &c) {
   constexpr int G = 64 / TW;
   const int lane = lane_id();
   const int g = lane / TW, j = lane % TW;
@@ -313,7 +358,7 @@ __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, u
   const uint64_t hub_end = h.hubs == 0xffffffffu ? le : lb + h.hubs;
   uint64_t acc = 0;
   bool todo = need != 0;
-  if (SPARSE && lane < TW) accw[lane] = 0;
+  if (lane < TW) accw[lane] = 0;
   for (uint64_t base = b; base < e; base += 64) {
     const uint64_t idx = base + lane;
     uint32_t u = 0;
@@ -339,10 +384,19 @@ __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, u
       const uint64_t needw = __ballot(g == 0 && todo);
       if (needw) gather_pairs<TW>(a, nbr, nm, n, nh, needw, pairs, accw);
       acc = accw[j];
-    } else if (todo) {
-      uint32_t k = g;
-      gather_range<TW, true>(a, nbr, k, nh, j, need, acc);
-      gather_range<TW, false>(a, nbr, k, n, j, need, acc);
+    } else {
+      const uint64_t needw = __ballot(g == 0 && todo);
+      const uint32_t nw = __popcll(needw);
+      if (nw * 8 <= TW && n * nw <= kCross) {
+        if (needw) {
+          gather_cross<TW>(a, nbr, n, nh, needw, wl, accw);
+          acc |= accw[j];
+        }
+      } else if (todo) {
+        uint32_t k = g;
+        gather_range<TW, true>(a, nbr, k, nh, j, need, acc);
+        gather_range<TW, false>(a, nbr, k, n, j, need, acc);
+      }
     }
     wave_sync();
     acc = group_or<TW>(acc);
@@ -573,7 +627,8 @@ __global__ __launch_bounds__(kBlock) void k_pull(HopArgs a) {
   __shared__ uint32_t s_nbr[kWavesPerBlock][64];
   __shared__ uint64_t s_nm[kWavesPerBlock][SPARSE ? 64 : 1];
   __shared__ uint16_t s_pairs[kWavesPerBlock][SPARSE ? kPairCap : 1];
-  __shared__ uint64_t s_acc[kWavesPerBlock][SPARSE ? 64 : 1];
+  __shared__ uint64_t s_acc[kWavesPerBlock][64];
+  __shared__ uint8_t s_wl[kWavesPerBlock][64];
   const int wid = wave_id();
   const LaneMasks mk = lane_masks(a, lane_id() % TW);
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
@@ -587,8 +642,8 @@ __global__ __launch_bounds__(kBlock) void k_pull(HopArgs a) {
     const uint64_t nx = it + nwaves;
     PullHead hn{};
     if (nx < n) hn = pull_head<TW, SHORTEST>(a, a.pitems[nx], j);
-    pull_body<TW, SHORTEST, SPARSE>(a, h, s_nbr[wid], s_nm[wid], s_pairs[wid], s_acc[wid], mk,
-                                    c);
+    pull_body<TW, SHORTEST, SPARSE>(a, h, s_nbr[wid], s_nm[wid], s_pairs[wid], s_acc[wid],
+                                    s_wl[wid], mk, c);
     h = hn;
     it = nx;
   }
