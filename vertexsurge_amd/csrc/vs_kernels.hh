@@ -291,7 +291,7 @@ __device__ __forceinline__ void gather_pairs(const HopArgs &a, const uint32_t *n
   }
 }
 
-// Dense rows, few words still needed: the pairs (neighbour k, needed word)
+// Dense rows, few (neighbour, word) pairs still needed: the pairs (neighbour k, needed word)
 // of all n active neighbours are laid flat over the lanes without reading
 // word masks (neighbour-major, so a neighbour's words are neighbouring
 // lanes), at most kCross of them: one round of loads instead of one load
@@ -336,8 +336,8 @@ __device__ __forceinline__ void gather_cross(const HopArgs &a, const uint32_t *n
 
 // SPARSE (the frontier's word masks are read, a.Fm set): pair gathers;
 // otherwise every lane gathers its word of every active neighbour, or, when
-// few words are still needed, the (neighbour, needed word) cross product is
-// gathered flat over the lanes.
+// the (neighbour, needed word) cross product fits kCross pairs, that cross
+// product is gathered flat over the lanes.
 template <int TW, bool SHORTEST, bool SPARSE>
 __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, uint32_t *nbr,
                                           uint64_t *nm, uint16_t *pairs, uint64_t *accw,
@@ -387,7 +387,7 @@ __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, u
     } else {
       const uint64_t needw = __ballot(g == 0 && todo);
       const uint32_t nw = __popcll(needw);
-      if (nw * 8 <= TW && n * nw <= kCross) {
+      if (n * nw <= kCross) {
         if (needw) {
           gather_cross<TW>(a, nbr, n, nh, needw, wl, accw);
           acc |= accw[j];
