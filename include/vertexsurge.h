@@ -296,6 +296,17 @@ vs_status vs_result_export(vs_result_t r, uint32_t i, int32_t layout,
 /* single bit (DenseVarStackMatrix::get, bitmap.cpp:953-956) */
 vs_status vs_result_get(vs_result_t r, uint32_t i, uint32_t row, uint32_t col,
                         int *bit);
+/* The result as the flat table of VExpand's CSR strategy (merge_to_Flat,
+ * vexpand.cpp:380-446): one (start, target[, distance]) tuple per pair in
+ * the union of the result's matrices; start = the row's source vertex,
+ * target = the column's vertex.  With per-hop layers (with_distance) the
+ * distance is that of the first layer holding the pair (Shortest layers are
+ * disjoint; for Any it is the reference's dedup keeping the smallest
+ * distance).  Order: by row, then target column ascending (the reference's
+ * order depends on its threads).  Two-phase: with start == NULL only *n is
+ * set; dist may be NULL, and needs a with_distance result. */
+vs_status vs_result_flat(vs_result_t r, uint32_t *start, uint32_t *target, int32_t *dist,
+                         uint64_t cap, uint64_t *n);
 vs_status vs_result_free(vs_result_t r);
 
 /* ------------------------------------------------------- batched VExpand */

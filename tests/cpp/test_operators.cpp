@@ -40,6 +40,15 @@ int main(int argc, char **argv) {
   std::printf("\"shortest_get_0_first\": %d,\n",
               (int)tab.get(0, tab.tuples().empty() ? 0 : tab.tuples()[0].second));
 
+  // the CSR strategy: the same pairs as a flat table (merge_to_Flat)
+  VExpand vc("Person", "Person", "knows", Direction::Both, 1, 3, ExpandPathType::Shortest,
+             ExpandStrategy::CSR, "p1.id", std::nullopt);
+  vc.run(starts);
+  const FlatTable ft = vc.flat_results();
+  std::printf("\"csr_flat_tuples\": %zu,\n", ft.tuple_count());
+  std::printf("\"csr_flat_first\": [%u, %u],\n", ft.start.empty() ? 0u : ft.start[0],
+              ft.target.empty() ? 0u : ft.target[0]);
+
   VExpand va("Person", "Person", "knows", Direction::Forward, 0, 2,
              VExpandPathParameters::Any(), ExpandStrategy::Dense, "p1.id", std::nullopt);
   va.run(starts);

@@ -26,6 +26,9 @@ def test_cpp_operators_match_oracle(orc):
     assert r["shortest_tuple_count"] == int(orc.row_counts(m[0]).sum())
     assert r["shortest_row_counts"] == orc.row_counts(m[0]).tolist()
     assert r["shortest_get_0_first"] == 1
+    assert r["csr_flat_tuples"] == r["shortest_tuple_count"]
+    first = np.nonzero(np.unpackbits(m[0].view(np.uint8), axis=1, bitorder="little"))
+    assert r["csr_flat_first"] == [int(starts[first[0][0]]), int(first[1][0])]
     m, _, _, _ = orc.vexpand_rows(g.offsets, g.adj, starts, 0, 2, "any")
     assert r["any_row_counts"] == orc.row_counts(m[0]).tolist()
     lay, ka, _, _ = orc.vexpand_rows(g.offsets, g.adj, starts, 1, 3, "shortest",

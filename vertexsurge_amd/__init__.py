@@ -304,6 +304,20 @@ class VExpandResult:
         _lib.call("vs_result_get", self._h, i, row, col, C.byref(b))
         return bool(b.value)
 
+    def to_flat(self, with_distance: bool = False):
+        """(start, target[, distance]) arrays, the CSR strategy's flat table
+        (merge_to_Flat, vexpand.cpp:380-446), ordered by row then target."""
+        n = C.c_uint64()
+        _lib.call("vs_result_flat", self._h, None, None, None, 0, C.byref(n))
+        s = np.zeros(max(n.value, 1), dtype=np.uint32)
+        t = np.zeros(max(n.value, 1), dtype=np.uint32)
+        d = np.zeros(max(n.value, 1), dtype=np.int32) if with_distance else None
+        if n.value:
+            _lib.call("vs_result_flat", self._h, _ptr(s, C.c_uint32), _ptr(t, C.c_uint32),
+                      _ptr(d, C.c_int32) if with_distance else None, n.value, C.byref(n))
+        s, t = s[:n.value], t[:n.value]
+        return (s, t, d[:n.value]) if with_distance else (s, t)
+
     def col_map(self):
         """Vertex ids of the columns of a column select; None for VExpand output."""
         if not getattr(self, "_selected", False):

@@ -124,6 +124,7 @@ SIGNATURES = {
     "vs_graph_trim": (C.c_int, [vp]),
     "vs_graph_set_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
     "vs_graph_get_tuning": (C.c_int, [vp, C.POINTER(Tuning)]),
+    "vs_result_flat": (C.c_int, [vp, u32p, u32p, i32p, C.c_uint64, u64p]),
     "vs_store_open": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(vp)]),
     "vs_store_get_info": (C.c_int, [vp, C.POINTER(StoreInfo)]),
     "vs_store_csr": (C.c_int, [vp, C.POINTER(u64p), C.POINTER(u32p)]),

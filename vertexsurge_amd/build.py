@@ -22,7 +22,7 @@ LIB = PKG / "libvertexsurge.so"
 INCLUDE = ROOT / "include"
 
 HIP_SOURCES = ["vs_graph.hip", "vs_vexpand.hip", "vs_mintersect.hip", "vs_mlp.hip",
-               "vs_batch.hip", "vs_tables.hip"]
+               "vs_batch.hip", "vs_tables.hip", "vs_flat.hip"]
 HOST_SOURCES = ["vs_host.cpp", "vs_store.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -344,6 +344,15 @@ struct MultipleMatrixTable {
   std::vector<std::pair<size_t, VarRowMatrixTable>> mats;
 };
 
+// FlatTable of VExpand's CSR strategy: columns (start.id, target.id[,
+// distance]) (vexpand.cpp:103-164, 380-446), rows ordered by start row, then
+// target.
+struct FlatTable {
+  std::vector<VID> start, target;
+  std::vector<int32_t> distance;  // empty without with_distance
+  size_t tuple_count() const { return start.size(); }
+};
+
 class VExpand {
  public:
   // big_operators.hh:101-105 / vexpand.cpp:7-14
@@ -376,8 +385,8 @@ class VExpand {
     rel_ = GraphStore::instance().get_edge(src_type, edge_type, dst_type);
     if (dir == Direction::Undecide) dir = Direction::Forward;
     dir_ = dir;
-    if (strategy != ExpandStrategy::Dense && strategy != ExpandStrategy::Adaptive)
-      throw std::runtime_error("only the Dense (bit-matrix) strategy runs on the GPU");
+    // every strategy expands through the bit matrices; CSR hands its result
+    // over as a flat table (flat_results)
     if (k_max >= VS_MAX_HOPS) throw std::runtime_error("k_max too large");
   }
 
@@ -436,6 +445,22 @@ class VExpand {
     }
     return m;
   }
+  // FlatTable result (the CSR strategy's merge_to_Flat, vexpand.cpp:380-446)
+  FlatTable flat_results() const {
+    if (!res_) throw std::runtime_error("no result");
+    FlatTable t;
+    uint64_t n = 0;
+    check(vs_result_flat(res_.get(), nullptr, nullptr, nullptr, 0, &n), "vs_result_flat");
+    t.start.resize(n);
+    t.target.resize(n);
+    if (path_params_.with_distance) t.distance.resize(n);
+    if (n)
+      check(vs_result_flat(res_.get(), t.start.data(), t.target.data(),
+                           path_params_.with_distance ? t.distance.data() : nullptr, n, &n),
+            "vs_result_flat");
+    return t;
+  }
+  ExpandStrategy strategy() const { return strategy_; }
   // FindShortest: the dependency's tuples extended with "distance" (-1 = none)
   const std::vector<int32_t> &distance() const { return distance_; }
 
