@@ -37,7 +37,9 @@ typedef enum {
   VS_ERR_CAPACITY = 4,    /* caller buffer too small; size reported back */
   VS_ERR_UNSUPPORTED = 5, /* option not implemented */
   VS_ERR_NODEVICE = 6,    /* no GPU visible */
-  VS_ERR_IO = 7           /* a file could not be opened, read or written */
+  VS_ERR_IO = 7,          /* a file could not be opened, read or written */
+  VS_ERR_INTERNAL = 8     /* unexpected failure inside the library (no C++ exception
+                           * ever crosses this interface) */
 } vs_status;
 
 /* Message of the last failing call on this thread ("" if none). */

@@ -143,6 +143,32 @@ def test_corrupt_files_are_rejected(tmp_path):
     assert e.value.code == vs._lib.VS_ERR_INVALID
 
 
+def test_schema_string_escapes(tmp_path):
+    """Type names go through JSON string escapes: a \\u surrogate pair names the
+    same type as its UTF-8 text; an unpaired surrogate is a malformed schema."""
+    sch = json.loads((STORE / "knows" / "schema.json").read_text())
+    data = {n: (STORE / "knows" / n).read_bytes() for n in os.listdir(STORE / "knows")
+            if n != "schema.json"}
+    vt = "P\U0001F600"  # outside the BMP: written as a surrogate pair by json.dumps
+    sch["vertex"][0]["type"] = vt
+    sch["edge"][0]["src"] = sch["edge"][0]["dst"] = vt
+    d = tmp_path / "esc"
+    d.mkdir()
+    for n, b in data.items():
+        (d / n.replace("Person", vt)).write_bytes(b)
+    text = json.dumps(sch)  # ensure_ascii writes the pair as two \\u escapes
+    assert "\\ud83d\\ude00" in text
+    (d / "schema.json").write_text(text)
+    o, a, _ = lib_open(d)
+    eo, ea, _ = lib_open(STORE / "knows")
+    assert np.array_equal(o, eo) and np.array_equal(a, ea)
+    for bad in ("\\ud83d", "\\ud83d\\u0041", "\\ude00"):
+        (d / "schema.json").write_text(text.replace("\\ud83d\\ude00", bad, 1))
+        with pytest.raises(vs.VSError) as e:
+            vs.GraphStore(d)
+        assert e.value.code == vs._lib.VS_ERR_INVALID and "surrogate" in str(e.value), bad
+
+
 def test_unsorted_lists_are_sorted(tmp_path):
     d = tmp_path / "u"
     _copy_store(STORE / "knows", d)

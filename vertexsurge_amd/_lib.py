@@ -18,6 +18,7 @@ VS_ERR_CAPACITY = 4
 VS_ERR_UNSUPPORTED = 5
 VS_ERR_NODEVICE = 6
 VS_ERR_IO = 7
+VS_ERR_INTERNAL = 8
 
 PATH_ANY, PATH_SHORTEST, PATH_FIND_SHORTEST = 0, 1, 2
 DIR_FORWARD, DIR_BACKWARD, DIR_BOTH = 0, 1, 2

@@ -1205,7 +1205,7 @@ extern "C" {
 
 vs_status vs_vexpand_batch(vs_graph_t g, const vs_query *queries, uint32_t nq,
                            const vs_batch_params *p, uint64_t *row_counts,
-                           uint64_t *row_digests, vs_batch_stats *stats) {
+                           uint64_t *row_digests, vs_batch_stats *stats) try {
   clear_error();
   if (!g || !p || (nq > 0 && !queries)) {
     set_error("vs_vexpand_batch: null argument");
@@ -1255,13 +1255,13 @@ vs_status vs_vexpand_batch(vs_graph_t g, const vs_query *queries, uint32_t nq,
   }
   if (stats) *stats = st;
   return VS_OK;
-}
+} VS_CATCH("vs_vexpand_batch")
 
 // Candidate storage tile widths x push thresholds, timed on the caller's
 // sources (Shortest or Any, hops 1..k_max), best kept in the graph tuning.
 vs_status vs_autotune_vexpand(vs_graph_t g, const uint32_t *sources, uint32_t S,
                               uint32_t k_max, int32_t path_type, vs_tuning *best,
-                              double *best_ms) {
+                              double *best_ms) try {
   clear_error();
   if (!g || !sources || S == 0 || k_max == 0) {
     set_error("vs_autotune_vexpand: need a graph, sources and k_max >= 1");
@@ -1300,6 +1300,6 @@ vs_status vs_autotune_vexpand(vs_graph_t g, const uint32_t *sources, uint32_t S,
   if (best) *best = bt;
   if (best_ms) *best_ms = bms;
   return VS_OK;
-}
+} VS_CATCH("vs_autotune_vexpand")
 
 } // extern "C"

@@ -155,7 +155,7 @@ static void launch_flat_tw(uint32_t tw, bool fill, uint32_t grid, hipStream_t s,
 using namespace vs;
 
 extern "C" vs_status vs_result_flat(vs_result_t r, uint32_t *start, uint32_t *target,
-                                    int32_t *dist, uint64_t cap, uint64_t *n) {
+                                    int32_t *dist, uint64_t cap, uint64_t *n) try {
   clear_error();
   if (!r || !n) {
     set_error("vs_result_flat: null argument");
@@ -257,4 +257,4 @@ extern "C" vs_status vs_result_flat(vs_result_t r, uint32_t *start, uint32_t *ta
   if (dist) VS_HIP(hipMemcpyAsync(dist, dd, total * 4, hipMemcpyDeviceToHost, s));
   VS_HIP(hipStreamSynchronize(s));
   return VS_OK;
-}
+} VS_CATCH("vs_result_flat")

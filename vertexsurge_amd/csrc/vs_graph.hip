@@ -451,7 +451,7 @@ using namespace vs;
 // ---------------------------------------------------------------- C ABI
 extern "C" {
 
-vs_status vs_device_count(int *count) {
+vs_status vs_device_count(int *count) try {
   clear_error();
   if (!count) { set_error("vs_device_count: null"); return VS_ERR_INVALID; }
   int n = 0;
@@ -463,21 +463,21 @@ vs_status vs_device_count(int *count) {
   }
   *count = n;
   return VS_OK;
-}
+} VS_CATCH("vs_device_count")
 
-vs_status vs_set_device(int device) {
+vs_status vs_set_device(int device) try {
   clear_error();
   VS_HIP(hipSetDevice(device));
   return VS_OK;
-}
+} VS_CATCH("vs_set_device")
 
-vs_status vs_device_synchronize(void) {
+vs_status vs_device_synchronize(void) try {
   clear_error();
   VS_HIP(hipDeviceSynchronize());
   return VS_OK;
-}
+} VS_CATCH("vs_device_synchronize")
 
-vs_status vs_dev_alloc(size_t bytes, void **dptr) {
+vs_status vs_dev_alloc(size_t bytes, void **dptr) try {
   clear_error();
   if (!dptr) { set_error("vs_dev_alloc: null"); return VS_ERR_INVALID; }
   *dptr = nullptr;
@@ -489,27 +489,27 @@ vs_status vs_dev_alloc(size_t bytes, void **dptr) {
     return VS_ERR_NOMEM;
   }
   return VS_OK;
-}
+} VS_CATCH("vs_dev_alloc")
 
-vs_status vs_dev_free(void *dptr) {
+vs_status vs_dev_free(void *dptr) try {
   clear_error();
   if (dptr) VS_HIP(hipFree(dptr));
   return VS_OK;
-}
+} VS_CATCH("vs_dev_free")
 
-vs_status vs_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+vs_status vs_memcpy_h2d(void *dst, const void *src, size_t bytes) try {
   clear_error();
   if (bytes) VS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
   return VS_OK;
-}
+} VS_CATCH("vs_memcpy_h2d")
 
-vs_status vs_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+vs_status vs_memcpy_d2h(void *dst, const void *src, size_t bytes) try {
   clear_error();
   if (bytes) VS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
   return VS_OK;
-}
+} VS_CATCH("vs_memcpy_d2h")
 
-vs_status vs_device_copy(void *dst, const void *src, size_t bytes) {
+vs_status vs_device_copy(void *dst, const void *src, size_t bytes) try {
   clear_error();
   if ((bytes % 8) || (bytes && (!dst || !src))) {
     set_error("vs_device_copy: bytes must be a multiple of 8 and pointers non-null");
@@ -527,11 +527,11 @@ vs_status vs_device_copy(void *dst, const void *src, size_t bytes) {
   VS_HIP(hipGetLastError());
   VS_HIP(hipDeviceSynchronize());
   return VS_OK;
-}
+} VS_CATCH("vs_device_copy")
 
 vs_status vs_graph_create(const uint64_t *offsets, const uint32_t *adj,
                           uint32_t num_vertices, uint64_t num_edges,
-                          uint32_t flags, vs_graph_t *out) {
+                          uint32_t flags, vs_graph_t *out) try {
   clear_error();
   if (!out || !offsets || (num_edges > 0 && !adj) || num_vertices == 0) {
     set_error("vs_graph_create: invalid arguments");
@@ -568,19 +568,19 @@ vs_status vs_graph_create(const uint64_t *offsets, const uint32_t *adj,
   if (!g->symmetric) g->hadj.assign(adj, adj + num_edges);
   *out = g.release();
   return VS_OK;
-}
+} VS_CATCH("vs_graph_create")
 
-vs_status vs_graph_destroy(vs_graph_t g) {
+vs_status vs_graph_destroy(vs_graph_t g) try {
   clear_error();
   if (!g) return VS_OK;
   if (g->destroyed) { set_error("vs_graph_destroy: double destroy"); return VS_ERR_INVALID; }
   g->destroyed = true;
   graph_release(g);
   return VS_OK;
-}
+} VS_CATCH("vs_graph_destroy")
 
 vs_status vs_graph_info(vs_graph_t g, uint32_t *num_vertices, uint64_t *num_edges,
-                        uint64_t *device_bytes) {
+                        uint64_t *device_bytes) try {
   clear_error();
   if (!g) { set_error("vs_graph_info: null"); return VS_ERR_INVALID; }
   if (num_vertices) *num_vertices = g->V;
@@ -592,17 +592,17 @@ vs_status vs_graph_info(vs_graph_t g, uint32_t *num_vertices, uint64_t *num_edge
     *device_bytes = b;
   }
   return VS_OK;
-}
+} VS_CATCH("vs_graph_info")
 
-vs_status vs_graph_trim(vs_graph_t g) {
+vs_status vs_graph_trim(vs_graph_t g) try {
   clear_error();
   if (!g) { set_error("vs_graph_trim: null"); return VS_ERR_INVALID; }
   VS_HIP(hipStreamSynchronize(g->stream));
   g->pool.trim();
   return VS_OK;
-}
+} VS_CATCH("vs_graph_trim")
 
-vs_status vs_graph_set_tuning(vs_graph_t g, const vs_tuning *t) {
+vs_status vs_graph_set_tuning(vs_graph_t g, const vs_tuning *t) try {
   clear_error();
   if (!g || !t) { set_error("vs_graph_set_tuning: null"); return VS_ERR_INVALID; }
   if (t->tile_words > 64 || (t->tile_words & (t->tile_words - 1))) {
@@ -612,14 +612,14 @@ vs_status vs_graph_set_tuning(vs_graph_t g, const vs_tuning *t) {
   }
   g->tuning = *t;
   return VS_OK;
-}
+} VS_CATCH("vs_graph_set_tuning")
 
-vs_status vs_graph_get_tuning(vs_graph_t g, vs_tuning *t) {
+vs_status vs_graph_get_tuning(vs_graph_t g, vs_tuning *t) try {
   clear_error();
   if (!g || !t) { set_error("vs_graph_get_tuning: null"); return VS_ERR_INVALID; }
   *t = g->tuning;
   return VS_OK;
-}
+} VS_CATCH("vs_graph_get_tuning")
 
 // ------------------------------------------------------------- results
 static vs_status check_mat(vs_result_t r, uint32_t i) {
@@ -632,7 +632,7 @@ static vs_status check_mat(vs_result_t r, uint32_t i) {
 }
 
 vs_status vs_result_info(vs_result_t r, uint32_t *rows, uint32_t *cols,
-                         uint32_t *num_matrices, uint32_t *k_actual_max) {
+                         uint32_t *num_matrices, uint32_t *k_actual_max) try {
   clear_error();
   if (!r) { set_error("vs_result_info: null"); return VS_ERR_INVALID; }
   if (rows) *rows = r->S;
@@ -640,16 +640,16 @@ vs_status vs_result_info(vs_result_t r, uint32_t *rows, uint32_t *cols,
   if (num_matrices) *num_matrices = static_cast<uint32_t>(r->mats.size());
   if (k_actual_max) *k_actual_max = r->k_actual_max;
   return VS_OK;
-}
+} VS_CATCH("vs_result_info")
 
-vs_status vs_result_distance(vs_result_t r, uint32_t i, uint32_t *distance) {
+vs_status vs_result_distance(vs_result_t r, uint32_t i, uint32_t *distance) try {
   clear_error();
   VS_TRY(check_mat(r, i));
   if (distance) *distance = r->mats[i].distance;
   return VS_OK;
-}
+} VS_CATCH("vs_result_distance")
 
-vs_status vs_result_count(vs_result_t r, uint32_t i, uint64_t *count) {
+vs_status vs_result_count(vs_result_t r, uint32_t i, uint64_t *count) try {
   clear_error();
   VS_TRY(check_mat(r, i));
   vs_graph_s *g = r->g;
@@ -667,7 +667,7 @@ vs_status vs_result_count(vs_result_t r, uint32_t i, uint64_t *count) {
   g->pool.put(d, 8, false);
   if (count) *count = h;
   return VS_OK;
-}
+} VS_CATCH("vs_result_count")
 
 static vs_status row_reduce(vs_result_t r, uint32_t i, uint64_t *host, bool digest) {
   VS_TRY(check_mat(r, i));
@@ -694,15 +694,15 @@ static vs_status row_reduce(vs_result_t r, uint32_t i, uint64_t *host, bool dige
   return s;
 }
 
-vs_status vs_result_row_counts(vs_result_t r, uint32_t i, uint64_t *counts) {
+vs_status vs_result_row_counts(vs_result_t r, uint32_t i, uint64_t *counts) try {
   clear_error();
   return row_reduce(r, i, counts, false);
-}
+} VS_CATCH("vs_result_row_counts")
 
-vs_status vs_result_row_digests(vs_result_t r, uint32_t i, uint64_t *digests) {
+vs_status vs_result_row_digests(vs_result_t r, uint32_t i, uint64_t *digests) try {
   clear_error();
   return row_reduce(r, i, digests, true);
-}
+} VS_CATCH("vs_result_row_digests")
 
 static size_t export_bytes(vs_result_t r, uint32_t i, int32_t layout) {
   const uint64_t V = r->V;
@@ -720,7 +720,7 @@ static size_t export_bytes(vs_result_t r, uint32_t i, int32_t layout) {
 }
 
 vs_status vs_result_export_size(vs_result_t r, uint32_t i, int32_t layout,
-                                size_t *bytes) {
+                                size_t *bytes) try {
   clear_error();
   VS_TRY(check_mat(r, i));
   if (layout < VS_LAYOUT_NATIVE || layout > VS_LAYOUT_ROWMAJOR) {
@@ -729,10 +729,10 @@ vs_status vs_result_export_size(vs_result_t r, uint32_t i, int32_t layout,
   }
   if (bytes) *bytes = export_bytes(r, i, layout);
   return VS_OK;
-}
+} VS_CATCH("vs_result_export_size")
 
 vs_status vs_result_export(vs_result_t r, uint32_t i, int32_t layout,
-                           void *host_buf, size_t bytes) {
+                           void *host_buf, size_t bytes) try {
   clear_error();
   VS_TRY(check_mat(r, i));
   if (layout < VS_LAYOUT_NATIVE || layout > VS_LAYOUT_ROWMAJOR) {
@@ -780,10 +780,10 @@ vs_status vs_result_export(vs_result_t r, uint32_t i, int32_t layout,
     return VS_ERR_HIP;
   }
   return VS_OK;
-}
+} VS_CATCH("vs_result_export")
 
 vs_status vs_result_get(vs_result_t r, uint32_t i, uint32_t row, uint32_t col,
-                        int *bit) {
+                        int *bit) try {
   clear_error();
   VS_TRY(check_mat(r, i));
   if (row >= r->S || col >= r->V || !bit) {
@@ -798,9 +798,9 @@ vs_status vs_result_get(vs_result_t r, uint32_t i, uint32_t row, uint32_t col,
   VS_HIP(hipMemcpy(&word, r->mats[i].d + idx, 8, hipMemcpyDeviceToHost));
   *bit = static_cast<int>((word >> (row % 64)) & 1);
   return VS_OK;
-}
+} VS_CATCH("vs_result_get")
 
-vs_status vs_result_free(vs_result_t r) {
+vs_status vs_result_free(vs_result_t r) try {
   clear_error();
   if (!r) return VS_OK;
   vs_graph_s *g = r->g;
@@ -811,6 +811,6 @@ vs_status vs_result_free(vs_result_t r) {
   }
   delete r;
   return VS_OK;
-}
+} VS_CATCH("vs_result_free")
 
 } // extern "C"

@@ -146,7 +146,7 @@ const char *vs_version(void) { return "vertexsurge-amd 0.1 (gfx950)"; }
 // high half for odd l.  r < A -> (0,0); r < A+B -> (0,1); r < A+B+C -> (1,0);
 // else (1,1), with A, A+B, A+B+C the probabilities scaled to 2^32 and rounded.
 vs_status vs_rmat_csr(uint32_t scale, uint32_t edge_factor, double a, double b,
-                      double c, uint64_t seed, vs_csr_t *out) {
+                      double c, uint64_t seed, vs_csr_t *out) try {
   clear_error();
   if (!out || scale == 0 || scale > 31 || edge_factor == 0 || a < 0 || b < 0 ||
       c < 0 || a + b + c > 1.0) {
@@ -186,10 +186,10 @@ vs_status vs_rmat_csr(uint32_t scale, uint32_t edge_factor, double a, double b,
   }
   *out = h;
   return VS_OK;
-}
+} VS_CATCH("vs_rmat_csr")
 
 vs_status vs_er_csr(uint32_t num_vertices, uint64_t num_pairs, uint64_t seed,
-                    vs_csr_t *out) {
+                    vs_csr_t *out) try {
   clear_error();
   if (!out || num_vertices == 0) {
     set_error("vs_er_csr: invalid arguments");
@@ -213,11 +213,11 @@ vs_status vs_er_csr(uint32_t num_vertices, uint64_t num_pairs, uint64_t seed,
   }
   *out = h;
   return VS_OK;
-}
+} VS_CATCH("vs_er_csr")
 
 vs_status vs_csr_from_edges(uint32_t num_vertices, const uint32_t *src,
                             const uint32_t *dst, uint64_t n, int symmetrize,
-                            int drop_self_loops, vs_csr_t *out) {
+                            int drop_self_loops, vs_csr_t *out) try {
   clear_error();
   if (!out || (n > 0 && (!src || !dst))) {
     set_error("vs_csr_from_edges: invalid arguments");
@@ -240,10 +240,10 @@ vs_status vs_csr_from_edges(uint32_t num_vertices, const uint32_t *src,
   build_csr(num_vertices, n, symmetrize != 0, drop_self_loops != 0, gen, h->csr);
   *out = h;
   return VS_OK;
-}
+} VS_CATCH("vs_csr_from_edges")
 
 vs_status vs_csr_info(vs_csr_t csr, uint32_t *num_vertices, uint64_t *num_edges,
-                      const uint64_t **offsets, const uint32_t **adj) {
+                      const uint64_t **offsets, const uint32_t **adj) try {
   clear_error();
   if (!csr) { set_error("vs_csr_info: null handle"); return VS_ERR_INVALID; }
   if (num_vertices) *num_vertices = csr->csr.V;
@@ -251,18 +251,18 @@ vs_status vs_csr_info(vs_csr_t csr, uint32_t *num_vertices, uint64_t *num_edges,
   if (offsets) *offsets = csr->csr.off.data();
   if (adj) *adj = csr->csr.adj.data();
   return VS_OK;
-}
+} VS_CATCH("vs_csr_info")
 
-vs_status vs_csr_free(vs_csr_t csr) {
+vs_status vs_csr_free(vs_csr_t csr) try {
   delete csr;
   return VS_OK;
-}
+} VS_CATCH("vs_csr_free")
 
 // Sparse case (2S <= V): draw i gives draw_index(mix64(seed, i), V); keep the
 // first S distinct values.  Dense case: partial Fisher-Yates where step i swaps
 // position i with i + draw_index(mix64(seed, i), V - i).  Output sorted.
 vs_status vs_sample_sources(uint32_t num_vertices, uint32_t S, uint64_t seed,
-                            uint32_t *out) {
+                            uint32_t *out) try {
   clear_error();
   if (S > num_vertices || (S > 0 && !out)) {
     set_error("vs_sample_sources: S=%u exceeds V=%u", S, num_vertices);
@@ -292,12 +292,12 @@ vs_status vs_sample_sources(uint32_t num_vertices, uint32_t S, uint64_t seed,
   }
   std::sort(out, out + S);
   return VS_OK;
-}
+} VS_CATCH("vs_sample_sources")
 
 // Triple t uses draws mix64(seed, 3t), mix64(seed, 3t+1), mix64(seed, 3t+2).
 vs_status vs_sample_paths3(const uint64_t *offsets, const uint32_t *adj,
                            uint32_t num_vertices, uint64_t T, uint64_t seed,
-                           uint32_t *out) {
+                           uint32_t *out) try {
   clear_error();
   if (!offsets || (T > 0 && (!adj || !out))) {
     set_error("vs_sample_paths3: invalid arguments");
@@ -324,6 +324,6 @@ vs_status vs_sample_paths3(const uint64_t *offsets, const uint32_t *adj,
     out[3 * t + 2] = c;
   }
   return VS_OK;
-}
+} VS_CATCH("vs_sample_paths3")
 
 } // extern "C"

@@ -4,6 +4,8 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -14,6 +16,25 @@ namespace vs {
 // Thread-local last-error text behind vs_last_error().
 void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
 void clear_error();
+
+// Every C entry point returning vs_status is a function-try-block closed by
+// VS_CATCH(name): a host allocation that throws (std::vector, std::thread,
+// new) becomes VS_ERR_NOMEM, anything else VS_ERR_INTERNAL, so no C++
+// exception leaves the library.  Device and pool buffers held by the call
+// are RAII-owned and released while unwinding.
+#define VS_CATCH(fn)                                                   \
+  catch (const std::bad_alloc &) {                                     \
+    ::vs::set_error("%s: out of host memory", fn);                     \
+    return VS_ERR_NOMEM;                                               \
+  }                                                                    \
+  catch (const std::exception &e) {                                    \
+    ::vs::set_error("%s: internal error: %s", fn, e.what());           \
+    return VS_ERR_INTERNAL;                                            \
+  }                                                                    \
+  catch (...) {                                                        \
+    ::vs::set_error("%s: internal error", fn);                         \
+    return VS_ERR_INTERNAL;                                            \
+  }
 
 // Counter-based 64-bit mixer (splitmix64 finaliser over seed + ctr * golden).
 // Every synthetic draw in the library is mix64(seed, counter), so results do

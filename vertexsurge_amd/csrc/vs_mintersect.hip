@@ -588,16 +588,16 @@ vs_status vs_mintersect_device(const vs_graph_t *deps, uint32_t arity,
                                const uint32_t *d_tuples, uint64_t T,
                                uint32_t *d_counts, uint64_t *d_offsets,
                                uint32_t *d_lists, uint64_t list_cap,
-                               uint64_t *total, vs_mintersect_stats *stats) {
+                               uint64_t *total, vs_mintersect_stats *stats) try {
   clear_error();
   return mintersect_impl(deps, arity, d_tuples, T, d_counts, d_offsets, d_lists,
                          list_cap, total, stats);
-}
+} VS_CATCH("vs_mintersect_device")
 
 vs_status vs_mintersect(const vs_graph_t *deps, uint32_t arity,
                         const uint32_t *tuples, uint64_t T, uint32_t *counts,
                         uint64_t *offsets, uint32_t *lists, uint64_t list_cap,
-                        uint64_t *total, vs_mintersect_stats *stats) {
+                        uint64_t *total, vs_mintersect_stats *stats) try {
   clear_error();
   if (!deps || arity == 0 || arity > kMaxArity || !deps[0]) {
     set_error("vs_mintersect: bad dependencies / arity %u", arity);
@@ -659,6 +659,6 @@ vs_status vs_mintersect(const vs_graph_t *deps, uint32_t arity,
     if (tot) VS_HIP(hipMemcpy(lists, b.l, tot * 4, hipMemcpyDeviceToHost));
   }
   return VS_OK;
-}
+} VS_CATCH("vs_mintersect")
 
 } // extern "C"

@@ -218,14 +218,14 @@ using namespace vs;
 
 extern "C" {
 
-vs_status vs_sched_num_params(size_t *n) {
+vs_status vs_sched_num_params(size_t *n) try {
   clear_error();
   if (!n) { set_error("vs_sched_num_params: null"); return VS_ERR_INVALID; }
   *n = kNumParams;
   return VS_OK;
-}
+} VS_CATCH("vs_sched_num_params")
 
-vs_status vs_sched_create(uint64_t seed, vs_sched_t *out) {
+vs_status vs_sched_create(uint64_t seed, vs_sched_t *out) try {
   clear_error();
   if (!out) { set_error("vs_sched_create: null"); return VS_ERR_INVALID; }
   int ndev = 0;
@@ -261,7 +261,7 @@ vs_status vs_sched_create(uint64_t seed, vs_sched_t *out) {
   s->grid = kMlpBlocks;
   *out = s.release();
   return VS_OK;
-}
+} VS_CATCH("vs_sched_create")
 
 } // extern "C"
 
@@ -288,13 +288,13 @@ void sched_release(vs_sched_s *s) {
 
 extern "C" {
 
-vs_status vs_sched_free(vs_sched_t s) {
+vs_status vs_sched_free(vs_sched_t s) try {
   clear_error();
   sched_release(s);  // attached graphs keep their own reference
   return VS_OK;
-}
+} VS_CATCH("vs_sched_free")
 
-vs_status vs_sched_get_params(vs_sched_t s, float *params, size_t n) {
+vs_status vs_sched_get_params(vs_sched_t s, float *params, size_t n) try {
   clear_error();
   if (!s || !params || n != kNumParams) {
     set_error("vs_sched_get_params: need %zu floats", kNumParams);
@@ -304,9 +304,9 @@ vs_status vs_sched_get_params(vs_sched_t s, float *params, size_t n) {
   VS_HIP(hipStreamSynchronize(s->stream));
   VS_HIP(hipMemcpy(params, s->d_params, kNumParams * 4, hipMemcpyDeviceToHost));
   return VS_OK;
-}
+} VS_CATCH("vs_sched_get_params")
 
-vs_status vs_sched_set_params(vs_sched_t s, const float *params, size_t n) {
+vs_status vs_sched_set_params(vs_sched_t s, const float *params, size_t n) try {
   clear_error();
   if (!s || !params || n != kNumParams) {
     set_error("vs_sched_set_params: need %zu floats", kNumParams);
@@ -319,7 +319,7 @@ vs_status vs_sched_set_params(vs_sched_t s, const float *params, size_t n) {
   VS_HIP(hipMemset(s->d_v, 0, kNumParams * 4));
   s->step = 0;
   return VS_OK;
-}
+} VS_CATCH("vs_sched_set_params")
 
 // Uploads a batch; returns device pointers owned by the caller's lease.
 struct Batch {
@@ -361,7 +361,7 @@ static vs_status grad_step(vs_sched_s *s, const Batch &b, uint32_t n) {
 }
 
 vs_status vs_sched_loss_grad(vs_sched_t s, const float *X, const float *Y,
-                             const float *M, uint32_t n, float *loss, float *grad) {
+                             const float *M, uint32_t n, float *loss, float *grad) try {
   clear_error();
   if (!s) { set_error("vs_sched_loss_grad: null"); return VS_ERR_INVALID; }
   VS_HIP(hipSetDevice(s->device));
@@ -374,11 +374,11 @@ vs_status vs_sched_loss_grad(vs_sched_t s, const float *X, const float *Y,
   VS_HIP(hipStreamSynchronize(s->stream));
   if (loss) *loss = l;
   return VS_OK;
-}
+} VS_CATCH("vs_sched_loss_grad")
 
 vs_status vs_sched_train(vs_sched_t s, const float *X, const float *Y,
                          const float *M, uint32_t n, uint32_t epochs, float lr,
-                         float *loss_curve) {
+                         float *loss_curve) try {
   clear_error();
   if (!s) { set_error("vs_sched_train: null"); return VS_ERR_INVALID; }
   VS_HIP(hipSetDevice(s->device));
@@ -401,9 +401,9 @@ vs_status vs_sched_train(vs_sched_t s, const float *X, const float *Y,
   VS_HIP(hipStreamSynchronize(s->stream));
   if (loss_curve) std::memcpy(loss_curve, curve.data(), epochs * 4);
   return VS_OK;
-}
+} VS_CATCH("vs_sched_train")
 
-vs_status vs_sched_predict(vs_sched_t s, const float *X, uint32_t n, float *Y) {
+vs_status vs_sched_predict(vs_sched_t s, const float *X, uint32_t n, float *Y) try {
   clear_error();
   if (!s || (n > 0 && (!X || !Y))) { set_error("vs_sched_predict: null"); return VS_ERR_INVALID; }
   if (n == 0) return VS_OK;
@@ -424,15 +424,15 @@ vs_status vs_sched_predict(vs_sched_t s, const float *X, uint32_t n, float *Y) {
     return VS_ERR_HIP;
   }
   return st;
-}
+} VS_CATCH("vs_sched_predict")
 
-vs_status vs_graph_set_scheduler(vs_graph_t g, vs_sched_t s) {
+vs_status vs_graph_set_scheduler(vs_graph_t g, vs_sched_t s) try {
   clear_error();
   if (!g) { set_error("vs_graph_set_scheduler: null graph"); return VS_ERR_INVALID; }
   if (s) s->refs++;  // the graph's own reference: the caller may free its handle
   if (g->sched) sched_release(g->sched);
   g->sched = s;
   return VS_OK;
-}
+} VS_CATCH("vs_graph_set_scheduler")
 
 } // extern "C"

@@ -38,6 +38,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -157,7 +158,10 @@ class JParser {
             p_ += 2;
             uint32_t c2;
             if (!hex4(c2)) return false;
+            if (c2 < 0xDC00 || c2 >= 0xE000) return fail("bad surrogate pair");
             c1 = 0x10000 + ((c1 - 0xD800) << 10) + (c2 - 0xDC00);
+          } else if (c1 >= 0xD800 && c1 < 0xE000) {
+            return fail("unpaired surrogate");
           }
           utf8(out, c1);
           break;
@@ -440,7 +444,7 @@ using namespace vs;
 
 extern "C" {
 
-vs_status vs_store_open(const char *dir, const char *edge, vs_store_t *out) {
+vs_status vs_store_open(const char *dir, const char *edge, vs_store_t *out) try {
   clear_error();
   if (!dir || !out) {
     set_error("vs_store_open: null argument");
@@ -596,18 +600,18 @@ vs_status vs_store_open(const char *dir, const char *edge, vs_store_t *out) {
   }
   *out = s.release();
   return VS_OK;
-}
+} VS_CATCH("vs_store_open")
 
-vs_status vs_store_get_info(vs_store_t s, vs_store_info *info) {
+vs_status vs_store_get_info(vs_store_t s, vs_store_info *info) try {
   if (!s || !info) {
     set_error("vs_store_get_info: null argument");
     return VS_ERR_INVALID;
   }
   *info = s->info;
   return VS_OK;
-}
+} VS_CATCH("vs_store_get_info")
 
-vs_status vs_store_csr(vs_store_t s, const uint64_t **offsets, const uint32_t **adj) {
+vs_status vs_store_csr(vs_store_t s, const uint64_t **offsets, const uint32_t **adj) try {
   if (!s || !offsets || !adj) {
     set_error("vs_store_csr: null argument");
     return VS_ERR_INVALID;
@@ -615,9 +619,9 @@ vs_status vs_store_csr(vs_store_t s, const uint64_t **offsets, const uint32_t **
   *offsets = s->off;
   *adj = s->adj;
   return VS_OK;
-}
+} VS_CATCH("vs_store_csr")
 
-vs_status vs_store_graph(vs_store_t s, vs_graph_t *out) {
+vs_status vs_store_graph(vs_store_t s, vs_graph_t *out) try {
   if (!s || !out) {
     set_error("vs_store_graph: null argument");
     return VS_ERR_INVALID;
@@ -627,16 +631,16 @@ vs_status vs_store_graph(vs_store_t s, vs_graph_t *out) {
   // reverse, as Graph::get_csr_edge_refs assumes (schema.cpp:287-300)
   return vs_graph_create(s->off, s->adj, s->V, s->info.num_edges,
                          s->rel.directed ? 0u : VS_GRAPH_SYMMETRIC, out);
-}
+} VS_CATCH("vs_store_graph")
 
-vs_status vs_store_close(vs_store_t s) {
+vs_status vs_store_close(vs_store_t s) try {
   delete s;
   return VS_OK;
-}
+} VS_CATCH("vs_store_close")
 
 vs_status vs_store_write(const char *dir, const char *graph_name, const char *vertex_type,
                          const char *edge_type, int32_t directed, const uint64_t *offsets,
-                         const uint32_t *adj, uint32_t V, uint64_t E) {
+                         const uint32_t *adj, uint32_t V, uint64_t E) try {
   clear_error();
   if (!dir || !vertex_type || !edge_type || !*vertex_type || !*edge_type || !offsets ||
       (E && !adj)) {
@@ -676,7 +680,12 @@ vs_status vs_store_write(const char *dir, const char *graph_name, const char *ve
   if (s == VS_OK && r.directed) {
     std::vector<uint64_t> roff;
     std::vector<uint32_t> radj;
-    reverse_csr(offsets, adj, V, E, roff, radj);
+    try {
+      reverse_csr(offsets, adj, V, E, roff, radj);
+    } catch (const std::bad_alloc &) {
+      set_error("vs_store_write: out of host memory for the reverse CSR");
+      return VS_ERR_NOMEM;
+    }
     s = write_array(d + "/" + r.bwd_name() + "_csr_dst_src_index", roff.data(), 8,
                     static_cast<uint64_t>(V) + 1);
     if (s == VS_OK) s = write_array(d + "/" + r.bwd_name() + "_csr_dst_src", radj.data(), 4, E);
@@ -716,6 +725,6 @@ vs_status vs_store_write(const char *dir, const char *graph_name, const char *ve
     return VS_ERR_IO;
   }
   return VS_OK;
-}
+} VS_CATCH("vs_store_write")
 
 }  // extern "C"

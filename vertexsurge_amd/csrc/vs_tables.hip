@@ -641,7 +641,7 @@ using namespace vs;
 extern "C" {
 
 vs_status vs_colmat_create(vs_graph_t g, uint32_t rows, uint32_t cols, const uint64_t *words,
-                           vs_colmat_t *out) {
+                           vs_colmat_t *out) try {
   clear_error();
   if (!g || !out) { set_error("vs_colmat_create: null argument"); return VS_ERR_INVALID; }
   *out = nullptr;
@@ -662,10 +662,10 @@ vs_status vs_colmat_create(vs_graph_t g, uint32_t rows, uint32_t cols, const uin
   }
   *out = m.release();
   return VS_OK;
-}
+} VS_CATCH("vs_colmat_create")
 
 vs_status vs_colmat_set_maps(vs_colmat_t m, const uint32_t *row_map, const uint32_t *col_map,
-                             uint32_t n_col_map) {
+                             uint32_t n_col_map) try {
   clear_error();
   if (!m) { set_error("vs_colmat_set_maps: null matrix"); return VS_ERR_INVALID; }
   if (col_map && n_col_map > m->cols) {
@@ -678,10 +678,10 @@ vs_status vs_colmat_set_maps(vs_colmat_t m, const uint32_t *row_map, const uint3
   if (col_map) m->col_map.assign(col_map, col_map + n_col_map);
   else m->col_map.clear();
   return colmat_upload_maps(m);
-}
+} VS_CATCH("vs_colmat_set_maps")
 
 vs_status vs_colmat_info(vs_colmat_t m, uint32_t *rows, uint32_t *cols, uint64_t *pitch_words,
-                         int *has_row_map, uint32_t *n_col_map) {
+                         int *has_row_map, uint32_t *n_col_map) try {
   clear_error();
   if (!m) { set_error("vs_colmat_info: null matrix"); return VS_ERR_INVALID; }
   if (rows) *rows = m->rows;
@@ -690,9 +690,9 @@ vs_status vs_colmat_info(vs_colmat_t m, uint32_t *rows, uint32_t *cols, uint64_t
   if (has_row_map) *has_row_map = !m->row_map.empty();
   if (n_col_map) *n_col_map = static_cast<uint32_t>(m->col_map.size());
   return VS_OK;
-}
+} VS_CATCH("vs_colmat_info")
 
-vs_status vs_colmat_get_maps(vs_colmat_t m, uint32_t *row_map, uint32_t *col_map) {
+vs_status vs_colmat_get_maps(vs_colmat_t m, uint32_t *row_map, uint32_t *col_map) try {
   clear_error();
   if (!m) { set_error("vs_colmat_get_maps: null matrix"); return VS_ERR_INVALID; }
   if (row_map) {
@@ -704,9 +704,9 @@ vs_status vs_colmat_get_maps(vs_colmat_t m, uint32_t *row_map, uint32_t *col_map
     std::memcpy(col_map, m->col_map.data(), m->col_map.size() * 4);
   }
   return VS_OK;
-}
+} VS_CATCH("vs_colmat_get_maps")
 
-vs_status vs_colmat_export(vs_colmat_t m, uint64_t *words, size_t bytes) {
+vs_status vs_colmat_export(vs_colmat_t m, uint64_t *words, size_t bytes) try {
   clear_error();
   if (!m) { set_error("vs_colmat_export: null matrix"); return VS_ERR_INVALID; }
   const size_t need = static_cast<size_t>(m->cols) * m->pitch * 8;
@@ -720,9 +720,9 @@ vs_status vs_colmat_export(vs_colmat_t m, uint64_t *words, size_t bytes) {
   VS_HIP(hipMemcpyAsync(words, m->d, need, hipMemcpyDeviceToHost, m->g->stream));
   VS_HIP(hipStreamSynchronize(m->g->stream));
   return VS_OK;
-}
+} VS_CATCH("vs_colmat_export")
 
-vs_status vs_colmat_get(vs_colmat_t m, uint32_t row, uint32_t col, int *bit) {
+vs_status vs_colmat_get(vs_colmat_t m, uint32_t row, uint32_t col, int *bit) try {
   clear_error();
   if (!m || !bit || row >= m->rows || col >= m->cols) {
     set_error("vs_colmat_get: (%u,%u) outside the matrix", row, col);
@@ -735,15 +735,15 @@ vs_status vs_colmat_get(vs_colmat_t m, uint32_t row, uint32_t col, int *bit) {
                    hipMemcpyDeviceToHost));
   *bit = static_cast<int>((w >> (row % 64)) & 1);
   return VS_OK;
-}
+} VS_CATCH("vs_colmat_get")
 
-vs_status vs_colmat_free(vs_colmat_t m) {
+vs_status vs_colmat_free(vs_colmat_t m) try {
   clear_error();
   colmat_release(m);
   return VS_OK;
-}
+} VS_CATCH("vs_colmat_free")
 
-vs_status vs_table_convert(vs_result_t r, uint32_t i, int transpose, vs_colmat_t *out) {
+vs_status vs_table_convert(vs_result_t r, uint32_t i, int transpose, vs_colmat_t *out) try {
   clear_error();
   VS_TRY(result_mat(r, i));
   if (!out) { set_error("vs_table_convert: null output"); return VS_ERR_INVALID; }
@@ -775,10 +775,10 @@ vs_status vs_table_convert(vs_result_t r, uint32_t i, int transpose, vs_colmat_t
   VS_TRY(colmat_upload_maps(m.get()));
   *out = m.release();
   return VS_OK;
-}
+} VS_CATCH("vs_table_convert")
 
 vs_status vs_result_select_cols(vs_result_t r, uint32_t i, const uint32_t *col_ids, uint32_t n,
-                                vs_result_t *out) {
+                                vs_result_t *out) try {
   clear_error();
   VS_TRY(result_mat(r, i));
   if (!out || (n && !col_ids)) { set_error("vs_result_select_cols: null argument"); return VS_ERR_INVALID; }
@@ -839,59 +839,59 @@ vs_status vs_result_select_cols(vs_result_t r, uint32_t i, const uint32_t *col_i
   g->refs++;
   *out = res.release();
   return VS_OK;
-}
+} VS_CATCH("vs_result_select_cols")
 
-vs_status vs_result_col_map(vs_result_t r, uint32_t *col_ids) {
+vs_status vs_result_col_map(vs_result_t r, uint32_t *col_ids) try {
   clear_error();
   if (!r || !col_ids) { set_error("vs_result_col_map: null argument"); return VS_ERR_INVALID; }
   if (r->col_map.empty()) { set_error("result has no column map"); return VS_ERR_INVALID; }
   std::memcpy(col_ids, r->col_map.data(), r->col_map.size() * 4);
   return VS_OK;
-}
+} VS_CATCH("vs_result_col_map")
 
 vs_status vs_result_col_counts(vs_result_t r, uint32_t i, int32_t order, uint32_t *ids,
-                               int32_t *counts) {
+                               int32_t *counts) try {
   clear_error();
   VS_TRY(result_mat(r, i));
   VS_HIP(hipSetDevice(r->g->device));
   Scratch cnt(r->g);
   VS_TRY(result_col_counts_dev(r, i, cnt));
   return finish_col_counts(r->g, cnt.as<uint32_t>(), r->V, r->col_map, order, ids, counts);
-}
+} VS_CATCH("vs_result_col_counts")
 
-vs_status vs_colmat_col_counts(vs_colmat_t m, int32_t order, uint32_t *ids, int32_t *counts) {
+vs_status vs_colmat_col_counts(vs_colmat_t m, int32_t order, uint32_t *ids, int32_t *counts) try {
   clear_error();
   if (!m) { set_error("vs_colmat_col_counts: null matrix"); return VS_ERR_INVALID; }
   VS_HIP(hipSetDevice(m->g->device));
   Scratch cnt(m->g);
   VS_TRY(colmat_col_counts_dev(m, cnt));
   return finish_col_counts(m->g, cnt.as<uint32_t>(), m->cols, m->col_map, order, ids, counts);
-}
+} VS_CATCH("vs_colmat_col_counts")
 
-vs_status vs_result_row_or(vs_result_t r, uint32_t i, uint32_t *ids, uint32_t cap, uint32_t *n) {
+vs_status vs_result_row_or(vs_result_t r, uint32_t i, uint32_t *ids, uint32_t cap, uint32_t *n) try {
   clear_error();
   VS_TRY(result_mat(r, i));
   VS_HIP(hipSetDevice(r->g->device));
   Scratch cnt(r->g);
   VS_TRY(result_col_counts_dev(r, i, cnt));
   return finish_row_or(r->g, cnt.as<uint32_t>(), r->V, r->col_map, ids, cap, n);
-}
+} VS_CATCH("vs_result_row_or")
 
-vs_status vs_colmat_row_or(vs_colmat_t m, uint32_t *ids, uint32_t cap, uint32_t *n) {
+vs_status vs_colmat_row_or(vs_colmat_t m, uint32_t *ids, uint32_t cap, uint32_t *n) try {
   clear_error();
   if (!m) { set_error("vs_colmat_row_or: null matrix"); return VS_ERR_INVALID; }
   VS_HIP(hipSetDevice(m->g->device));
   Scratch cnt(m->g);
   VS_TRY(colmat_col_counts_dev(m, cnt));
   return finish_row_or(m->g, cnt.as<uint32_t>(), m->cols, m->col_map, ids, cap, n);
-}
+} VS_CATCH("vs_colmat_row_or")
 
 vs_status vs_mintersect_colmat(const vs_colmat_t *deps, uint32_t nd, const uint32_t *tuples,
                                uint64_t T, uint32_t *counts, uint64_t *offsets, uint32_t *lists,
-                               uint64_t list_cap, uint64_t *total, vs_mintersect_stats *stats) {
+                               uint64_t list_cap, uint64_t *total, vs_mintersect_stats *stats) try {
   clear_error();
   return mintersect_colmat_impl(deps, nd, tuples, T, counts, offsets, lists, list_cap, total,
                                 stats);
-}
+} VS_CATCH("vs_mintersect_colmat")
 
 } // extern "C"

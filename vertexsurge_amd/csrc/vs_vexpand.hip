@@ -721,7 +721,7 @@ extern "C" {
 
 vs_status vs_vexpand(vs_graph_t g, const uint32_t *sources, uint32_t S,
                      const vs_vexpand_params *p, vs_result_t *out,
-                     vs_vexpand_stats *stats) {
+                     vs_vexpand_stats *stats) try {
   clear_error();
   if (!g || !p || !out) { set_error("vs_vexpand: null argument"); return VS_ERR_INVALID; }
   *out = nullptr;
@@ -729,12 +729,12 @@ vs_status vs_vexpand(vs_graph_t g, const uint32_t *sources, uint32_t S,
   vs_status s = vexpand_run(g, sources, S, *p, nullptr, nullptr, &r, stats);
   if (s == VS_OK) *out = r;
   return s;
-}
+} VS_CATCH("vs_vexpand")
 
 vs_status vs_find_shortest(vs_graph_t g, const uint32_t *sources,
                            const uint32_t *targets, uint32_t S,
                            int32_t direction, uint32_t k_max, int32_t *dist,
-                           vs_vexpand_stats *stats) {
+                           vs_vexpand_stats *stats) try {
   clear_error();
   if (!g || (S > 0 && (!sources || !targets || !dist))) {
     set_error("vs_find_shortest: null argument");
@@ -749,6 +749,6 @@ vs_status vs_find_shortest(vs_graph_t g, const uint32_t *sources,
   p.sched = VS_SCHED_HEURISTIC;
   vs_result_s *r = nullptr;
   return vexpand_run(g, sources, S, p, targets, dist, &r, stats);
-}
+} VS_CATCH("vs_find_shortest")
 
 } // extern "C"
