@@ -74,3 +74,65 @@ def test_lists_must_be_sets():
     c, o, l, _ = vs.mintersect([G, G], tup)
     assert c.tolist() == [2] and l.tolist() == [0, 2]
     G.close()
+
+
+@pytest.mark.parametrize("stage_mb", [None, "0", "1"])
+def test_staged_and_two_pass_agree(orc, monkeypatch, stage_mb):
+    """Lists come from one staged pass by default; VS_MI_STAGE_MAX_MB=0 forces
+    count -> scan -> fill, and a 1 MB limit (below this workload's 4-byte
+    per driver entry staging) falls back the same way.  All match the oracle."""
+    if stage_mb is not None:
+        monkeypatch.setenv("VS_MI_STAGE_MAX_MB", stage_mb)
+    g = vs.HostCSR.rmat(16, 16, seed=21)
+    G = vs.Graph.from_csr(g)
+    tup = vs.sample_paths3(g.offsets, g.adj, 40000, seed=8)
+    b, d = g.offsets[tup], g.offsets[tup + 1]
+    assert (d - b).min(axis=1).sum() * 4 > (1 << 20)  # the 1 MB case cannot stage
+    counts, offsets, lists, st = vs.mintersect([G, G, G], tup)
+    c2, o2, l2 = orc.mintersect_sorted([(g.offsets, g.adj)] * 3, tup)
+    assert np.array_equal(counts, c2) and np.array_equal(offsets, o2)
+    assert np.array_equal(lists, l2) and st["total"] == len(l2)
+    G.close()
+
+
+def test_device_capacity_error_keeps_counts(orc):
+    """On HBM buffers, a list capacity one short of the total fails with
+    VS_ERR_CAPACITY after the staged pass, with the counts, offsets and total
+    already written; the same call with enough room gives the lists."""
+    import ctypes as C
+    lib = vs._lib
+    g = vs.HostCSR.rmat(12, 16, seed=12)
+    G = vs.Graph.from_csr(g)
+    tup = vs.sample_paths3(g.offsets, g.adj, 3000, seed=5)
+    T = len(tup)
+    c2, o2, l2 = orc.mintersect_sorted([(g.offsets, g.adj)] * 3, tup)
+    need = int(o2[-1])
+    assert need > 1
+    d_t, d_c, d_o, d_l = (C.c_void_p() for _ in range(4))
+    lib.call("vs_dev_alloc", tup.nbytes, C.byref(d_t))
+    lib.call("vs_dev_alloc", T * 4, C.byref(d_c))
+    lib.call("vs_dev_alloc", (T + 1) * 8, C.byref(d_o))
+    lib.call("vs_dev_alloc", need * 4, C.byref(d_l))
+    try:
+        lib.call("vs_memcpy_h2d", d_t, tup.ctypes.data_as(C.c_void_p), tup.nbytes)
+        deps = (C.c_void_p * 3)(G.handle, G.handle, G.handle)
+        total = C.c_uint64()
+        st = lib.MintersectStats()
+        with pytest.raises(vs.VSError) as e:
+            lib.call("vs_mintersect_device", deps, 3, d_t, T, d_c, d_o, d_l, need - 1,
+                     C.byref(total), C.byref(st))
+        assert e.value.code == lib.VS_ERR_CAPACITY and total.value == need
+        counts = np.zeros(T, dtype=np.uint32)
+        offs = np.zeros(T + 1, dtype=np.uint64)
+        lib.call("vs_memcpy_d2h", counts.ctypes.data_as(C.c_void_p), d_c, T * 4)
+        lib.call("vs_memcpy_d2h", offs.ctypes.data_as(C.c_void_p), d_o, (T + 1) * 8)
+        assert np.array_equal(counts, c2) and np.array_equal(offs, o2)
+        lib.call("vs_mintersect_device", deps, 3, d_t, T, d_c, d_o, d_l, need,
+                 C.byref(total), C.byref(st))
+        lists = np.zeros(need, dtype=np.uint32)
+        lib.call("vs_memcpy_d2h", lists.ctypes.data_as(C.c_void_p), d_l, need * 4)
+        assert total.value == need and np.array_equal(lists, l2)
+    finally:
+        for p in (d_t, d_c, d_o, d_l):
+            lib.call("vs_dev_free", p)
+        G.close()

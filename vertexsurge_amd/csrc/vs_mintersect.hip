@@ -19,8 +19,17 @@
 // long (hub) lists -- and then the at most 64 entries between two fence
 // posts, one or two cache lines: a lookup in a list of a million entries
 // misses the cache once or twice instead of at every one of its last eight
-// or so probes.  Two passes: count -> exclusive scan -> fill.
+// or so probes.
+//
+// One intersection pass when the lists are wanted: the split also writes each
+// tuple's bound (its driver length: the intersection is a subset of the
+// driver), whose exclusive scan places every tuple's survivors in a staging
+// buffer; the pass writes counts and staged lists at once, then the counts'
+// scan gives the final offsets and a copy kernel moves each tuple's run to
+// its place.  When the staging buffer (4 bytes per driver entry) would pass
+// a size limit, two passes instead: count -> exclusive scan -> fill.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include <rocprim/device/device_scan.hpp>
@@ -152,10 +161,15 @@ __device__ __forceinline__ bool member_all(const MiArgs &a, const uint64_t (&b)[
   return keep;
 }
 
+// What a pass writes: counts only, lists at a.offsets only (their counts are
+// known), or both (lists staged at a.offsets = the bounds' scan).
+constexpr int kMiCount = 0, kMiFill = 1, kMiStage = 2;
+
 // Tuples with a long shortest list: one wave per tuple, 64 driver elements
 // per step; each step's searches start where lane 63's landed.
-template <bool FILL, int N>
+template <int MODE, int N>
 __global__ __launch_bounds__(kBlock) void k_mintersect(MiArgs a) {
+  constexpr bool WRITE = MODE != kMiCount, COUNT = MODE != kMiFill;
   const int lane = lane_id(), wid = wave_id();
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
   const uint64_t cnt = a.norder[kBuckets - 1];
@@ -167,7 +181,7 @@ __global__ __launch_bounds__(kBlock) void k_mintersect(MiArgs a) {
     uint64_t best;
     const bool ok = tuple_lists<N>(a, t, b, n, drv, best);
     uint32_t total = 0;
-    uint64_t out = FILL ? a.offsets[t] : 0;
+    uint64_t out = WRITE ? a.offsets[t] : 0;
     if (ok && best > 0) {
       const uint32_t *dl = a.adj[drv];
       uint32_t from[N];
@@ -183,19 +197,20 @@ __global__ __launch_bounds__(kBlock) void k_mintersect(MiArgs a) {
 #pragma unroll
         for (int d = 0; d < N; ++d) from[d] = __shfl(lo[d], 63);
         const uint64_t m = __ballot(keep);
-        if (FILL && keep) a.lists[out + lane_rank(m)] = x;
+        if (WRITE && keep) a.lists[out + lane_rank(m)] = x;
         out += __popcll(m);
         total += __popcll(m);
       }
     }
-    if (!FILL && lane == 0) a.counts[t] = total;
+    if (COUNT && lane == 0) a.counts[t] = total;
   }
 }
 
 // Tuples of bucket K, whose shortest list has at most GS = bucket_len(K)
 // entries: 64 / GS tuples per wave, GS lanes each, one step.
-template <bool FILL, int N, int K>
+template <int MODE, int N, int K>
 __global__ __launch_bounds__(kBlock) void k_mintersect_short(MiArgs a) {
+  constexpr bool WRITE = MODE != kMiCount, COUNT = MODE != kMiFill;
   constexpr int GS = bucket_len(K), NG = 64 / GS;
   const int lane = lane_id(), wid = wave_id();
   const int g = lane / GS, gl = lane % GS;
@@ -218,17 +233,19 @@ __global__ __launch_bounds__(kBlock) void k_mintersect_short(MiArgs a) {
     const bool keep = member_all<N>(a, b, n, drv, from, x, have, lo);
     const uint64_t m = __ballot(keep);
     const uint64_t gm = (m >> (g * GS)) & ((1ull << GS) - 1);
-    if (FILL && keep)
+    if (WRITE && keep)
       a.lists[a.offsets[t] + __popcll(gm & ((1ull << gl) - 1))] = x;
-    if (!FILL && mine && gl == 0) a.counts[t] = static_cast<uint32_t>(__popcll(gm));
+    if (COUNT && mine && gl == 0) a.counts[t] = static_cast<uint32_t>(__popcll(gm));
   }
 }
 
 // Splits the tuples by the length of their shortest list into the buckets'
 // lists (one thread per tuple; positions taken with one atomic per wave and
-// bucket).  lists[k] has room for every tuple.
+// bucket).  lists[k] has room for every tuple.  bound, if set, receives
+// each tuple's driver length (0 for a tuple with an id outside its list set).
 template <int N>
-__global__ __launch_bounds__(kBlock) void k_mi_split(MiArgs a, uint32_t *lists, ull *norder) {
+__global__ __launch_bounds__(kBlock) void k_mi_split(MiArgs a, uint32_t *lists, ull *norder,
+                                                    uint32_t *bound) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   const uint64_t lim = (a.T + 63) / 64 * 64;  // whole waves stay in the loop
   for (uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; t < lim;
@@ -244,6 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_mi_split(MiArgs a, uint32_t *lists, 
       if (!ok) best = 0;
       for (int q = kBuckets - 2; q >= 0; --q)
         if (best <= bucket_len(q)) k = q;
+      if (bound) bound[t] = static_cast<uint32_t>(best);  // a list holds < 2^32 ids
     }
 #pragma unroll
     for (int q = 0; q < kBuckets; ++q) {
@@ -256,53 +274,83 @@ __global__ __launch_bounds__(kBlock) void k_mi_split(MiArgs a, uint32_t *lists, 
   }
 }
 
-template <bool FILL, int N>
+template <int MODE, int N>
 static void launch_fn(uint32_t grid, hipStream_t s, const MiArgs &a) {
-  hipLaunchKernelGGL((k_mintersect<FILL, N>), dim3(grid), dim3(kBlock), 0, s, a);
-  hipLaunchKernelGGL((k_mintersect_short<FILL, N, 0>), dim3(grid), dim3(kBlock), 0, s, a);
-  hipLaunchKernelGGL((k_mintersect_short<FILL, N, 1>), dim3(grid), dim3(kBlock), 0, s, a);
-  hipLaunchKernelGGL((k_mintersect_short<FILL, N, 2>), dim3(grid), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((k_mintersect<MODE, N>), dim3(grid), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((k_mintersect_short<MODE, N, 0>), dim3(grid), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((k_mintersect_short<MODE, N, 1>), dim3(grid), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((k_mintersect_short<MODE, N, 2>), dim3(grid), dim3(kBlock), 0, s, a);
 }
 
 template <int N>
-static void launch_n(bool fill, uint32_t grid, hipStream_t s, const MiArgs &a) {
-  if (fill) launch_fn<true, N>(grid, s, a);
-  else launch_fn<false, N>(grid, s, a);
+static void launch_n(int mode, uint32_t grid, hipStream_t s, const MiArgs &a) {
+  if (mode == kMiFill) launch_fn<kMiFill, N>(grid, s, a);
+  else if (mode == kMiStage) launch_fn<kMiStage, N>(grid, s, a);
+  else launch_fn<kMiCount, N>(grid, s, a);
 }
 
 template <int N>
 static void launch_split(uint32_t grid, hipStream_t s, const MiArgs &a, uint32_t *lists,
-                         ull *norder) {
-  hipLaunchKernelGGL((k_mi_split<N>), dim3(grid), dim3(kBlock), 0, s, a, lists, norder);
+                         ull *norder, uint32_t *bound) {
+  hipLaunchKernelGGL((k_mi_split<N>), dim3(grid), dim3(kBlock), 0, s, a, lists, norder, bound);
 }
 
 // arity as a template parameter keeps each variant's per-list state in
 // registers
-template <bool FILL>
-static void launch_mintersect(uint32_t arity, uint32_t grid, hipStream_t s, const MiArgs &a) {
+static void launch_mintersect(int mode, uint32_t arity, uint32_t grid, hipStream_t s,
+                              const MiArgs &a) {
   switch (arity) {
-    case 1: launch_n<1>(FILL, grid, s, a); break;
-    case 2: launch_n<2>(FILL, grid, s, a); break;
-    case 3: launch_n<3>(FILL, grid, s, a); break;
-    case 4: launch_n<4>(FILL, grid, s, a); break;
-    case 5: launch_n<5>(FILL, grid, s, a); break;
-    case 6: launch_n<6>(FILL, grid, s, a); break;
-    case 7: launch_n<7>(FILL, grid, s, a); break;
-    default: launch_n<8>(FILL, grid, s, a); break;
+    case 1: launch_n<1>(mode, grid, s, a); break;
+    case 2: launch_n<2>(mode, grid, s, a); break;
+    case 3: launch_n<3>(mode, grid, s, a); break;
+    case 4: launch_n<4>(mode, grid, s, a); break;
+    case 5: launch_n<5>(mode, grid, s, a); break;
+    case 6: launch_n<6>(mode, grid, s, a); break;
+    case 7: launch_n<7>(mode, grid, s, a); break;
+    default: launch_n<8>(mode, grid, s, a); break;
   }
 }
 
 static void launch_mi_split(uint32_t arity, uint32_t grid, hipStream_t s, const MiArgs &a,
-                            uint32_t *lists, ull *norder) {
+                            uint32_t *lists, ull *norder, uint32_t *bound) {
   switch (arity) {
-    case 1: launch_split<1>(grid, s, a, lists, norder); break;
-    case 2: launch_split<2>(grid, s, a, lists, norder); break;
-    case 3: launch_split<3>(grid, s, a, lists, norder); break;
-    case 4: launch_split<4>(grid, s, a, lists, norder); break;
-    case 5: launch_split<5>(grid, s, a, lists, norder); break;
-    case 6: launch_split<6>(grid, s, a, lists, norder); break;
-    case 7: launch_split<7>(grid, s, a, lists, norder); break;
-    default: launch_split<8>(grid, s, a, lists, norder); break;
+    case 1: launch_split<1>(grid, s, a, lists, norder, bound); break;
+    case 2: launch_split<2>(grid, s, a, lists, norder, bound); break;
+    case 3: launch_split<3>(grid, s, a, lists, norder, bound); break;
+    case 4: launch_split<4>(grid, s, a, lists, norder, bound); break;
+    case 5: launch_split<5>(grid, s, a, lists, norder, bound); break;
+    case 6: launch_split<6>(grid, s, a, lists, norder, bound); break;
+    case 7: launch_split<7>(grid, s, a, lists, norder, bound); break;
+    default: launch_split<8>(grid, s, a, lists, norder, bound); break;
+  }
+}
+
+// Moves each tuple's staged run stg[soff[t] .. + counts[t]) to
+// lists[off[t] ..].  One thread per tuple copies a short run itself; the
+// wave copies the long runs of its lanes together, 64 entries per step.
+constexpr uint32_t kCompactSerial = 16;
+
+__global__ __launch_bounds__(kBlock) void k_mi_compact(const uint32_t *__restrict__ counts,
+                                                      const uint64_t *__restrict__ soff,
+                                                      const uint64_t *__restrict__ off,
+                                                      const uint32_t *__restrict__ stg,
+                                                      uint32_t *__restrict__ lists, uint64_t T) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  const uint64_t lim = (T + 63) / 64 * 64;  // whole waves stay in the loop
+  for (uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; t < lim;
+       t += stride) {
+    const bool valid = t < T;
+    const uint32_t n = valid ? counts[t] : 0;
+    const uint64_t src = n ? soff[t] : 0, dst = n ? off[t] : 0;
+    const bool big = n > kCompactSerial;
+    if (!big)
+      for (uint32_t i = 0; i < n; ++i) lists[dst + i] = stg[src + i];
+    for (uint64_t m = __ballot(big); m; m &= m - 1) {
+      const int l = __ffsll(static_cast<long long>(m)) - 1;
+      const uint32_t nl = __shfl(n, l);
+      const uint64_t sl = shfl64(src, l), dl = shfl64(dst, l);
+      for (uint32_t i = lane_id(); i < nl; i += 64) lists[dl + i] = stg[sl + i];
+    }
   }
 }
 
@@ -411,6 +459,44 @@ __global__ void k_marked_bytes(const uint64_t *bm, uint32_t V, const uint64_t *o
   if (lane_id() == 0 && acc) atomicAdd(bytes, static_cast<ull>(acc));
 }
 
+// offsets[0..T) = exclusive scan of counts, offsets[T] = their sum, also
+// returned in *sum (the stream is synchronised).
+static vs_status scan_counts(vs_graph_s *g, hipStream_t s, const uint32_t *counts,
+                             uint64_t *offsets, uint64_t T, uint64_t *sum) {
+  size_t tmp_bytes = 0;
+  VS_HIP(rocprim::exclusive_scan(nullptr, tmp_bytes, counts, offsets, static_cast<uint64_t>(0),
+                                 static_cast<size_t>(T), rocprim::plus<uint64_t>(), s));
+  void *tmp = nullptr;
+  VS_TRY(g->pool.get(tmp_bytes, false, s, &tmp));
+  hipError_t e = rocprim::exclusive_scan(tmp, tmp_bytes, counts, offsets,
+                                         static_cast<uint64_t>(0), static_cast<size_t>(T),
+                                         rocprim::plus<uint64_t>(), s);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_last_offset, dim3(1), dim3(64), 0, s, counts, offsets, T);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(sum, offsets + T, 8, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  g->pool.put(tmp, tmp_bytes, false);
+  if (e != hipSuccess) {
+    set_error("vs_mintersect scan: %s", hipGetErrorString(e));
+    return VS_ERR_HIP;
+  }
+  return VS_OK;
+}
+
+// Largest staging buffer (bytes) one call may take for its single pass:
+// VS_MI_STAGE_MAX_MB from the environment, 4096 MB by default; 0 always
+// takes two passes.
+static uint64_t stage_limit_bytes() {
+  const char *e = getenv("VS_MI_STAGE_MAX_MB");
+  if (!e || !*e) return 4096ull << 20;
+  char *end = nullptr;
+  const unsigned long long mb = strtoull(e, &end, 10);
+  if (end == e) return 4096ull << 20;
+  return static_cast<uint64_t>(mb) << 20;
+}
+
 static vs_status mintersect_impl(const vs_graph_t *deps, uint32_t arity,
                                  const uint32_t *d_tuples, uint64_t T,
                                  uint32_t *d_counts, uint64_t *d_offsets,
@@ -462,58 +548,87 @@ static vs_status mintersect_impl(const vs_graph_t *deps, uint32_t arity,
     return VS_ERR_INVALID;
   }
   const uint32_t grid = grid_for(T, g->num_cus);
+  // pool buffers of this call, returned on every exit
+  struct Back {
+    Pool &p;
+    void *ptr[4] = {};
+    size_t bytes[4] = {};
+    int n = 0;
+    void add(void *q, size_t b) { ptr[n] = q; bytes[n++] = b; }
+    ~Back() { for (int i = 0; i < n; ++i) p.put(ptr[i], bytes[i], false); }
+  } back{g->pool};
   // tuple lists by driver length: room for T ids in each bucket, counters
   void *lp = nullptr, *np = nullptr;
   const size_t lbytes = kBuckets * T * 4, nbytes = kBuckets * 8;
   VS_TRY(g->pool.get(lbytes, false, s, &lp));
-  struct Back {
-    Pool &p; void *l, *n; size_t lb, nb;
-    ~Back() { if (l) p.put(l, lb, false); if (n) p.put(n, nb, false); }
-  } back{g->pool, lp, nullptr, lbytes, nbytes};
+  back.add(lp, lbytes);
   VS_TRY(g->pool.get(nbytes, false, s, &np));
-  back.n = np;
+  back.add(np, nbytes);
   VS_HIP(hipMemsetAsync(np, 0, nbytes, s));
   for (int k = 0; k < kBuckets; ++k) a.order[k] = static_cast<const uint32_t *>(lp) + k * T;
   a.norder = static_cast<const ull *>(np);
+  const uint64_t stage_max = stage_limit_bytes();
+  const bool try_stage = d_offsets && d_lists && stage_max > 0;
   VS_HIP(hipEventRecord(e0, s));
+  // with staging in view the split also writes each tuple's bound into counts
   launch_mi_split(arity, grid_for((T + 63) / 64, g->num_cus), s, a, static_cast<uint32_t *>(lp),
-                  static_cast<ull *>(np));
-  VS_HIP(hipGetLastError());
-  launch_mintersect<false>(arity, grid, s, a);
+                  static_cast<ull *>(np), try_stage ? d_counts : nullptr);
   VS_HIP(hipGetLastError());
   uint64_t tot = 0;
-  if (d_offsets) {
-    size_t tmp_bytes = 0;
-    VS_HIP(rocprim::exclusive_scan(nullptr, tmp_bytes, d_counts, d_offsets,
-                                   static_cast<uint64_t>(0), static_cast<size_t>(T),
-                                   rocprim::plus<uint64_t>(), s));
-    void *tmp = nullptr;
-    VS_TRY(g->pool.get(tmp_bytes, false, s, &tmp));
-    hipError_t e = rocprim::exclusive_scan(tmp, tmp_bytes, d_counts, d_offsets,
-                                           static_cast<uint64_t>(0), static_cast<size_t>(T),
-                                           rocprim::plus<uint64_t>(), s);
-    if (e == hipSuccess) {
-      hipLaunchKernelGGL(k_last_offset, dim3(1), dim3(64), 0, s, d_counts, d_offsets, T);
-      e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipMemcpyAsync(&tot, d_offsets + T, 8, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    g->pool.put(tmp, tmp_bytes, false);
-    if (e != hipSuccess) {
-      set_error("vs_mintersect scan: %s", hipGetErrorString(e));
-      return VS_ERR_HIP;
-    }
-    if (total) *total = tot;
-    if (d_lists) {
+  bool staged = false;
+  if (try_stage) {
+    void *so = nullptr;
+    const size_t sobytes = (T + 1) * 8;
+    VS_TRY(g->pool.get(sobytes, false, s, &so));
+    back.add(so, sobytes);
+    uint64_t *soff = static_cast<uint64_t *>(so);
+    uint64_t bound = 0;
+    VS_TRY(scan_counts(g, s, d_counts, soff, T, &bound));
+    // staging sizes rounded up to powers of two, so calls of similar sizes
+    // reuse one pooled buffer
+    const bool fits = bound <= stage_max / 4;
+    size_t sbytes = size_t{1} << 20;
+    while (fits && sbytes < bound * 4) sbytes <<= 1;
+    void *stg = nullptr;
+    if (fits && g->pool.get(sbytes, false, s, &stg) == VS_OK) {
+      back.add(stg, sbytes);
+      a.offsets = soff;
+      a.lists = static_cast<uint32_t *>(stg);
+      launch_mintersect(kMiStage, arity, grid, s, a);
+      VS_HIP(hipGetLastError());
+      VS_TRY(scan_counts(g, s, d_counts, d_offsets, T, &tot));
+      if (total) *total = tot;
       if (tot > list_cap) {
         set_error("vs_mintersect: list capacity %llu < total %llu",
                   (unsigned long long)list_cap, (unsigned long long)tot);
         return VS_ERR_CAPACITY;
       }
-      a.offsets = d_offsets;
-      a.lists = d_lists;
-      launch_mintersect<true>(arity, grid, s, a);
+      hipLaunchKernelGGL(k_mi_compact, dim3(grid_for((T + 63) / 64, g->num_cus)), dim3(kBlock),
+                         0, s, d_counts, soff, d_offsets, static_cast<const uint32_t *>(stg),
+                         d_lists, T);
       VS_HIP(hipGetLastError());
+      staged = true;
+    } else {
+      clear_error();  // no room to stage: two passes
+    }
+  }
+  if (!staged) {
+    launch_mintersect(kMiCount, arity, grid, s, a);
+    VS_HIP(hipGetLastError());
+    if (d_offsets) {
+      VS_TRY(scan_counts(g, s, d_counts, d_offsets, T, &tot));
+      if (total) *total = tot;
+      if (d_lists) {
+        if (tot > list_cap) {
+          set_error("vs_mintersect: list capacity %llu < total %llu",
+                    (unsigned long long)list_cap, (unsigned long long)tot);
+          return VS_ERR_CAPACITY;
+        }
+        a.offsets = d_offsets;
+        a.lists = d_lists;
+        launch_mintersect(kMiFill, arity, grid, s, a);
+        VS_HIP(hipGetLastError());
+      }
     }
   }
   VS_HIP(hipEventRecord(e1, s));
