@@ -368,7 +368,7 @@ vs_status vs_vexpand_batch(vs_graph_t g, const vs_query *queries, uint32_t nq,
 
 /* ------------------------------------------------------------- Mintersect */
 typedef struct {
-  double ms;            /* device time of count + scan + fill */
+  double ms;            /* device time of the call's kernels (split to last copy) */
   uint64_t total;       /* sum of counts */
   uint64_t bytes_alg;
 } vs_mintersect_stats;
@@ -385,7 +385,13 @@ vs_status vs_mintersect(const vs_graph_t *deps, uint32_t arity,
                         const uint32_t *tuples, uint64_t T, uint32_t *counts,
                         uint64_t *offsets, uint32_t *lists, uint64_t list_cap,
                         uint64_t *total, vs_mintersect_stats *stats);
-/* Same with every array already in device memory (from vs_dev_alloc). */
+/* Same with every array already in device memory (from vs_dev_alloc).
+ * With d_lists, one intersection pass stages every tuple's survivors
+ * (4 bytes per entry of its shortest list, pooled device memory) and copies
+ * them into place; on VS_ERR_CAPACITY d_counts, d_offsets and *total are
+ * already written.  When the staging would exceed VS_MI_STAGE_MAX_MB
+ * megabytes (environment, read per call; default 4096, 0 = never) the call
+ * counts, scans and intersects a second time to fill instead. */
 vs_status vs_mintersect_device(const vs_graph_t *deps, uint32_t arity,
                                const uint32_t *d_tuples, uint64_t T,
                                uint32_t *d_counts, uint64_t *d_offsets,
