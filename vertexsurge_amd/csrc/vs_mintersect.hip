@@ -240,37 +240,80 @@ __global__ __launch_bounds__(kBlock) void k_mintersect_short(MiArgs a) {
 }
 
 // Splits the tuples by the length of their shortest list into the buckets'
-// lists (one thread per tuple; positions taken with one atomic per wave and
-// bucket).  lists[k] has room for every tuple.  bound, if set, receives
-// each tuple's driver length (0 for a tuple with an id outside its list set).
+// lists; lists[k] has room for every tuple.  bound, if set, receives each
+// tuple's driver length (0 for a tuple with an id outside its list set).
+// A block takes kSplitRounds x kBlock consecutive tuples at a time and
+// reserves each bucket's positions for all of them with one atomic (wave
+// scans of the per-thread counts, then the block's waves in LDS): the four
+// bucket counters are the only shared words, and one atomic per wave and
+// bucket kept them busy for most of the kernel.
+constexpr int kSplitRounds = 8;
+
 template <int N>
 __global__ __launch_bounds__(kBlock) void k_mi_split(MiArgs a, uint32_t *lists, ull *norder,
                                                     uint32_t *bound) {
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  const uint64_t lim = (a.T + 63) / 64 * 64;  // whole waves stay in the loop
-  for (uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; t < lim;
-       t += stride) {
-    const bool valid = t < a.T;
-    int k = kBuckets - 1;
-    if (valid) {
-      uint64_t b[N];
-      uint32_t n[N];
-      int drv;
-      uint64_t best;
-      const bool ok = tuple_lists<N>(a, t, b, n, drv, best);
-      if (!ok) best = 0;
-      for (int q = kBuckets - 2; q >= 0; --q)
-        if (best <= bucket_len(q)) k = q;
-      if (bound) bound[t] = static_cast<uint32_t>(best);  // a list holds < 2^32 ids
+  __shared__ uint32_t s_wave[kWavesPerBlock][kBuckets];
+  __shared__ ull s_base[kBuckets];
+  const int lane = lane_id(), wid = wave_id();
+  constexpr uint64_t kChunk = static_cast<uint64_t>(kBlock) * kSplitRounds;
+  // block-uniform loop: every thread reaches the barriers
+  for (uint64_t c0 = blockIdx.x * kChunk; c0 < a.T; c0 += gridDim.x * kChunk) {
+    int kk[kSplitRounds];
+    uint32_t cnt[kBuckets] = {};
+#pragma unroll
+    for (int r = 0; r < kSplitRounds; ++r) {
+      const uint64_t t = c0 + static_cast<uint64_t>(r) * kBlock + threadIdx.x;
+      kk[r] = -1;
+      if (t < a.T) {
+        uint64_t b[N];
+        uint32_t n[N];
+        int drv;
+        uint64_t best;
+        if (!tuple_lists<N>(a, t, b, n, drv, best)) best = 0;
+        int k = kBuckets - 1;
+        for (int q = kBuckets - 2; q >= 0; --q)
+          if (best <= bucket_len(q)) k = q;
+        if (bound) bound[t] = static_cast<uint32_t>(best);  // a list holds < 2^32 ids
+        kk[r] = k;
+#pragma unroll
+        for (int q = 0; q < kBuckets; ++q) cnt[q] += k == q;
+      }
     }
+    uint32_t inc[kBuckets];
 #pragma unroll
     for (int q = 0; q < kBuckets; ++q) {
-      const uint64_t m = __ballot(valid && k == q);
-      ull base = 0;
-      if (lane_id() == 0 && m) base = atomicAdd(&norder[q], static_cast<ull>(__popcll(m)));
-      base = shfl64(base, 0);
-      if (valid && k == q) lists[q * a.T + base + lane_rank(m)] = static_cast<uint32_t>(t);
+      inc[q] = cnt[q];
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(inc[q], d);
+        if (lane >= d) inc[q] += y;
+      }
+      if (lane == 63) s_wave[wid][q] = inc[q];
     }
+    __syncthreads();
+    if (threadIdx.x < kBuckets) {
+      const int q = threadIdx.x;
+      uint32_t tot = 0;
+      for (int w = 0; w < kWavesPerBlock; ++w) {
+        const uint32_t x = s_wave[w][q];
+        s_wave[w][q] = tot;
+        tot += x;
+      }
+      s_base[q] = tot ? atomicAdd(&norder[q], static_cast<ull>(tot)) : 0;
+    }
+    __syncthreads();
+    uint64_t pos[kBuckets];
+#pragma unroll
+    for (int q = 0; q < kBuckets; ++q) pos[q] = s_base[q] + s_wave[wid][q] + inc[q] - cnt[q];
+#pragma unroll
+    for (int r = 0; r < kSplitRounds; ++r) {
+      const uint32_t t = static_cast<uint32_t>(c0 + static_cast<uint64_t>(r) * kBlock +
+                                               threadIdx.x);  // T < 2^32 - 1
+#pragma unroll
+      for (int q = 0; q < kBuckets; ++q)
+        if (kk[r] == q) lists[q * a.T + pos[q]++] = t;
+    }
+    __syncthreads();  // s_wave / s_base are rewritten by the next chunk
   }
 }
 
@@ -571,8 +614,10 @@ static vs_status mintersect_impl(const vs_graph_t *deps, uint32_t arity,
   const bool try_stage = d_offsets && d_lists && stage_max > 0;
   VS_HIP(hipEventRecord(e0, s));
   // with staging in view the split also writes each tuple's bound into counts
-  launch_mi_split(arity, grid_for((T + 63) / 64, g->num_cus), s, a, static_cast<uint32_t *>(lp),
-                  static_cast<ull *>(np), try_stage ? d_counts : nullptr);
+  const uint64_t split_chunks = (T + kBlock * kSplitRounds - 1) / (kBlock * kSplitRounds);
+  launch_mi_split(arity, grid_for(split_chunks * kWavesPerBlock, g->num_cus), s, a,
+                  static_cast<uint32_t *>(lp), static_cast<ull *>(np),
+                  try_stage ? d_counts : nullptr);
   VS_HIP(hipGetLastError());
   uint64_t tot = 0;
   bool staged = false;
