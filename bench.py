@@ -444,16 +444,26 @@ def run_mintersect_workload(vs, np, csr, G, args):
     for p in (d_t, d_c, d_o, d_l):
         lib.call("vs_dev_free", p)
     achieved = st.bytes_alg / (ms / 1e3) / 1e9
+    # HBM bytes per call measured by the rocprofv3 counter passes on this
+    # workload (committed with the profile summary); null for other sizes
+    workload = f"rmat{args.scale}_ef{args.edge_factor}_paths3_{T}"
+    traffic = None
+    tfile = ROOT / "profiles" / "traffic_mintersect_rmat24_paths3.json"
+    if tfile.exists():
+        tj = json.loads(tfile.read_text())
+        if tj.get("workload") == workload:
+            traffic = int(tj["traffic_bytes_per_launch"])
     out = {"metric": "Mintersect triples/s, |N(a)∩N(b)∩N(c)| + sorted lists, R-MAT-24",
            "value": round(T / (ms / 1e3), 1), "unit": "triples/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
            "higher_is_better": True, "dtype": "u32", "data": "synthetic",
-           "config": {"workload": f"rmat{args.scale}_ef{args.edge_factor}_paths3_{T}",
+           "config": {"workload": workload,
                       "triples": T, "list_entries": total.value,
                       "bit_exact_sample": exact, "sample": ns},
            "roofline": {"bound": "hbm", "kernel": "k_mintersect", "achieved": round(achieved, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                        "traffic_unit": "bytes per call (split, scans, intersection, compaction)"},
            "cpu_baseline": {"value": round(ns / cpu_s, 1), "unit": "triples/s",
                             "cores": threads, "kind": "port",
                             "sample": f"first {ns} of {T} triples, std::set_intersection chain"}}
