@@ -258,3 +258,48 @@ def test_empty_relation(tmp_path):
     vs.store_write(d, off, np.zeros(0, dtype=np.uint32), "V", "e")
     o, a, info = lib_open(d)
     assert o.tolist() == [0] * 11 and len(a) == 0 and info["num_edges"] == 0
+
+
+def test_mutated_stores_fail_cleanly(tmp_path):
+    """Seeded random damage to schema.json (byte flips, truncation, inserted
+    characters) and to the DiskArray headers never crashes the reader: every
+    open either succeeds or raises VSError with INVALID / IO / UNSUPPORTED."""
+    base = STORE / "knows"
+    text = (base / "schema.json").read_bytes()
+    names = [n for n in os.listdir(base) if n != "schema.json"]
+    rng = np.random.default_rng(11)
+    ok = failed = 0
+    for i in range(300):
+        d = tmp_path / f"m{i}"
+        _copy_store(base, d)
+        kind = i % 3
+        if kind == 0:  # flip bytes of the schema
+            raw = bytearray(text)
+            for p in rng.integers(0, len(raw), size=int(rng.integers(1, 4))):
+                raw[p] = int(rng.integers(0, 256))
+            (d / "schema.json").write_bytes(bytes(raw))
+        elif kind == 1:  # truncate it, or insert JSON punctuation
+            raw = bytearray(text)
+            p = int(rng.integers(0, len(raw)))
+            if rng.random() < 0.5:
+                raw = raw[:p]
+            else:
+                raw[p:p] = bytes(rng.choice(list(b'{}[]",:\\-0e'), size=3))
+            (d / "schema.json").write_bytes(bytes(raw))
+        else:  # damage a DiskArray header (count or element size)
+            name = names[int(rng.integers(0, len(names)))]
+            raw = bytearray((d / name).read_bytes())
+            field = int(rng.integers(0, 2)) * 8
+            val = int(rng.choice([-1, 0, 1, 3, 7, 2**31, 2**62]))
+            raw[field:field + 8] = np.array([val], dtype="<i8").tobytes()
+            (d / name).write_bytes(bytes(raw))
+        try:
+            with vs.GraphStore(d) as st:
+                o, a = st.csr()
+                assert o[0] == 0 and o[-1] == len(a)
+            ok += 1
+        except vs.VSError as e:
+            assert e.code in (vs._lib.VS_ERR_INVALID, vs._lib.VS_ERR_IO,
+                              vs._lib.VS_ERR_UNSUPPORTED), (i, e)
+            failed += 1
+    assert failed > 100 and ok + failed == 300
