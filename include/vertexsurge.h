@@ -76,6 +76,11 @@ typedef enum {
 
 /* vs_vexpand_params.flags */
 #define VS_VX_TIMING 0x1u /* record per-hop / per-launch HIP event timings */
+/* Expand the rows in the caller's order.  By default a call expands them
+ * sorted (within each tile of 64 * tile_words rows) by the source's 2-hop
+ * volume and puts its result matrices back into the caller's order before
+ * returning; results are the same either way. */
+#define VS_VX_KEEP_ORDER 0x2u
 /* vs_batch_params.flags: pack rows in input order (default: by source degree;
  * outputs are in input order either way) */
 #define VS_BATCH_KEEP_ORDER 0x2u
@@ -235,7 +240,9 @@ typedef struct {
   uint32_t num_tiles;
   uint32_t reserved;
   uint64_t te;           /* traversed edges: sum_i sum_u deg(u)*popcount(F_i[.,u]) */
-  uint64_t bytes_alg;    /* compulsory HBM bytes (words touched once) */
+  uint64_t bytes_alg;    /* compulsory HBM bytes (words touched once), counted
+                          * over the row order the call expands (see
+                          * VS_VX_KEEP_ORDER) */
   uint64_t visit_count;  /* ExpandStatus::visit_count after the last hop */
   double ms_total;       /* device time of init + hops + finalize */
   double ms_hops;        /* device time of the hop loop only */

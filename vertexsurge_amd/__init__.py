@@ -365,11 +365,15 @@ class VExpandResult:
 
 def vexpand(graph: Graph, sources, k_min: int, k_max: int, path: str = "shortest",
             direction: str = "forward", with_distance: bool = False,
-            sched: str = "heuristic", tile_words: int = 0, timing: bool = False):
-    """Multi-source VExpand; returns (VExpandResult, stats dict)."""
+            sched: str = "heuristic", tile_words: int = 0, timing: bool = False,
+            keep_order: bool = False):
+    """Multi-source VExpand; returns (VExpandResult, stats dict).  Rows are
+    expanded packed by 2-hop volume and returned in the caller's order unless
+    keep_order is set (then they are also expanded in that order)."""
     src = _u32(sources)
+    flags = (_lib.VX_TIMING if timing else 0) | (_lib.VX_KEEP_ORDER if keep_order else 0)
     p = _lib.VexpandParams(k_min, k_max, _PATHS[path], _DIRS[direction], int(with_distance),
-                           _SCHED[sched], tile_words, _lib.VX_TIMING if timing else 0)
+                           _SCHED[sched], tile_words, flags)
     st = _lib.VexpandStats()
     h = C.c_void_p()
     _lib.call("vs_vexpand", graph.handle, _ptr(src, C.c_uint32), len(src), C.byref(p),

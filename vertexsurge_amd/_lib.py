@@ -25,6 +25,7 @@ DIR_FORWARD, DIR_BACKWARD, DIR_BOTH = 0, 1, 2
 SCHED_HEURISTIC, SCHED_PUSH, SCHED_PULL, SCHED_LEARNED = 0, 1, 2, 3
 LAYOUT_NATIVE, LAYOUT_DVSM, LAYOUT_ROWMAJOR = 0, 1, 2
 VX_TIMING = 0x1
+VX_KEEP_ORDER = 0x2
 BATCH_KEEP_ORDER = 0x2
 ORDER_NONE, ORDER_ASC, ORDER_DESC = 0, 1, 2
 GRAPH_SYMMETRIC = 0x1

@@ -845,15 +845,14 @@ __global__ __launch_bounds__(kBlock) void k_src_vol2(const uint64_t *__restrict_
   }
 }
 
-static vs_status source_volumes(vs_graph_s *g, const DevCSR *push,
-                                const std::vector<uint32_t> &src, std::vector<uint64_t> &key) {
-  const size_t S = src.size();
+vs_status source_volumes(vs_graph_s *g, const DevCSR *push, const uint32_t *src, size_t S,
+                         uint64_t *key) {
   if (S == 0) return VS_OK;
   void *ds = nullptr, *dk = nullptr;
   VS_TRY(g->pool.get(S * 4, false, g->stream, &ds));
   vs_status st = g->pool.get(S * 8, false, g->stream, &dk);
   hipError_t e = hipSuccess;
-  if (st == VS_OK) e = hipMemcpyAsync(ds, src.data(), S * 4, hipMemcpyHostToDevice, g->stream);
+  if (st == VS_OK) e = hipMemcpyAsync(ds, src, S * 4, hipMemcpyHostToDevice, g->stream);
   if (st == VS_OK && e == hipSuccess) {
     hipLaunchKernelGGL(k_src_vol2, dim3(grid_for(S, g->num_cus)), dim3(kBlock), 0, g->stream,
                        push->off, push->adj, static_cast<const uint32_t *>(ds),
@@ -861,10 +860,10 @@ static vs_status source_volumes(vs_graph_s *g, const DevCSR *push,
     e = hipGetLastError();
   }
   if (st == VS_OK && e == hipSuccess)
-    e = hipMemcpyAsync(key.data(), dk, S * 8, hipMemcpyDeviceToHost, g->stream);
+    e = hipMemcpyAsync(key, dk, S * 8, hipMemcpyDeviceToHost, g->stream);
   if (st == VS_OK && e == hipSuccess) e = hipStreamSynchronize(g->stream);
   if (st == VS_OK && e != hipSuccess) {
-    set_error("batch: source volumes: %s", hipGetErrorString(e));
+    set_error("source volumes: %s", hipGetErrorString(e));
     st = VS_ERR_HIP;
   }
   g->pool.put(ds, S * 4, false);
@@ -929,7 +928,7 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
     for (uint32_t r = 0; r < S; ++r) order[r] = r;
     if (!(p.flags & VS_BATCH_KEEP_ORDER)) {
       std::vector<uint64_t> key(S);
-      VS_TRY(source_volumes(g, push, src, key));
+      VS_TRY(source_volumes(g, push, src.data(), S, key.data()));
       std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
         if (kmax[x] != kmax[y]) return kmax[x] > kmax[y];
         return key[x] > key[y];

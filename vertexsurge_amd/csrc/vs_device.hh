@@ -123,6 +123,10 @@ struct vs_graph_s {
 
 namespace vs {
 
+// key[r] = sum over u in N(src[r]) of deg(u) in the push CSR (the source's
+// 2-hop volume), computed on the device; S host ids in, S keys out.
+vs_status source_volumes(vs_graph_s *g, const DevCSR *push, const uint32_t *src, size_t S,
+                         uint64_t *key);
 // (push CSR, pull CSR) for a direction: push follows edges forward from a
 // frontier vertex, pull gathers over the edges that arrive at a vertex.
 vs_status graph_csrs(vs_graph_s *g, int direction, const DevCSR **push,
