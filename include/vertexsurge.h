@@ -76,10 +76,11 @@ typedef enum {
 
 /* vs_vexpand_params.flags */
 #define VS_VX_TIMING 0x1u /* record per-hop / per-launch HIP event timings */
-/* Expand the rows in the caller's order.  By default a call expands them
- * sorted (within each tile of 64 * tile_words rows) by the source's 2-hop
- * volume and puts its result matrices back into the caller's order before
- * returning; results are the same either way. */
+/* Expand the rows in the caller's order.  By default a call with k_max >= 3
+ * (>= 2 for vs_find_shortest) expands them sorted, within each tile of
+ * 64 * tile_words rows, by the source's 2-hop volume and puts its result
+ * matrices back into the caller's order before returning; results are the
+ * same either way. */
 #define VS_VX_KEEP_ORDER 0x2u
 /* vs_batch_params.flags: pack rows in input order (default: by source degree;
  * outputs are in input order either way) */

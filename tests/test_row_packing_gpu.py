@@ -47,7 +47,7 @@ def _same(a, b):
 def test_packed_equals_caller_order(orc, rmat13, path, tw, S):
     g, G = rmat13
     src = _sources(g, S, seed=S + tw)
-    for k_min, k_max in ((1, 3), (0, 2), (2, 4)):
+    for k_min, k_max in ((1, 3), (0, 3), (2, 4), (1, 2)):  # k_max 2 expands unpacked
         a, sa = vs.vexpand(G, src, k_min, k_max, path, tile_words=tw)
         b, sb = vs.vexpand(G, src, k_min, k_max, path, tile_words=tw, keep_order=True)
         assert sa["te"] == sb["te"]

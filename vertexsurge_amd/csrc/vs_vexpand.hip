@@ -374,9 +374,13 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
     st->num_tiles = T;
   }
   // Row packing (k_unpack_rows): the rows run in packed order, sources and
-  // targets permuted to match; pos[r] is caller row r's packed row.
+  // targets permuted to match; pos[r] is caller row r's packed row.  Packing
+  // pays from hop 2 on (fewer needed words per pull), restoring the order
+  // costs about one pass over each result matrix: used from k_max = 3, or 2
+  // for FindShortest, whose per-row distances are permuted on the host.
   const uint32_t *const caller_sources = sources;
-  const bool pack = !(p.flags & VS_VX_KEEP_ORDER) && W > 1;
+  const bool pack = !(p.flags & VS_VX_KEEP_ORDER) && W > 1 &&
+                    k_max >= (find_shortest ? 2u : 3u);
   const uint32_t RT = 64 * TW;
   std::vector<uint32_t> pk_src, pk_tgt, pos;
   std::vector<uint16_t> unpack;
