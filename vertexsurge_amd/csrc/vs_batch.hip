@@ -400,10 +400,19 @@ __global__ __launch_bounds__(kBlock) void k_bw_expand(BatchConst c, uint32_t hop
   const int j = lane_id() % TW;
   PullHead pre{};
   ull pre_u = ~0ull;  // unit whose head is prefetched in `pre`
+  // the current slot's unit range [sb, se): u only grows and the prefixes are
+  // fixed during the launch, so the search (dependent loads of the prefix
+  // table) runs only when u leaves the range
+  ull sb = 0, se = 0;
   for (ull u = blockIdx.x * kWavesPerBlock + wid; u < total; u += nwaves) {
-    uint32_t s = cur == UINT32_MAX ? 0 : cur;
-    while (s + 1 <= c.nslots && prefix[s + 1] <= u) ++s;
-    while (prefix[s] > u) --s;
+    uint32_t s = cur;
+    if (cur == UINT32_MAX || u < sb || u >= se) {
+      s = cur == UINT32_MAX ? 0 : cur;
+      while (s + 1 <= c.nslots && prefix[s + 1] <= u) ++s;
+      while (prefix[s] > u) --s;
+      sb = prefix[s];
+      se = prefix[s + 1];
+    }
     if (s != cur) {
       if (cur != UINT32_MAX && mode == 2) flush_ctr(lc, a.ctr, a.front_next, j);
       cur = s;
@@ -411,13 +420,13 @@ __global__ __launch_bounds__(kBlock) void k_bw_expand(BatchConst c, uint32_t hop
       mode = c.slots[s].mode;
       mk = lane_masks(a, j);
     }
-    const ull slot_end = prefix[s + 1];
-    const ull local = u - prefix[s];
+    const ull slot_end = se;
+    const ull local = u - sb;
     if (mode == 2) {
       const PullHead h = pre_u == u ? pre : pull_head<TW, SHORTEST>(a, a.pitems[local], j);
       const ull nu = u + nwaves;  // prefetch the next unit of the same slot
       if (nu < slot_end) {
-        pre = pull_head<TW, SHORTEST>(a, a.pitems[nu - prefix[s]], j);
+        pre = pull_head<TW, SHORTEST>(a, a.pitems[nu - sb], j);
         pre_u = nu;
       }
       pull_body<TW, SHORTEST, SPARSE>(a, h, s_nbr[wid], s_nm[wid], s_pairs[wid], s_acc[wid],
