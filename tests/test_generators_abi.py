@@ -84,6 +84,21 @@ def test_sample_paths3_are_paths():
     assert np.array_equal(t, t2)
 
 
+def test_sample_paths3_rejects_malformed_csr():
+    """The sampler reads the caller's CSR without an edge count: decreasing
+    offsets or a neighbour id >= V are reported instead of read past."""
+    off = np.array([0, 2, 1, 3], dtype=np.uint64)  # vertex 1's list runs backwards
+    adj = np.array([1, 2, 0], dtype=np.uint32)
+    with pytest.raises(vs.VSError) as e:
+        vs.sample_paths3(off, adj, 10, seed=1)
+    assert e.value.code == _lib.VS_ERR_INVALID and "decrease" in str(e.value)
+    off = np.array([0, 1, 2, 2], dtype=np.uint64)
+    adj = np.array([1, 7], dtype=np.uint32)  # 7 >= V = 3
+    with pytest.raises(vs.VSError) as e:
+        vs.sample_paths3(off, adj, 50, seed=1)
+    assert e.value.code == _lib.VS_ERR_INVALID and ">= V" in str(e.value)
+
+
 def test_csr_from_edges_checks_range():
     with pytest.raises(vs.VSError):
         vs.HostCSR.from_edges(4, [0, 1], [1, 9])

@@ -303,6 +303,13 @@ vs_status vs_sample_paths3(const uint64_t *offsets, const uint32_t *adj,
     set_error("vs_sample_paths3: invalid arguments");
     return VS_ERR_INVALID;
   }
+  // the lists must not overlap or run backwards: every adj index below is
+  // then inside [offsets[0], offsets[V])
+  for (uint32_t v = 0; v < num_vertices; ++v)
+    if (offsets[v + 1] < offsets[v]) {
+      set_error("vs_sample_paths3: offsets decrease at vertex %u", v);
+      return VS_ERR_INVALID;
+    }
   std::vector<uint32_t> nonisolated;
   for (uint32_t v = 0; v < num_vertices; ++v)
     if (offsets[v + 1] > offsets[v]) nonisolated.push_back(v);
@@ -311,17 +318,27 @@ vs_status vs_sample_paths3(const uint64_t *offsets, const uint32_t *adj,
     return VS_ERR_INVALID;
   }
   const uint32_t n = static_cast<uint32_t>(nonisolated.size());
-#pragma omp parallel for schedule(static)
+  int bad = 0;  // a neighbour id >= V: the path stops there
+#pragma omp parallel for schedule(static) reduction(| : bad)
   for (uint64_t t = 0; t < T; ++t) {
     const uint32_t a = nonisolated[draw_index(mix64(seed, 3 * t), n)];
     const uint32_t da = static_cast<uint32_t>(offsets[a + 1] - offsets[a]);
     const uint32_t b = adj[offsets[a] + draw_index(mix64(seed, 3 * t + 1), da)];
-    const uint32_t db = static_cast<uint32_t>(offsets[b + 1] - offsets[b]);
-    const uint32_t c =
-        db ? adj[offsets[b] + draw_index(mix64(seed, 3 * t + 2), db)] : b;
+    uint32_t c = b;
+    if (b >= num_vertices) {
+      bad = 1;
+    } else {
+      const uint32_t db = static_cast<uint32_t>(offsets[b + 1] - offsets[b]);
+      if (db) c = adj[offsets[b] + draw_index(mix64(seed, 3 * t + 2), db)];
+      if (c >= num_vertices) bad = 1;
+    }
     out[3 * t] = a;
     out[3 * t + 1] = b;
     out[3 * t + 2] = c;
+  }
+  if (bad) {
+    set_error("vs_sample_paths3: a neighbour id is >= V=%u", num_vertices);
+    return VS_ERR_INVALID;
   }
   return VS_OK;
 } VS_CATCH("vs_sample_paths3")
