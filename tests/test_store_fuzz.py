@@ -57,3 +57,27 @@ def test_store_reader_fuzz_under_sanitizers(fuzz_exe, tmp_path, which):
     assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-4000:])
     ok, rejected = (int(x) for x in r.stdout.split()[1::2])
     assert ok + rejected == iters and ok > 0 and rejected > iters // 2
+
+
+def test_host_generators_fuzz_under_sanitizers(tmp_path):
+    """tests/cpp/fuzz_host.cpp: random edge lists (some ids out of range) into
+    vs_csr_from_edges, random and damaged CSRs into vs_sample_paths3, and
+    source samples of every size, under ASan + UBSan; built CSRs must be
+    ascending sets and sampled paths must follow them."""
+    if shutil.which("g++") is None:
+        pytest.skip("needs g++")
+    exe = tmp_path / "fuzz_host"
+    cmd = ["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined",
+           "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer", "-fopenmp",
+           f"-I{ROOT / 'include'}", str(ROOT / "tests" / "cpp" / "fuzz_host.cpp"),
+           str(CSRC / "vs_host.cpp"), str(CSRC / "vs_store.cpp"), "-o", str(exe)]
+    b = subprocess.run(cmd, capture_output=True, text=True)
+    assert b.returncode == 0, b.stderr[-4000:]
+    iters = 1500
+    r = subprocess.run([str(exe), str(iters)], capture_output=True, text=True, timeout=300,
+                       env={"ASAN_OPTIONS": "detect_leaks=1:abort_on_error=0",
+                            "UBSAN_OPTIONS": "print_stacktrace=1", "OMP_NUM_THREADS": "2",
+                            "PATH": "/usr/bin:/bin"})
+    assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-4000:])
+    built, rejected = (int(x) for x in r.stdout.split()[1::2])
+    assert built + rejected == iters and built > iters // 2 and rejected > 0
