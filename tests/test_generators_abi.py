@@ -114,3 +114,24 @@ def test_graph_create_without_gpu_is_an_error():
     with pytest.raises(vs.VSError) as e:
         vs.Graph.from_csr(g)
     assert e.value.code == _lib.VS_ERR_NODEVICE
+
+
+def test_graph_create_rejects_malformed_csr():
+    """Checked on the host before anything reaches the device (so also here,
+    without a GPU): decreasing offsets and neighbour ids >= V are INVALID."""
+    cases = [
+        (np.array([0, 2, 1, 3], dtype=np.uint64), np.array([1, 2, 0], dtype=np.uint32),
+         "decrease"),
+        (np.array([0, 1, 2, 2], dtype=np.uint64), np.array([1, 3], dtype=np.uint32), ">= V"),
+    ]
+    for off, adj, msg in cases:
+        with pytest.raises(vs.VSError) as e:
+            vs.Graph(off, adj, symmetric=False)
+        assert e.value.code == _lib.VS_ERR_INVALID and msg in str(e.value)
+    # a large valid CSR passes the check (then needs a device)
+    g = vs.HostCSR.rmat(14, 16, seed=3)
+    try:
+        G = vs.Graph.from_csr(g)
+        G.close()
+    except vs.VSError as e:
+        assert e.code == _lib.VS_ERR_NODEVICE

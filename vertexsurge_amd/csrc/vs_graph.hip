@@ -529,6 +529,37 @@ vs_status vs_device_copy(void *dst, const void *src, size_t bytes) try {
   return VS_OK;
 } VS_CATCH("vs_device_copy")
 
+// The kernels index adj through offsets and rows through adj without bounds
+// checks of their own, so a CSR is checked once, before it reaches the device:
+// offsets non-decreasing and every id below V (threads over ranges).
+static vs_status check_csr(const uint64_t *off, const uint32_t *adj, uint32_t V, uint64_t E) {
+  const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<uint64_t> bad_off(nt, UINT64_MAX), bad_id(nt, UINT64_MAX);
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nt; ++t)
+    th.emplace_back([&, t] {
+      const uint64_t v0 = static_cast<uint64_t>(V) * t / nt, v1 = static_cast<uint64_t>(V) * (t + 1) / nt;
+      for (uint64_t v = v0; v < v1; ++v)
+        if (off[v + 1] < off[v]) { bad_off[t] = v; break; }
+      const uint64_t e0 = E * t / nt, e1 = E * (t + 1) / nt;
+      for (uint64_t e = e0; e < e1; ++e)
+        if (adj[e] >= V) { bad_id[t] = e; break; }
+    });
+  for (auto &x : th) x.join();
+  const uint64_t bo = *std::min_element(bad_off.begin(), bad_off.end());
+  const uint64_t bi = *std::min_element(bad_id.begin(), bad_id.end());
+  if (bo != UINT64_MAX) {
+    set_error("vs_graph_create: offsets decrease at vertex %llu", (unsigned long long)bo);
+    return VS_ERR_INVALID;
+  }
+  if (bi != UINT64_MAX) {
+    set_error("vs_graph_create: neighbour id %u >= V=%u at entry %llu", adj[bi], V,
+              (unsigned long long)bi);
+    return VS_ERR_INVALID;
+  }
+  return VS_OK;
+}
+
 vs_status vs_graph_create(const uint64_t *offsets, const uint32_t *adj,
                           uint32_t num_vertices, uint64_t num_edges,
                           uint32_t flags, vs_graph_t *out) try {
@@ -544,6 +575,7 @@ vs_status vs_graph_create(const uint64_t *offsets, const uint32_t *adj,
               (unsigned long long)num_edges);
     return VS_ERR_INVALID;
   }
+  VS_TRY(check_csr(offsets, adj, num_vertices, num_edges));
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
     (void)hipGetLastError();
