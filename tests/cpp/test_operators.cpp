@@ -91,6 +91,44 @@ int main(int argc, char **argv) {
   }
   std::printf("\"unknown_schema_throws\": %d,\n", threw ? 1 : 0);
 
+  // schema and strategy checks of the reference constructor /
+  // finishAddingDependency (vexpand.cpp:88-91, 125-163): a relation between
+  // two vertex types allows k_max = 1 only; Adaptive is not an output
+  // strategy; each strategy has its own result table kind
+  {
+    GraphStore::instance().add_edge("Person", "likes", "Post", rel);
+    auto throws = [](auto fn) {
+      try {
+        fn();
+      } catch (const std::runtime_error &) {
+        return 1;
+      }
+      return 0;
+    };
+    const int two_types_k2 = throws([] {
+      VExpand v("Person", "Post", "likes", Direction::Forward, 1, 2, ExpandPathType::Any,
+                ExpandStrategy::Dense, "p.id", std::nullopt);
+    });
+    const int two_types_k1 = throws([] {
+      VExpand v("Person", "Post", "likes", Direction::Forward, 1, 1, ExpandPathType::Any,
+                ExpandStrategy::Dense, "p.id", std::nullopt);
+    });
+    const int adaptive = throws([] {
+      VExpand v("Person", "Person", "knows", Direction::Forward, 1, 2,
+                ExpandPathType::Shortest, ExpandStrategy::Adaptive, "p.id", std::nullopt);
+    });
+    VExpand vcs("Person", "Person", "knows", Direction::Forward, 1, 2, ExpandPathType::Shortest,
+                ExpandStrategy::CSR, "p.id", std::nullopt);
+    vcs.run(std::vector<VID>(starts.begin(), starts.begin() + 3));
+    const int csr_matrix = throws([&] { (void)vcs.results(); });
+    VExpand vdn("Person", "Person", "knows", Direction::Forward, 1, 2, ExpandPathType::Shortest,
+                ExpandStrategy::Dense, "p.id", std::nullopt);
+    vdn.run(std::vector<VID>(starts.begin(), starts.begin() + 3));
+    const int dense_flat = throws([&] { (void)vdn.flat_results(); });
+    std::printf("\"schema_checks_throw\": [%d, %d, %d, %d, %d],\n", two_types_k2, two_types_k1,
+                adaptive, csr_matrix, dense_flat);
+  }
+
   // Mintersect KAT (mintersect.cpp:347-409): 3 result tuples
   {
     // vertices 1..8 have one neighbour each (1,1,1,2,2,3,3,4); 0 and 9 none

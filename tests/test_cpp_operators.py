@@ -43,6 +43,9 @@ def test_cpp_operators_match_oracle(orc):
     assert r["find_shortest_dist_plus1"] == exp
     assert r["empty_tuple_count"] == 0
     assert r["unknown_schema_throws"] == 1
+    # k_max 2 over Person-likes-Post throws, k_max 1 does not; Adaptive throws;
+    # CSR has no matrix result, Dense no flat result
+    assert r["schema_checks_throw"] == [1, 0, 1, 1, 1]
     assert r["mintersect_kat_tuples"] == 3 and r["mintersect_kat_count"] == 3
     assert r["mintersect_kat_flat"] == [1, 1, 1, 2, 1, 1, 3, 1, 1]
     assert r["mintersect_matmat_tuples"] == 3

@@ -188,3 +188,68 @@ def test_bytes_alg_formula_small(orc):
     # final R = {1, 2}: 8*nz(R) = 16
     assert b == (16 + 24 + 20) + (16 + 40 + 24) + 16
     assert te == 1 + 2
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_bfs_dist_matches_scipy(orc, name):
+    # orc_bfs_dist is the checker of FindShortest; the scipy distances
+    # (shortest_path, unweighted, directed: tests/golden/make_golden.py) pin it
+    g = load_golden(name)
+    for r, s in enumerate(g["sources"].tolist()):
+        d = orc.bfs_dist(g["offsets"], g["adj"], s)
+        assert np.array_equal(d, g["dist"][r]), (name, r, s)
+
+
+def test_bfs_dist_closed_forms(orc):
+    # path 0-1-...-69: dist(s, v) = |s - v|; two components: the other side is -1
+    g = load_golden("path70")
+    for s in (0, 10, 69):
+        assert orc.bfs_dist(g["offsets"], g["adj"], s).tolist() == [abs(s - v) for v in range(70)]
+    g = load_golden("star33")
+    d = orc.bfs_dist(g["offsets"], g["adj"], 7)
+    assert d[7] == 0 and d[0] == 1 and all(d[v] == 2 for v in range(1, 33) if v != 7)
+
+
+def _dvsm_hand_case():
+    # S = 1100 rows (two 1024-row stacks, the second ragged), V = 3 columns.
+    # Every row holds one bit, at its source column: column 0 except rows
+    # 1023 (column 2), 1024 (column 1) and 1099 (column 2).
+    src = np.zeros(1100, dtype=np.uint32)
+    src[1023], src[1024], src[1099] = 2, 1, 2
+    # DenseVarStackMatrix (bitmap.hh:348-356, bitmap.cpp:910-920): K = 1024,
+    # K / 64 = 16 words per column per stack, v_per_stack = align8(16 * 3) = 48,
+    # 2 stacks -> align8(96) = 96 words.  Worked by hand:
+    #   rows 0..1022, column 0  -> words 0..15 (row r in word r / 64, bit r % 64),
+    #                              word 15 lacks bit 63 (row 1023)
+    #   row 1023, column 2      -> word 0 + 2 * 16 + 15 = 47, bit 63
+    #   rows 1025..1098, col 0  -> stack 1 base 48: word 48 (rows 1024..1087,
+    #                              without bit 0 = row 1024), word 49 (rows
+    #                              1088..1098 = bits 0..10)
+    #   row 1024, column 1      -> 48 + 16 + 0 = 64, bit 0
+    #   row 1099, column 2      -> 48 + 32 + 1 = 81, bit 1099 % 64 = 11
+    exp = np.zeros(96, dtype=np.uint64)
+    exp[0:15] = np.uint64(0xFFFFFFFFFFFFFFFF)
+    exp[15] = np.uint64(0x7FFFFFFFFFFFFFFF)
+    exp[47] = np.uint64(1 << 63)
+    exp[48] = np.uint64(0xFFFFFFFFFFFFFFFE)
+    exp[49] = np.uint64(0x7FF)
+    exp[64] = np.uint64(1)
+    exp[81] = np.uint64(1 << 11)
+    return src, exp
+
+
+def test_dvsm_layout_hand_worked(orc):
+    src, exp = _dvsm_hand_case()
+    assert int(orc.lib().orc_dvsm_words(1100, 3)) == 96
+    rm = np.zeros((1100, 1), dtype=np.uint64)
+    rm[np.arange(1100), 0] = np.left_shift(np.uint64(1), src.astype(np.uint64))
+    assert np.array_equal(orc.rowmajor_to_dvsm(rm, 3), exp)
+    # oracle B's own indexing: an edgeless graph, Shortest with k_min = 0 is
+    # the visited matrix = L0 = the source bits
+    off = np.zeros(4, dtype=np.uint64)
+    adj = np.zeros(0, dtype=np.uint32)
+    blocks = orc.EdgeBlocks(off, adj, nblocks=1)
+    dv, ka, vc, _ = blocks.vexpand(src, 0, 2, "shortest")
+    blocks.close()
+    assert ka == 1 and vc == 1100
+    assert np.array_equal(dv, exp)

@@ -60,13 +60,25 @@ void Pool::trim() {
   free_.clear();
 }
 
+// On failure nothing stays allocated: d.off / d.adj are freed and reset, so
+// a later free_csr (which frees them only when owned) cannot leak them.
 static vs_status upload_csr(DevCSR &d, const uint64_t *off, const uint32_t *adj,
                             uint32_t V, uint64_t E) {
-  VS_HIP(hipMalloc(&d.off, (static_cast<size_t>(V) + 1) * sizeof(uint64_t)));
-  VS_HIP(hipMalloc(&d.adj, std::max<uint64_t>(E, 1) * sizeof(uint32_t)));
-  VS_HIP(hipMemcpy(d.off, off, (static_cast<size_t>(V) + 1) * sizeof(uint64_t),
-                   hipMemcpyHostToDevice));
-  if (E) VS_HIP(hipMemcpy(d.adj, adj, E * sizeof(uint32_t), hipMemcpyHostToDevice));
+  const size_t ob = (static_cast<size_t>(V) + 1) * sizeof(uint64_t);
+  hipError_t e = hipMalloc(&d.off, ob);
+  if (e == hipSuccess) e = hipMalloc(&d.adj, std::max<uint64_t>(E, 1) * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemcpy(d.off, off, ob, hipMemcpyHostToDevice);
+  if (e == hipSuccess && E) e = hipMemcpy(d.adj, adj, E * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    if (d.off) (void)hipFree(d.off);
+    if (d.adj) (void)hipFree(d.adj);
+    d.off = nullptr;
+    d.adj = nullptr;
+    set_error("CSR upload (V=%u, E=%llu): %s", V, static_cast<unsigned long long>(E),
+              hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? VS_ERR_NOMEM : VS_ERR_HIP;
+  }
   d.E = E;
   d.hoff.assign(off, off + static_cast<size_t>(V) + 1);
   d.owned = true;

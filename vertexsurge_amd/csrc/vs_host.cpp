@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <memory>
 #include <new>
 
 #include "vs_internal.hh"
@@ -235,10 +236,11 @@ vs_status vs_csr_from_edges(uint32_t num_vertices, const uint32_t *src,
     *pv = dst[e];
     return true;
   };
-  vs_csr_s *h = new (std::nothrow) vs_csr_s;
-  if (!h) { set_error("vs_csr_from_edges: out of host memory"); return VS_ERR_NOMEM; }
+  // owned until handed out: an exception thrown by build_csr (host OOM)
+  // frees the handle on its way to VS_CATCH
+  std::unique_ptr<vs_csr_s> h(new vs_csr_s);
   build_csr(num_vertices, n, symmetrize != 0, drop_self_loops != 0, gen, h->csr);
-  *out = h;
+  *out = h.release();
   return VS_OK;
 } VS_CATCH("vs_csr_from_edges")
 

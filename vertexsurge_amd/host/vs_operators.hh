@@ -383,10 +383,19 @@ class VExpand {
         start_variable_name_(std::move(variable_name)),
         target_variable_name_(std::move(target_variable_name)) {
     rel_ = GraphStore::instance().get_edge(src_type, edge_type, dst_type);
+    // a walk longer than one hop needs the relation to end where it starts
+    // (EdgeSchema::is_self_edge, schema.hh:79; vexpand.cpp:88-91 exits)
+    if (k_max > 1 && src_type != dst_type)
+      throw std::runtime_error("VExpand: k_max must be 1 when the edge " + src_type + "-" +
+                               edge_type + "-" + dst_type + " is not a self edge");
+    // finishAddingDependency builds a FlatTable for CSR and a matrix table
+    // for Dense; any other strategy asserts (vexpand.cpp:125-163)
+    if (strategy != ExpandStrategy::CSR && strategy != ExpandStrategy::Dense)
+      throw std::runtime_error("VExpand: the output strategy must be CSR or Dense");
     if (dir == Direction::Undecide) dir = Direction::Forward;
     dir_ = dir;
-    // every strategy expands through the bit matrices; CSR hands its result
-    // over as a flat table (flat_results)
+    // both strategies expand through the bit matrices; CSR hands its result
+    // over as a flat table (flat_results), Dense as matrix tables
     if (k_max >= VS_MAX_HOPS) throw std::runtime_error("k_max too large");
   }
 
@@ -427,12 +436,16 @@ class VExpand {
 
   // VarRowMatrixTable result (merge_to_MT)
   VarRowMatrixTable results() const {
+    if (strategy_ != ExpandStrategy::Dense)
+      throw std::runtime_error("VExpand: the CSR strategy's result is a FlatTable");
     if (!res_ || path_params_.with_distance)
       throw std::runtime_error("no merged matrix result");
     return VarRowMatrixTable(res_, 0, starts_);
   }
   // MultipleMatrixTable result (copy_to_multi_MT)
   MultipleMatrixTable multiple_results() const {
+    if (strategy_ != ExpandStrategy::Dense)
+      throw std::runtime_error("VExpand: the CSR strategy's result is a FlatTable");
     if (!res_ || !path_params_.with_distance)
       throw std::runtime_error("no per-distance result");
     MultipleMatrixTable m;
@@ -447,6 +460,8 @@ class VExpand {
   }
   // FlatTable result (the CSR strategy's merge_to_Flat, vexpand.cpp:380-446)
   FlatTable flat_results() const {
+    if (strategy_ != ExpandStrategy::CSR)
+      throw std::runtime_error("VExpand: the Dense strategy's result is a matrix table");
     if (!res_) throw std::runtime_error("no result");
     FlatTable t;
     uint64_t n = 0;
