@@ -57,6 +57,7 @@ def lib() -> C.CDLL:
                                        C.c_uint32, C.c_uint32, C.c_int, u64p, u32p, u64p,
                                        C.POINTER(C.c_double)]
         L.orc_rowmajor_to_dvsm.argtypes = [u64p, C.c_uint32, C.c_uint32, u64p]
+        L.orc_dvsm_row_counts.argtypes = [u64p, C.c_uint32, C.c_uint32, u64p]
         L.orc_mintersect_sorted.argtypes = [C.c_uint32, C.POINTER(u64p), C.POINTER(u32p),
                                             u32p, u32p, C.c_uint64, u32p, u64p, u32p]
         L.orc_mintersect_bitmap.argtypes = [C.c_uint32, C.POINTER(u64p), C.POINTER(u32p),
@@ -165,6 +166,16 @@ def rowmajor_to_dvsm(rm, V):
     S = rm.shape[0]
     out = np.zeros(int(lib().orc_dvsm_words(S, V)), dtype=np.uint64)
     lib().orc_rowmajor_to_dvsm(_p(rm, C.c_uint64), S, V, _p(out, C.c_uint64))
+    return out
+
+
+def dvsm_row_counts(dv, S, V):
+    """Per-row popcounts of a DVSM word array of S rows and V columns."""
+    dv = np.ascontiguousarray(dv, dtype=np.uint64)
+    if dv.size != int(lib().orc_dvsm_words(S, V)):
+        raise ValueError("DVSM array size does not match S and V")
+    out = np.zeros(S, dtype=np.uint64)
+    lib().orc_dvsm_row_counts(_p(dv, C.c_uint64), S, V, _p(out, C.c_uint64))
     return out
 
 

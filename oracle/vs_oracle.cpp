@@ -426,6 +426,27 @@ void orc_rowmajor_to_dvsm(const uint64_t *rm, uint32_t S, uint32_t V, uint64_t *
       }
 }
 
+// Per-row popcounts of a DenseVarStackMatrix of S rows and V columns: row r
+// lives in stack r / 1024, word (r % 1024) / 64 of each column's 16, bit
+// r % 64 (bitmap.hh:348-356).  One task per (stack, word) position.
+void orc_dvsm_row_counts(const uint64_t *dv, uint32_t S, uint32_t V, uint64_t *counts) {
+  const uint64_t stacks = (static_cast<uint64_t>(S) + kStack - 1) / kStack;
+  const uint64_t v_per_stack = kWpc * static_cast<uint64_t>(V);
+  std::fill(counts, counts + S, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t task = 0; task < static_cast<int64_t>(stacks * kWpc); ++task) {
+    const uint64_t st = task / kWpc, w = task % kWpc;
+    uint64_t cnt[64] = {0};
+    const uint64_t *base = dv + st * v_per_stack + w;
+    for (uint64_t c = 0; c < V; ++c)
+      for (uint64_t x = base[c * kWpc]; x; x &= x - 1) cnt[__builtin_ctzll(x)]++;
+    for (uint64_t b = 0; b < 64; ++b) {
+      const uint64_t r = st * kStack + w * 64 + b;
+      if (r < S) counts[r] = cnt[b];
+    }
+  }
+}
+
 // ----------------------------------------------------------------- oracle C
 // deps: arity CSRs; tuples: T x arity row-major.  counts[T]; if lists != null,
 // offsets[T+1] must be the exclusive scan of counts and lists gets the sets.

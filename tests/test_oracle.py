@@ -253,3 +253,19 @@ def test_dvsm_layout_hand_worked(orc):
     blocks.close()
     assert ka == 1 and vc == 1100
     assert np.array_equal(dv, exp)
+
+
+def test_dvsm_row_counts(orc):
+    # hand case: every row of the 1100-row matrix holds exactly one bit
+    src, exp = _dvsm_hand_case()
+    assert orc.dvsm_row_counts(exp, 1100, 3).tolist() == [1] * 1100
+    # against the row-major popcounts of random matrices (ragged stacks and
+    # columns), through the independently pinned rowmajor_to_dvsm
+    rng = np.random.default_rng(4)
+    for S, V in ((1, 1), (70, 130), (1100, 77), (2048, 64), (2050, 200)):
+        rm = rng.integers(0, 2**63, size=(S, (V + 63) // 64), dtype=np.uint64)
+        if V % 64:
+            rm[:, -1] &= np.uint64((1 << (V % 64)) - 1)
+        rm[S // 2] = 0
+        dv = orc.rowmajor_to_dvsm(rm, V)
+        assert np.array_equal(orc.dvsm_row_counts(dv, S, V), orc.row_counts(rm)), (S, V)
