@@ -406,6 +406,30 @@ vs_status vs_mintersect_device(const vs_graph_t *deps, uint32_t arity,
                                uint32_t *d_lists, uint64_t list_cap,
                                uint64_t *total, vs_mintersect_stats *stats);
 
+/* vs_mintersect_device without any allocation.  No counterpart in the
+ * reference: this entry point is for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  It always takes two passes (count -> scan -> fill) and
+ * keeps its working memory (tuple buckets, counters, scan partials) in one
+ * fixed device array of about 72 MiB per GPU, held from the library's first
+ * use of it on that GPU until the process ends; it calls no allocator and
+ * records no memset.  Limits and contract:
+ *   T <= VS_NOALLOC_MAX_TUPLES, else VS_ERR_INVALID;
+ *   d_offsets is required (T + 1 entries: the exclusive scan of d_counts and
+ *   the total); d_lists may be NULL (counts and offsets only), and when the
+ *   total exceeds list_cap the call returns VS_ERR_CAPACITY with d_counts,
+ *   d_offsets and *total written;
+ *   every call on one GPU shares that GPU's array: calls that overlap on one
+ *   GPU (two host threads, or graphs with different streams) race, and the
+ *   caller must serialise them.  The array keeps the last call's contents;
+ *   each call clears what it needs. */
+#define VS_NOALLOC_MAX_TUPLES 4718592u
+vs_status vs_mintersect_device_noalloc(const vs_graph_t *deps, uint32_t arity,
+                                       const uint32_t *d_tuples, uint64_t T,
+                                       uint32_t *d_counts, uint64_t *d_offsets,
+                                       uint32_t *d_lists, uint64_t list_cap,
+                                       uint64_t *total);
+
 /* ------------------------------------------------- learned scheduler (MLP) */
 /* 3-layer FP32 MLP: in -> 64 -> 64 -> out, ReLU, trained and evaluated by
  * this library's own HIP kernels. */

@@ -31,6 +31,7 @@ ORDER_NONE, ORDER_ASC, ORDER_DESC = 0, 1, 2
 GRAPH_SYMMETRIC = 0x1
 MAX_HOPS = 64
 MLP_IN, MLP_HIDDEN, MLP_OUT = 8, 64, 8
+NOALLOC_MAX_TUPLES = 4718592  # VS_NOALLOC_MAX_TUPLES
 
 u32p = C.POINTER(C.c_uint32)
 u64p = C.POINTER(C.c_uint64)
@@ -153,6 +154,8 @@ SIGNATURES = {
                                 C.c_uint64, u64p, C.POINTER(MintersectStats)]),
     "vs_mintersect_device": (C.c_int, [C.POINTER(vp), C.c_uint32, vp, C.c_uint64, vp, vp, vp,
                                        C.c_uint64, u64p, C.POINTER(MintersectStats)]),
+    "vs_mintersect_device_noalloc": (C.c_int, [C.POINTER(vp), C.c_uint32, vp, C.c_uint64, vp,
+                                               vp, vp, C.c_uint64, u64p]),
     "vs_colmat_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, u64p, C.POINTER(vp)]),
     "vs_colmat_set_maps": (C.c_int, [vp, u32p, u32p, C.c_uint32]),
     "vs_colmat_info": (C.c_int, [vp, u32p, u32p, u64p, C.POINTER(C.c_int), u32p]),

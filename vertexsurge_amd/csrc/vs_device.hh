@@ -132,6 +132,10 @@ vs_status source_volumes(vs_graph_s *g, const DevCSR *push, const uint32_t *src,
 vs_status graph_csrs(vs_graph_s *g, int direction, const DevCSR **push,
                      const DevCSR **pull);
 void graph_release(vs_graph_s *g);
+// Builds a CSR's intersection fence (every 64th adjacency entry) and checks
+// once whether its lists are strictly ascending sets (DevCSR::sets); done at
+// graph creation for the forward CSR (vs_mintersect.hip).
+vs_status build_fence(vs_graph_s *g, DevCSR &c);
 
 struct Mat {
   uint64_t *d = nullptr;
@@ -193,6 +197,12 @@ __device__ __forceinline__ uint64_t group_or(uint64_t x) {
 #pragma unroll
   for (int m = TW; m < 64; m <<= 1) x |= shfl_xor64(x, m);
   return x;
+}
+
+__device__ __forceinline__ ull shfl_up64(ull x, int d) {
+  const uint32_t lo = __shfl_up(static_cast<uint32_t>(x), d);
+  const uint32_t hi = __shfl_up(static_cast<uint32_t>(x >> 32), d);
+  return (static_cast<ull>(hi) << 32) | lo;
 }
 
 __device__ __forceinline__ uint64_t wave_sum(uint64_t x) {

@@ -610,6 +610,10 @@ vs_status vs_graph_create(const uint64_t *offsets, const uint32_t *adj,
     return s;
   }
   if (!g->symmetric) g->hadj.assign(adj, adj + num_edges);
+  // the intersection fence of the forward lists (E / 64 ids) is built here,
+  // once, so intersections never allocate it later; if this fails the first
+  // vs_mintersect call builds it instead
+  if (build_fence(g.get(), g->fwd) != VS_OK) clear_error();
   *out = g.release();
   return VS_OK;
 } VS_CATCH("vs_graph_create")

@@ -574,11 +574,6 @@ __device__ __forceinline__ void clear_block_masked(uint64_t *F, uint64_t *Fm, ui
 // Item builder for one 256-word slice of an active bitmap: per-thread counts
 // of kPushChunk-neighbour chunks, block-level exclusive scan, one atomicAdd
 // per block.  Must be called by the whole block (uses __syncthreads).
-__device__ __forceinline__ ull shfl_up64(ull x, int d) {
-  const uint32_t lo = __shfl_up(static_cast<uint32_t>(x), d);
-  const uint32_t hi = __shfl_up(static_cast<uint32_t>(x >> 32), d);
-  return (static_cast<ull>(hi) << 32) | lo;
-}
 
 __device__ __forceinline__ void items_slice(const uint64_t *actbm, uint32_t nvb,
                                             uint32_t base, const uint64_t *xoff,

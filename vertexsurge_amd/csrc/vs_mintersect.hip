@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 #include <rocprim/device/device_scan.hpp>
 
@@ -434,14 +435,17 @@ __global__ __launch_bounds__(kBlock) void k_check_sets(const uint32_t *__restric
 // Builds the fence of a CSR and checks, once, that its lists are sets: the
 // searches need strictly ascending lists (a repeated id in the driving list
 // would be emitted twice, where the reference's bitmap AND yields it once).
-static vs_status ensure_fence(vs_graph_s *g, DevCSR &c) {
-  if (!c.fence) {
+// A build that fails part-way leaves c.sets at -1 and is redone next time.
+vs_status build_fence(vs_graph_s *g, DevCSR &c) {
+  if (!c.fence || c.sets < 0) {
     const uint64_t nf = (c.E + 63) / 64;
     void *p = nullptr, *flag = nullptr;
     VS_HIP(hipMalloc(&flag, sizeof(int)));
     struct Free { void *p; ~Free() { (void)hipFree(p); } } fr{flag};
-    VS_HIP(hipMalloc(&p, std::max<uint64_t>(nf, 1) * 4));
-    c.fence = static_cast<uint32_t *>(p);
+    if (!c.fence) {
+      VS_HIP(hipMalloc(&p, std::max<uint64_t>(nf, 1) * 4));
+      c.fence = static_cast<uint32_t *>(p);
+    }
     VS_HIP(hipMemsetAsync(flag, 0, sizeof(int), g->stream));
     void *starts = nullptr;
     const size_t sbytes = ((c.E + 63) / 64) * 8;
@@ -464,7 +468,12 @@ static vs_status ensure_fence(vs_graph_s *g, DevCSR &c) {
     if (starts) g->pool.put(starts, sbytes, false);
     c.sets = bad ? 0 : 1;
   }
-  if (!c.sets) {
+  return VS_OK;
+}
+
+static vs_status ensure_fence(vs_graph_s *g, DevCSR &c) {
+  VS_TRY(build_fence(g, c));
+  if (c.sets != 1) {
     set_error("vs_mintersect: an adjacency list of a dependency repeats or misorders ids; "
               "intersections need ascending sets (vs_store_open builds them from a store)");
     return VS_ERR_INVALID;
@@ -738,11 +747,247 @@ static vs_status mintersect_impl(const vs_graph_t *deps, uint32_t arity,
   return VS_OK;
 }
 
+// ------------------------------------------------- allocation-free variant
+// vs_mintersect_device_noalloc runs the two-pass intersection (split ->
+// count -> scan -> fill) in a fixed device array instead of pool buffers,
+// one copy per GPU, held until the process ends:
+//   [0, 256)                       the four bucket counters (ull)
+//   [256, kNaHeader)               per-block sums of the counts scan and
+//                                  their total (ull, at most 1153 entries)
+//   [kNaHeader, kNaScratchBytes)   the four bucket lists, T ids each
+// Host code never passes the array's name to a kernel: it only stands for
+// the device address, which hipGetSymbolAddress gives per device (with that
+// device current) on the first call there.
+constexpr size_t kNaHeader = 64 << 10;
+constexpr size_t kNaScratchBytes =
+    kNaHeader + static_cast<size_t>(kBuckets) * VS_NOALLOC_MAX_TUPLES * 4;
+constexpr uint32_t kNaScanPer = 16;                         // counts per thread
+constexpr uint32_t kNaScanChunk = kBlock * kNaScanPer;      // counts per block
+constexpr int kNaMaxDevices = 64;
+static_assert(256 + 8 * ((VS_NOALLOC_MAX_TUPLES + kNaScanChunk - 1) / kNaScanChunk + 1) <=
+                  kNaHeader,
+              "scan partials outgrow the scratch header");
+
+static __device__ __align__(256) unsigned char scratch[kNaScratchBytes];
+
+static std::mutex g_na_mu;
+static unsigned char *g_na_ptr[kNaMaxDevices];
+
+// Device address of this GPU's scratch (the calling thread has it current).
+static vs_status na_scratch(int dev, unsigned char **out) {
+  if (dev < 0 || dev >= kNaMaxDevices) {
+    set_error("vs_mintersect_device_noalloc: device %d beyond %d", dev, kNaMaxDevices);
+    return VS_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      set_error("vs_mintersect_device_noalloc: scratch address on device %d: %s", dev,
+                hipGetErrorString(e));
+      return VS_ERR_HIP;
+    }
+    g_na_ptr[dev] = static_cast<unsigned char *>(p);
+  }
+  *out = g_na_ptr[dev];
+  return VS_OK;
+}
+
+// Zeroes n words (a fill kernel, so a captured launch sequence holds no
+// memset node).
+__global__ void k_na_zero(ull *p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+// part[b] = sum of counts[b * kNaScanChunk .. + kNaScanChunk)
+__global__ __launch_bounds__(kBlock) void k_na_block_sums(const uint32_t *__restrict__ counts,
+                                                         uint64_t T, ull *__restrict__ part) {
+  __shared__ ull s_w[kWavesPerBlock];
+  const uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * kNaScanChunk;
+  ull acc = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kNaScanPer; ++i) {
+    const uint64_t k = b0 + static_cast<uint64_t>(i) * kBlock + threadIdx.x;
+    if (k < T) acc += counts[k];
+  }
+  acc = wave_sum(acc);
+  if (lane_id() == 0) s_w[wave_id()] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ull t = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) t += s_w[w];
+    part[blockIdx.x] = t;
+  }
+}
+
+// Exclusive scan of part[0 .. nb) in place and part[nb] = the total; one wave.
+__global__ __launch_bounds__(64) void k_na_scan_part(ull *part, uint32_t nb) {
+  const int lane = lane_id();
+  ull base = 0;
+  for (uint32_t b0 = 0; b0 < nb; b0 += 64) {
+    const uint32_t i = b0 + lane;
+    const ull x = i < nb ? part[i] : 0;
+    ull inc = x;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const ull y = shfl_up64(inc, d);
+      if (lane >= d) inc += y;
+    }
+    if (i < nb) part[i] = base + inc - x;
+    base += shfl64(inc, 63);
+  }
+  if (lane == 0) part[nb] = base;
+}
+
+// offsets[k] = exclusive prefix sum of counts; offsets[T] = the total.  A
+// thread scans kNaScanPer consecutive counts of its block's chunk.
+__global__ __launch_bounds__(kBlock) void k_na_offsets(const uint32_t *__restrict__ counts,
+                                                      uint64_t T, const ull *__restrict__ part,
+                                                      uint32_t nb, uint64_t *__restrict__ offsets) {
+  __shared__ ull s_w[kWavesPerBlock];
+  const int lane = lane_id(), wid = wave_id();
+  const uint64_t k0 = static_cast<uint64_t>(blockIdx.x) * kNaScanChunk +
+                      static_cast<uint64_t>(threadIdx.x) * kNaScanPer;
+  uint32_t c[kNaScanPer];
+  ull sum = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < kNaScanPer; ++i) {
+    c[i] = k0 + i < T ? counts[k0 + i] : 0u;
+    sum += c[i];
+  }
+  ull inc = sum;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const ull y = shfl_up64(inc, d);
+    if (lane >= d) inc += y;
+  }
+  if (lane == 63) s_w[wid] = inc;
+  __syncthreads();
+  ull run = part[blockIdx.x] + inc - sum;
+  for (int w = 0; w < wid; ++w) run += s_w[w];
+#pragma unroll
+  for (uint32_t i = 0; i < kNaScanPer; ++i) {
+    if (k0 + i < T) offsets[k0 + i] = run;
+    run += c[i];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) offsets[T] = part[nb];
+}
+
+static vs_status mintersect_noalloc(const vs_graph_t *deps, uint32_t arity,
+                                    const uint32_t *d_tuples, uint64_t T, uint32_t *d_counts,
+                                    uint64_t *d_offsets, uint32_t *d_lists, uint64_t list_cap,
+                                    uint64_t *total) {
+  const char *fn = "vs_mintersect_device_noalloc";
+  if (!deps || arity == 0 || arity > kMaxArity) {
+    set_error("%s: arity %u not in 1..%d", fn, arity, kMaxArity);
+    return VS_ERR_INVALID;
+  }
+  for (uint32_t d = 0; d < arity; ++d) {
+    if (!deps[d]) { set_error("%s: null dependency %u", fn, d); return VS_ERR_INVALID; }
+    if (deps[d]->device != deps[0]->device) {
+      set_error("%s: dependencies on different devices", fn);
+      return VS_ERR_INVALID;
+    }
+  }
+  if (!d_offsets) { set_error("%s: needs d_offsets (T + 1 entries)", fn); return VS_ERR_INVALID; }
+  if (T > VS_NOALLOC_MAX_TUPLES) {
+    set_error("%s: %llu tuples exceed the scratch limit of %u per call", fn,
+              static_cast<unsigned long long>(T), VS_NOALLOC_MAX_TUPLES);
+    return VS_ERR_INVALID;
+  }
+  if (T > 0 && (!d_tuples || !d_counts)) {
+    set_error("%s: null tuples/counts", fn);
+    return VS_ERR_INVALID;
+  }
+  if (total) *total = 0;
+  vs_graph_s *g = deps[0];
+  VS_HIP(hipSetDevice(g->device));
+  hipStream_t s = g->stream;
+  unsigned char *base = nullptr;
+  VS_TRY(na_scratch(g->device, &base));
+  ull *norder = reinterpret_cast<ull *>(base);
+  ull *part = reinterpret_cast<ull *>(base + 256);
+  uint32_t *buckets = reinterpret_cast<uint32_t *>(base + kNaHeader);
+  MiArgs a{};
+  a.arity = arity;
+  a.T = T;
+  a.tuples = d_tuples;
+  for (uint32_t d = 0; d < arity; ++d) {
+    const DevCSR &c = deps[d]->fwd;
+    if (!c.fence || c.sets < 0) {
+      set_error("%s: dependency %u has no intersection fence (building it when the graph "
+                "was created failed; a vs_mintersect call builds it)", fn, d);
+      return VS_ERR_INVALID;
+    }
+    if (c.sets != 1) {
+      set_error("%s: an adjacency list of dependency %u repeats or misorders ids; "
+                "intersections need ascending sets", fn, d);
+      return VS_ERR_INVALID;
+    }
+    a.off[d] = c.off;
+    a.adj[d] = c.adj;
+    a.fence[d] = c.fence;
+    a.V[d] = deps[d]->V;
+  }
+  const uint32_t nb = static_cast<uint32_t>((T + kNaScanChunk - 1) / kNaScanChunk);
+  if (T == 0) {  // offsets[0] = 0
+    hipLaunchKernelGGL(k_na_zero, dim3(1), dim3(64), 0, s, reinterpret_cast<ull *>(d_offsets), 1u);
+    VS_HIP(hipGetLastError());
+    VS_HIP(hipStreamSynchronize(s));
+    return VS_OK;
+  }
+  a.counts = d_counts;
+  for (int k = 0; k < kBuckets; ++k) a.order[k] = buckets + static_cast<size_t>(k) * T;
+  a.norder = norder;
+  hipLaunchKernelGGL(k_na_zero, dim3(1), dim3(64), 0, s, norder, static_cast<uint32_t>(kBuckets));
+  const uint32_t grid = grid_for(T, g->num_cus);
+  const uint64_t split_chunks = (T + kBlock * kSplitRounds - 1) / (kBlock * kSplitRounds);
+  launch_mi_split(arity, grid_for(split_chunks * kWavesPerBlock, g->num_cus), s, a, buckets,
+                  norder, nullptr);
+  launch_mintersect(kMiCount, arity, grid, s, a);
+  hipLaunchKernelGGL(k_na_block_sums, dim3(nb), dim3(kBlock), 0, s,
+                     static_cast<const uint32_t *>(d_counts), T, part);
+  hipLaunchKernelGGL(k_na_scan_part, dim3(1), dim3(64), 0, s, part, nb);
+  hipLaunchKernelGGL(k_na_offsets, dim3(nb), dim3(kBlock), 0, s,
+                     static_cast<const uint32_t *>(d_counts), T,
+                     static_cast<const ull *>(part), nb, d_offsets);
+  VS_HIP(hipGetLastError());
+  uint64_t tot = 0;
+  VS_HIP(hipMemcpyAsync(&tot, d_offsets + T, 8, hipMemcpyDeviceToHost, s));
+  VS_HIP(hipStreamSynchronize(s));
+  if (total) *total = tot;
+  if (!d_lists) return VS_OK;
+  if (tot > list_cap) {
+    set_error("%s: list capacity %llu < total %llu", fn, static_cast<unsigned long long>(list_cap),
+              static_cast<unsigned long long>(tot));
+    return VS_ERR_CAPACITY;
+  }
+  a.offsets = d_offsets;
+  a.lists = d_lists;
+  launch_mintersect(kMiFill, arity, grid, s, a);
+  VS_HIP(hipGetLastError());
+  VS_HIP(hipStreamSynchronize(s));
+  return VS_OK;
+}
+
 } // namespace vs
 
 using namespace vs;
 
 extern "C" {
+
+vs_status vs_mintersect_device_noalloc(const vs_graph_t *deps, uint32_t arity,
+                                       const uint32_t *d_tuples, uint64_t T,
+                                       uint32_t *d_counts, uint64_t *d_offsets,
+                                       uint32_t *d_lists, uint64_t list_cap,
+                                       uint64_t *total) try {
+  clear_error();
+  return mintersect_noalloc(deps, arity, d_tuples, T, d_counts, d_offsets, d_lists, list_cap,
+                            total);
+} VS_CATCH("vs_mintersect_device_noalloc")
 
 vs_status vs_mintersect_device(const vs_graph_t *deps, uint32_t arity,
                                const uint32_t *d_tuples, uint64_t T,
