@@ -91,3 +91,36 @@ def test_packed_find_shortest(orc, rmat13):
     exp = np.where(hit.any(axis=0), hit.argmax(axis=0) + 1, -1)
     exp[src == tgt] = 0
     assert np.array_equal(d, exp)
+
+
+@pytest.mark.parametrize("tw,S", [(64, 3000), (8, 1500)])
+def test_packed_table_operators(rmat13, tw, S):
+    # a packed result keeps its rows in expansion order and applies the
+    # stored-row map when read: every table operator must see caller order
+    g, G = rmat13
+    src = _sources(g, S, seed=11 * tw)
+    a, _ = vs.vexpand(G, src, 1, 3, "any", tile_words=tw)
+    b, _ = vs.vexpand(G, src, 1, 3, "any", tile_words=tw, keep_order=True)
+    for transpose in (False, True):
+        ca, cb = a.to_colmain(transpose=transpose), b.to_colmain(transpose=transpose)
+        assert np.array_equal(ca.export(), cb.export()), transpose
+        assert [None if x is None else x.tolist() for x in ca.maps()] == \
+               [None if x is None else x.tolist() for x in cb.maps()]
+        ca.close()
+        cb.close()
+    cols = np.unique(src[::7])[:50]
+    sa, sb = a.select_cols(cols), b.select_cols(cols)
+    _same(sa, sb)
+    for o in (None, "asc", "desc"):
+        x, y = a.col_counts(order=o), b.col_counts(order=o)
+        assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
+    assert np.array_equal(a.row_or(), b.row_or())
+    rng = np.random.default_rng(S)
+    for row, col in zip(rng.integers(0, S, 40), rng.integers(0, g.V, 40)):
+        assert a.get(int(row), int(col)) == b.get(int(row), int(col))
+    bits = a.export()
+    r0 = int(np.flatnonzero(bits.any(axis=1))[0])
+    c0 = int(np.flatnonzero(np.unpackbits(bits[r0].view(np.uint8), bitorder="little"))[0])
+    assert a.get(r0, c0) and b.get(r0, c0)
+    for x in (a, b, sa, sb):
+        x.close()

@@ -153,6 +153,14 @@ struct vs_result_s {
   std::vector<vs::Mat> mats;
   std::vector<uint32_t> row_map;  // source vertex of each row
   std::vector<uint32_t> col_map;  // empty: column c is vertex c
+  // Row order of the stored matrices.  Empty: stored row r is caller row r.
+  // Otherwise the engine expanded the rows in an order of its own -- sorted
+  // within each tile of 64 * tw rows by the source's 2-hop volume -- and
+  // stored[r] is the stored row of caller row r (same tile).  Every output
+  // indexed by row (row counts, digests, exports, get, the flat table, plain
+  // TableConvert) is given in caller order; row-order-free outputs (counts,
+  // column counts, row-OR, column select) read the stored matrices as they are.
+  std::vector<uint32_t> stored;
 };
 
 namespace vs {
@@ -269,8 +277,50 @@ vs_status mat_row_counts(vs_graph_s *g, const Mat &m, uint32_t S, uint32_t V,
 vs_status mat_row_digests(vs_graph_s *g, const Mat &m, uint32_t S, uint32_t V,
                           uint32_t tw, uint32_t ntiles, uint64_t *d_digests);
 // native [tile][col][tw] -> one bitmap of V bits per row at out + row*pitch
-// (words nvb .. pitch-1 of each row are left as they are)
+// (words nvb .. pitch-1 of each row are left as they are); with d_dest, stored
+// row r goes to row d_dest[r] instead
 vs_status mat_to_rowmajor(vs_graph_s *g, const Mat &m, uint32_t S, uint32_t V,
-                          uint32_t tw, uint64_t *out, uint64_t pitch);
+                          uint32_t tw, uint64_t *out, uint64_t pitch,
+                          const uint32_t *d_dest = nullptr);
+// In place, within each tile of a [T][V][tw] matrix: word bit i of tile t
+// (row t * 64 tw + i) takes the bit of row d_unpack[t * 64 tw + i] of the
+// same tile.
+vs_status mat_unpack_rows(vs_graph_s *g, uint64_t *M, uint32_t V, uint32_t T, uint32_t tw,
+                          const uint16_t *d_unpack);
+
+// Matrix i of a result in caller row order: the stored matrix itself when
+// its rows are stored in that order, else a pooled copy with the rows put
+// back (returned to the pool by the destructor).
+class CallerOrder {
+ public:
+  CallerOrder() = default;
+  CallerOrder(const CallerOrder &) = delete;
+  CallerOrder &operator=(const CallerOrder &) = delete;
+  ~CallerOrder();
+  vs_status open(vs_result_s *r, uint32_t i);
+  const uint64_t *d = nullptr;
+
+ private:
+  vs_graph_s *g_ = nullptr;
+  void *tmp_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+// Device table of the stored -> caller row map of r (S uint32, pooled; the
+// destructor returns it), or nullptr when the rows are in caller order.
+class DestRows {
+ public:
+  DestRows() = default;
+  DestRows(const DestRows &) = delete;
+  DestRows &operator=(const DestRows &) = delete;
+  ~DestRows();
+  vs_status open(vs_result_s *r);
+  const uint32_t *d = nullptr;
+
+ private:
+  vs_graph_s *g_ = nullptr;
+  void *p_ = nullptr;
+  size_t bytes_ = 0;
+};
 
 } // namespace vs

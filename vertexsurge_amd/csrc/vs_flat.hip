@@ -176,8 +176,12 @@ extern "C" vs_status vs_result_flat(vs_result_t r, uint32_t *start, uint32_t *ta
   VS_HIP(hipSetDevice(g->device));
   hipStream_t s = g->stream;
   FlatArgs f{};
+  // the table is ordered by caller row: matrices stored in another row order
+  // are read through caller-ordered copies
+  CallerOrder co[kFlatMaxLayers];
   for (size_t i = 0; i < nm; ++i) {
-    f.m[i] = r->mats[i].d;
+    VS_TRY(co[i].open(r, static_cast<uint32_t>(i)));
+    f.m[i] = co[i].d;
     f.dist[i] = static_cast<int32_t>(r->mats[i].distance);
   }
   f.nm = static_cast<uint32_t>(nm);

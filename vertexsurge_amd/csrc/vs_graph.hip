@@ -379,7 +379,8 @@ template <int TW>
 __global__ __launch_bounds__(kBlock) void k_to_rowmajor(const uint64_t *__restrict__ M,
                                                        uint32_t V, uint32_t S,
                                                        uint32_t W, uint64_t *out,
-                                                       uint64_t pitch) {
+                                                       uint64_t pitch,
+                                                       const uint32_t *__restrict__ dest) {
   const int lane = lane_id(), wid = wave_id();
   const uint32_t nvb = (V + 63) / 64;
   const uint64_t total = static_cast<uint64_t>(W) * nvb;
@@ -396,9 +397,70 @@ __global__ __launch_bounds__(kBlock) void k_to_rowmajor(const uint64_t *__restri
       if (lane == b) mine = y;
     }
     const uint64_t row = static_cast<uint64_t>(w) * 64 + lane;
-    if (row < S) out[row * pitch + vb] = mine;
+    if (row < S) out[(dest ? dest[row] : row) * pitch + vb] = mine;
   }
 }
+
+// Row packing of the plain engine: the rows of a result are stored sorted,
+// within each tile (64 * TW rows), by the source's 2-hop volume, so rows
+// that reach the graph at a similar pace share words.  Reading a row-indexed
+// output puts them back into the caller's order on a copy: for caller row i
+// of tile t, unpack[t * 64 TW + i] is its stored row within the tile.  One
+// group of TW lanes per (tile, vertex): the wave stages its 64 words in LDS
+// and lane j rebuilds word j from 64 dword reads.  The LDS byte address of
+// every read (and the bit within the dword) is computed once per tile and
+// kept in registers for the whole loop, so no register that addresses an LDS
+// read in flight is rewritten while the read is pending.
+template <int TW>
+__global__ __launch_bounds__(kBlock) void k_unpack_rows(uint64_t *M, uint32_t V, uint32_t T,
+                                                       const uint16_t *__restrict__ unpack) {
+  constexpr int G = 64 / TW;
+  constexpr uint32_t RT = 64 * TW;
+  __shared__ uint32_t s_words[kWavesPerBlock][128];  // each wave's 64 words, as dwords
+  const int lane = lane_id(), wid = wave_id(), g = lane / TW, j = lane % TW;
+  char *const lds = reinterpret_cast<char *>(&s_words[0][0]);
+  const uint64_t vpt = (V + G - 1) / G;  // wave units per tile
+  const uint64_t units = vpt * T;
+  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
+  uint32_t cur = UINT32_MAX;
+  uint32_t addr[64];  // byte offset in s_words of the dword holding output bit b
+  uint32_t sh[16];    // bit of that dword, four per register
+  for (uint64_t u = blockIdx.x * static_cast<uint64_t>(kWavesPerBlock) + wid; u < units;
+       u += nwaves) {
+    const uint32_t t = static_cast<uint32_t>(u / vpt);
+    const uint32_t v = static_cast<uint32_t>(u % vpt) * G + g;
+    if (t != cur) {  // wave-uniform
+      const uint16_t *m = unpack + static_cast<size_t>(t) * RT + j * 64;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) sh[q] = 0;
+#pragma unroll
+      for (int b = 0; b < 64; ++b) {
+        const uint32_t i = m[b];  // packed row within the tile
+        const uint32_t dw = (g * TW + (i >> 6)) * 2 + ((i >> 5) & 1);
+        addr[b] = (static_cast<uint32_t>(wid) * 128 + dw) * 4;
+        sh[b >> 2] |= (i & 31) << (8 * (b & 3));
+      }
+      cur = t;
+    }
+    const bool ok = v < V;
+    uint64_t *p = M + (static_cast<size_t>(t) * V + v) * TW + j;
+    const uint64_t x = ok ? *p : 0;
+    if (__ballot(x != 0) == 0) continue;  // an all-zero wave unit stays as it is
+    reinterpret_cast<uint2 *>(s_words[wid])[lane] =
+        make_uint2(static_cast<uint32_t>(x), static_cast<uint32_t>(x >> 32));
+    wave_sync();
+    uint64_t out = 0;
+#pragma unroll
+    for (int b = 0; b < 64; ++b) {
+      const uint32_t d = *reinterpret_cast<const uint32_t *>(lds + addr[b]);
+      out |= static_cast<uint64_t>((d >> ((sh[b >> 2] >> (8 * (b & 3))) & 31)) & 1u) << b;
+    }
+    wave_sync();  // every read is done before the next unit's words land
+    if (ok) *p = out;
+  }
+}
+
+
 
 __global__ __launch_bounds__(kBlock) void k_copy_words(uint64_t *__restrict__ dst,
                                                       const uint64_t *__restrict__ src,
@@ -445,15 +507,76 @@ vs_status mat_row_digests(vs_graph_s *g, const Mat &m, uint32_t S, uint32_t V,
 }
 
 vs_status mat_to_rowmajor(vs_graph_s *g, const Mat &m, uint32_t S, uint32_t V,
-                          uint32_t tw, uint64_t *out, uint64_t pitch) {
+                          uint32_t tw, uint64_t *out, uint64_t pitch, const uint32_t *d_dest) {
   const uint32_t W = (S + 63) / 64;
   const uint64_t items = static_cast<uint64_t>(W) * ((V + 63) / 64);
   if (items == 0) return VS_OK;
   const uint32_t grid = grid_for(items, g->num_cus);
   VS_TW_DISPATCH(tw, k_to_rowmajor, dim3(grid), dim3(kBlock), 0, g->stream, m.d, V,
-                 S, W, out, pitch);
+                 S, W, out, pitch, d_dest);
   VS_HIP(hipGetLastError());
   return VS_OK;
+}
+
+vs_status mat_unpack_rows(vs_graph_s *g, uint64_t *M, uint32_t V, uint32_t T, uint32_t tw,
+                          const uint16_t *d_unpack) {
+  const uint64_t units = static_cast<uint64_t>(T) * ((V + 64 / tw - 1) / (64 / tw));
+  if (units == 0) return VS_OK;
+  VS_TW_DISPATCH(tw, k_unpack_rows, dim3(grid_for(units, g->num_cus)), dim3(kBlock), 0,
+                 g->stream, M, V, T, d_unpack);
+  VS_HIP(hipGetLastError());
+  return VS_OK;
+}
+
+vs_status CallerOrder::open(vs_result_s *r, uint32_t i) {
+  const Mat &m = r->mats[i];
+  d = m.d;
+  if (r->stored.empty() || !m.d || m.bytes == 0) return VS_OK;
+  g_ = r->g;
+  const uint32_t RT = 64 * r->tw;
+  const size_t nu = static_cast<size_t>(r->ntiles) * RT;
+  std::vector<uint16_t> unpack(nu);
+  for (size_t q = 0; q < nu; ++q) unpack[q] = static_cast<uint16_t>(q % RT);  // padding rows
+  for (uint32_t row = 0; row < r->S; ++row)
+    unpack[row] = static_cast<uint16_t>(r->stored[row] % RT);
+  void *du = nullptr;
+  VS_TRY(g_->pool.get(nu * 2, false, g_->stream, &du));
+  struct Back { Pool &p; void *q; size_t b; ~Back() { p.put(q, b, false); } } back{g_->pool, du, nu * 2};
+  VS_TRY(g_->pool.get(m.bytes, false, g_->stream, &tmp_));
+  bytes_ = m.bytes;
+  VS_HIP(hipMemcpyAsync(du, unpack.data(), nu * 2, hipMemcpyHostToDevice, g_->stream));
+  const size_t n = m.bytes / 8;
+  hipLaunchKernelGGL(k_copy_words, dim3(grid_for((n + 255) / 256, g_->num_cus)), dim3(kBlock), 0,
+                     g_->stream, static_cast<uint64_t *>(tmp_), m.d, n);
+  VS_HIP(hipGetLastError());
+  VS_TRY(mat_unpack_rows(g_, static_cast<uint64_t *>(tmp_), r->V, r->ntiles, r->tw,
+                         static_cast<const uint16_t *>(du)));
+  // the host table and the pooled one are released when this returns: wait
+  VS_HIP(hipStreamSynchronize(g_->stream));
+  d = static_cast<const uint64_t *>(tmp_);
+  return VS_OK;
+}
+
+CallerOrder::~CallerOrder() {
+  if (tmp_) g_->pool.put(tmp_, bytes_, false);
+}
+
+vs_status DestRows::open(vs_result_s *r) {
+  d = nullptr;
+  if (r->stored.empty() || r->S == 0) return VS_OK;
+  g_ = r->g;
+  std::vector<uint32_t> dest(r->S);
+  for (uint32_t row = 0; row < r->S; ++row) dest[r->stored[row]] = row;
+  bytes_ = static_cast<size_t>(r->S) * 4;
+  VS_TRY(g_->pool.get(bytes_, false, g_->stream, &p_));
+  VS_HIP(hipMemcpyAsync(p_, dest.data(), bytes_, hipMemcpyHostToDevice, g_->stream));
+  VS_HIP(hipStreamSynchronize(g_->stream));  // dest leaves scope
+  d = static_cast<const uint32_t *>(p_);
+  return VS_OK;
+}
+
+DestRows::~DestRows() {
+  if (p_) g_->pool.put(p_, bytes_, false);
 }
 
 } // namespace vs
@@ -730,8 +853,14 @@ static vs_status row_reduce(vs_result_t r, uint32_t i, uint64_t *host, bool dige
                                          static_cast<uint64_t *>(d))
                        : mat_row_counts(g, r->mats[i], r->S, r->V, r->tw, r->ntiles,
                                         static_cast<uint64_t *>(d));
+  std::vector<uint64_t> tmp;
+  uint64_t *dst = host;
+  if (!r->stored.empty()) {  // stored row order: read, then reorder
+    tmp.resize(r->S);
+    dst = tmp.data();
+  }
   if (s == VS_OK) {
-    hipError_t e = hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, g->stream);
+    hipError_t e = hipMemcpyAsync(dst, d, bytes, hipMemcpyDeviceToHost, g->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
     if (e != hipSuccess) {
       set_error("row readout: %s", hipGetErrorString(e));
@@ -739,6 +868,8 @@ static vs_status row_reduce(vs_result_t r, uint32_t i, uint64_t *host, bool dige
     }
   }
   g->pool.put(d, bytes, false);
+  if (s == VS_OK && !tmp.empty())
+    for (uint32_t row = 0; row < r->S; ++row) host[row] = tmp[r->stored[row]];
   return s;
 }
 
@@ -797,9 +928,15 @@ vs_status vs_result_export(vs_result_t r, uint32_t i, int32_t layout,
   vs_graph_s *g = r->g;
   VS_HIP(hipSetDevice(g->device));
   const Mat &m = r->mats[i];
+  // word layouts need the rows in caller order; the row-major one writes
+  // every row to its caller position directly
+  CallerOrder co;
+  DestRows dr;
+  if (layout == VS_LAYOUT_ROWMAJOR) VS_TRY(dr.open(r));
+  else VS_TRY(co.open(r, i));
   if (layout == VS_LAYOUT_NATIVE || (layout == VS_LAYOUT_DVSM && r->tw == 16 &&
                                      m.bytes == need)) {
-    VS_HIP(hipMemcpyAsync(host_buf, m.d, need, hipMemcpyDeviceToHost, g->stream));
+    VS_HIP(hipMemcpyAsync(host_buf, co.d, need, hipMemcpyDeviceToHost, g->stream));
     VS_HIP(hipStreamSynchronize(g->stream));
     return VS_OK;
   }
@@ -809,11 +946,11 @@ vs_status vs_result_export(vs_result_t r, uint32_t i, int32_t layout,
   if (layout == VS_LAYOUT_DVSM) {
     const size_t nout = need / 8;
     const uint32_t grid = grid_for((nout + 63) / 64, g->num_cus);
-    VS_TW_DISPATCH(r->tw, k_to_dvsm, dim3(grid), dim3(kBlock), 0, g->stream, m.d,
+    VS_TW_DISPATCH(r->tw, k_to_dvsm, dim3(grid), dim3(kBlock), 0, g->stream, co.d,
                    r->V, W, static_cast<uint64_t *>(d), nout);
   } else {
     const vs_status st = mat_to_rowmajor(g, m, r->S, r->V, r->tw,
-                                         static_cast<uint64_t *>(d), (r->V + 63) / 64);
+                                         static_cast<uint64_t *>(d), (r->V + 63) / 64, dr.d);
     if (st != VS_OK) {
       g->pool.put(d, need, false);
       return st;
@@ -838,13 +975,14 @@ vs_status vs_result_get(vs_result_t r, uint32_t i, uint32_t row, uint32_t col,
     set_error("vs_result_get: (%u,%u) outside %ux%u", row, col, r->S, r->V);
     return VS_ERR_INVALID;
   }
-  const uint32_t w = row / 64, t = w / r->tw, j = w % r->tw;
+  const uint32_t sr = r->stored.empty() ? row : r->stored[row];  // stored row
+  const uint32_t w = sr / 64, t = w / r->tw, j = w % r->tw;
   const size_t idx = (static_cast<size_t>(t) * r->V + col) * r->tw + j;
   uint64_t word = 0;
   VS_HIP(hipSetDevice(r->g->device));
   VS_HIP(hipStreamSynchronize(r->g->stream));
   VS_HIP(hipMemcpy(&word, r->mats[i].d + idx, 8, hipMemcpyDeviceToHost));
-  *bit = static_cast<int>((word >> (row % 64)) & 1);
+  *bit = static_cast<int>((word >> (sr % 64)) & 1);
   return VS_OK;
 } VS_CATCH("vs_result_get")
 

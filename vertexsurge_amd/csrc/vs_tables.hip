@@ -756,10 +756,12 @@ vs_status vs_table_convert(vs_result_t r, uint32_t i, int transpose, vs_colmat_t
     VS_TRY(colmat_new(g, r->S, r->V, false, m));
     const uint64_t n = static_cast<uint64_t>(r->V) * m->pitch;
     if (n && r->S && mat.d) {
+      CallerOrder co;  // ColMain rows are the caller's rows, in order
+      VS_TRY(co.open(r, i));
       const uint32_t W = (r->S + 63) / 64;
       VS_TW_DISPATCH_T(r->tw, k_cm_from_native,
                        dim3(grid_for((n + 255) / 256, g->num_cus)), dim3(kBlock), 0, g->stream,
-                       mat.d, r->V, W, m->pitch, m->d);
+                       co.d, r->V, W, m->pitch, m->d);
       VS_HIP(hipGetLastError());
     } else if (n) {
       VS_HIP(hipMemsetAsync(m->d, 0, n * 8, g->stream));
@@ -768,7 +770,11 @@ vs_status vs_table_convert(vs_result_t r, uint32_t i, int transpose, vs_colmat_t
     m->col_map = r->col_map;
   } else {
     VS_TRY(colmat_new(g, r->V, r->S, true, m));
-    if (r->S && r->V && mat.d) VS_TRY(mat_to_rowmajor(g, mat, r->S, r->V, r->tw, m->d, m->pitch));
+    if (r->S && r->V && mat.d) {
+      DestRows dr;  // column k is caller row k
+      VS_TRY(dr.open(r));
+      VS_TRY(mat_to_rowmajor(g, mat, r->S, r->V, r->tw, m->d, m->pitch, dr.d));
+    }
     m->row_map = r->col_map;
     m->col_map = r->row_map;
   }
@@ -834,6 +840,7 @@ vs_status vs_result_select_cols(vs_result_t r, uint32_t i, const uint32_t *col_i
   res->ntiles = r->ntiles;
   res->k_actual_max = r->k_actual_max;
   res->row_map = r->row_map;
+  res->stored = r->stored;  // columns are selected, rows stay where they are
   res->col_map.assign(col_ids, col_ids + n);
   res->mats.push_back(m);
   g->refs++;

@@ -148,77 +148,6 @@ void launch_clear(uint32_t tw, uint32_t grid, hipStream_t s, uint64_t *F, uint64
   VS_TW_CALL(tw, launch_clear_tw, grid, s, F, Fm, bm, V);
 }
 
-// Row packing.  A call expands its rows in an order of its own: within each
-// tile (64 * TW rows) the sources are sorted by 2-hop volume, descending, as
-// the batched engine packs its rows, so rows that reach the graph at a
-// similar pace share words (a word whose rows are all visited drops out of
-// the pull's needed words, and the sparse pull sees fewer words per vertex).
-// The result matrices are then put back into the caller's row order in
-// place: for caller row i of tile t, unpack[t * 64 TW + i] is its packed row
-// within the tile.  One group of TW lanes per (tile, vertex): the wave stages
-// its 64 words in LDS and lane j rebuilds word j from 64 dword reads.  The
-// LDS byte address of every read (and the bit within the dword) is computed
-// once per tile and kept in registers for the whole loop, so no register that
-// addresses an LDS read in flight is rewritten while the read is pending.
-template <int TW>
-__global__ __launch_bounds__(kBlock) void k_unpack_rows(uint64_t *M, uint32_t V, uint32_t T,
-                                                       const uint16_t *__restrict__ unpack) {
-  constexpr int G = 64 / TW;
-  constexpr uint32_t RT = 64 * TW;
-  __shared__ uint32_t s_words[kWavesPerBlock][128];  // each wave's 64 words, as dwords
-  const int lane = lane_id(), wid = wave_id(), g = lane / TW, j = lane % TW;
-  char *const lds = reinterpret_cast<char *>(&s_words[0][0]);
-  const uint64_t vpt = (V + G - 1) / G;  // wave units per tile
-  const uint64_t units = vpt * T;
-  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
-  uint32_t cur = UINT32_MAX;
-  uint32_t addr[64];  // byte offset in s_words of the dword holding output bit b
-  uint32_t sh[16];    // bit of that dword, four per register
-  for (uint64_t u = blockIdx.x * static_cast<uint64_t>(kWavesPerBlock) + wid; u < units;
-       u += nwaves) {
-    const uint32_t t = static_cast<uint32_t>(u / vpt);
-    const uint32_t v = static_cast<uint32_t>(u % vpt) * G + g;
-    if (t != cur) {  // wave-uniform
-      const uint16_t *m = unpack + static_cast<size_t>(t) * RT + j * 64;
-#pragma unroll
-      for (int q = 0; q < 16; ++q) sh[q] = 0;
-#pragma unroll
-      for (int b = 0; b < 64; ++b) {
-        const uint32_t i = m[b];  // packed row within the tile
-        const uint32_t dw = (g * TW + (i >> 6)) * 2 + ((i >> 5) & 1);
-        addr[b] = (static_cast<uint32_t>(wid) * 128 + dw) * 4;
-        sh[b >> 2] |= (i & 31) << (8 * (b & 3));
-      }
-      cur = t;
-    }
-    const bool ok = v < V;
-    uint64_t *p = M + (static_cast<size_t>(t) * V + v) * TW + j;
-    const uint64_t x = ok ? *p : 0;
-    if (__ballot(x != 0) == 0) continue;  // an all-zero wave unit stays as it is
-    reinterpret_cast<uint2 *>(s_words[wid])[lane] =
-        make_uint2(static_cast<uint32_t>(x), static_cast<uint32_t>(x >> 32));
-    wave_sync();
-    uint64_t out = 0;
-#pragma unroll
-    for (int b = 0; b < 64; ++b) {
-      const uint32_t d = *reinterpret_cast<const uint32_t *>(lds + addr[b]);
-      out |= static_cast<uint64_t>((d >> ((sh[b >> 2] >> (8 * (b & 3))) & 31)) & 1u) << b;
-    }
-    wave_sync();  // every read is done before the next unit's words land
-    if (ok) *p = out;
-  }
-}
-
-template <int TW>
-static void launch_unpack_tw(uint32_t grid, hipStream_t s, uint64_t *M, uint32_t V, uint32_t T,
-                             const uint16_t *unpack) {
-  hipLaunchKernelGGL((k_unpack_rows<TW>), dim3(grid), dim3(kBlock), 0, s, M, V, T, unpack);
-}
-
-static void launch_unpack(uint32_t tw, uint32_t grid, hipStream_t s, uint64_t *M, uint32_t V,
-                          uint32_t T, const uint16_t *unpack) {
-  VS_TW_CALL(tw, launch_unpack_tw, grid, s, M, V, T, unpack);
-}
 
 void launch_items(uint32_t grid, hipStream_t s, const uint64_t *bm, uint32_t V,
                   const uint64_t *xoff, ull *items, ull *nitems) {
@@ -373,17 +302,17 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
     st->tile_words = TW;
     st->num_tiles = T;
   }
-  // Row packing (k_unpack_rows): the rows run in packed order, sources and
-  // targets permuted to match; pos[r] is caller row r's packed row.  Packing
-  // pays from hop 2 on (fewer needed words per pull), restoring the order
-  // costs about one pass over each result matrix: used from k_max = 3, or 2
-  // for FindShortest, whose per-row distances are permuted on the host.
+  // Row packing: the rows run in packed order, sources and targets permuted
+  // to match; pos[r] is caller row r's packed row (same tile).  The result
+  // keeps the packed order and pos as its stored-row map, which the
+  // row-indexed outputs apply when they are read.  Packing pays from hop 2
+  // on (fewer needed words per pull): used from k_max = 3, or 2 for
+  // FindShortest, whose per-row distances are permuted on the host.
   const uint32_t *const caller_sources = sources;
   const bool pack = !(p.flags & VS_VX_KEEP_ORDER) && W > 1 &&
                     k_max >= (find_shortest ? 2u : 3u);
   const uint32_t RT = 64 * TW;
   std::vector<uint32_t> pk_src, pk_tgt, pos;
-  std::vector<uint16_t> unpack;
   if (pack) {
     std::vector<uint64_t> key(S);
     VS_TRY(source_volumes(g, push, sources, S, key.data()));
@@ -405,10 +334,6 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
       for (uint32_t i = 0; i < S; ++i) pk_tgt[i] = targets[order[i]];
       targets = pk_tgt.data();
     }
-    // rows past S (the last tile's padding) are zero and map to themselves
-    unpack.resize(static_cast<size_t>(T) * RT);
-    for (size_t r = 0; r < unpack.size(); ++r)
-      unpack[r] = static_cast<uint16_t>((r < S ? pos[r] : r) % RT);
     sources = pk_src.data();
   }
   std::unique_ptr<vs_result_s> res(new vs_result_s);
@@ -803,18 +728,7 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
     lease.mark(touched, true);
   }
   if (Vis && res->mats.empty()) lease.mark(Vis, false);
-  if (pack && !res->mats.empty()) {  // back to the caller's row order
-    void *du = nullptr;
-    VS_TRY(lease.get(unpack.size() * 2, false, &du));
-    VS_HIP(hipMemcpyAsync(du, unpack.data(), unpack.size() * 2, hipMemcpyHostToDevice, s));
-    const uint64_t units = static_cast<uint64_t>(T) * ((V + 64 / TW - 1) / (64 / TW));
-    const uint32_t grid = grid_for(units, g->num_cus);
-    for (const Mat &m : res->mats) {
-      if (!m.d) continue;
-      launch_unpack(TW, grid, s, m.d, V, T, static_cast<const uint16_t *>(du));
-      VS_HIP(hipGetLastError());
-    }
-  }
+  if (pack) res->stored = pos;  // the matrices stay in packed row order
   VS_HIP(hipEventRecord(ev1, s));
   VS_HIP(hipStreamSynchronize(s));
   if (find_shortest && pack)
