@@ -374,6 +374,23 @@ vs_status vs_vexpand_batch(vs_graph_t g, const vs_query *queries, uint32_t nq,
                            const vs_batch_params *p, uint64_t *row_counts,
                            uint64_t *row_digests, vs_batch_stats *stats);
 
+/* The same batch, returning one result per query instead of per-row counts:
+ * results[q] is what vs_vexpand returns for query q (VExpand::run_internal's
+ * merge_to_MT, vexpand.cpp:298-354: a VarRowMatrixTable whose rows are the
+ * query's sources in its order, the merged matrix of its hop range, and
+ * k_actual_max as VExpand::after_child sets it, vexpand.cpp:222-233), to
+ * be freed with vs_result_free.  Rows are packed per query: each query's rows
+ * start at a 64-row word boundary and are sorted by 2-hop volume within each
+ * tile of its own, so a query owns whole words; a one-query batch expands
+ * straight into its result matrix, a larger batch copies every query's words
+ * into its matrix at the end of each wave.  All results stay in device memory
+ * (sum over queries of ceil(S_q / (64 tile_words)) * V * tile_words * 8
+ * bytes): VS_ERR_NOMEM when they do not fit next to the batch's working
+ * tiles.  On failure no result is returned. */
+vs_status vs_vexpand_batch_results(vs_graph_t g, const vs_query *queries, uint32_t nq,
+                                   const vs_batch_params *p, vs_result_t *results,
+                                   vs_batch_stats *stats);
+
 /* ------------------------------------------------------------- Mintersect */
 typedef struct {
   double ms;            /* device time of the call's kernels (split to last copy) */

@@ -131,3 +131,87 @@ def test_scheduler_handle_freed_while_attached(orc, g14):
         r.close()
     finally:
         G.set_scheduler(None)
+
+
+def _same_result(a, b, V):
+    """Two VExpand results hold the same rows, bit for bit, in every output."""
+    assert (a.rows, a.cols, a.num_matrices) == (b.rows, b.cols, b.num_matrices)
+    assert a.k_actual_max == b.k_actual_max
+    if a.rows == 0:
+        return
+    assert np.array_equal(a.export(), b.export())
+    assert np.array_equal(a.export(0, vs.LAYOUT_DVSM), b.export(0, vs.LAYOUT_DVSM))
+    assert np.array_equal(a.row_counts(), b.row_counts())
+    assert np.array_equal(a.row_digests(), b.row_digests())
+    assert a.tuple_count() == b.tuple_count()
+    sa, ta = a.to_flat()
+    sb, tb = b.to_flat()
+    assert np.array_equal(sa, sb) and np.array_equal(ta, tb)
+
+
+@pytest.mark.parametrize("mode", ["one_query", "many_queries", "many_waves"])
+@pytest.mark.parametrize("path,kmins", [("shortest", (0, 1)), ("shortest", (0, 1, 2, 3)),
+                                        ("any", (0, 1, 2))])
+def test_batch_results_equal_plain(orc, g14, mode, path, kmins):
+    g, G = g14
+    if mode == "one_query":
+        src = vs.sample_sources(g.V, 1500, seed=3).astype(np.uint32)
+        src[::50] = src[0]  # duplicate rows
+        qs = [vs.BatchQuery(src, kmins[-1], 4, path)]
+    else:
+        qs = make_queries(g.V, 9, seed=21, smax=900, kmax=(1, 5), kmins=kmins)
+        for q in qs:
+            q.path = path
+        qs.insert(3, vs.BatchQuery(np.zeros(0, dtype=np.uint32), 1, 3, path))  # empty query
+    kw = dict(tile_words=8, mem_budget=1) if mode == "many_waves" else dict(tile_words=16)
+    res, st = vs.vexpand_batch_results(G, qs, **kw)
+    assert len(res) == len(qs) and st["num_tiles"] >= 1
+    if mode == "many_waves":
+        assert st["num_waves"] > 1
+    counts, digests, _ = vs.vexpand_batch(G, qs, **kw)
+    row = 0
+    for q, r in zip(qs, res):
+        p, _ = vs.vexpand(G, q.sources, q.k_min, q.k_max, q.path, tile_words=16)
+        _same_result(r, p, g.V)
+        n = len(q.sources)
+        if n:
+            assert np.array_equal(r.row_counts(), counts[row:row + n])
+            assert np.array_equal(r.row_digests(), digests[row:row + n])
+            mats, ka, _, _ = orc.vexpand_rows(g.offsets, g.adj, q.sources, q.k_min, q.k_max,
+                                              q.path)
+            assert r.k_actual_max == ka
+            assert np.array_equal(r.export(), mats[0])
+        row += n
+        p.close()
+        r.close()
+
+
+def test_batch_results_table_operators(g14):
+    g, G = g14
+    qs = make_queries(g.V, 5, seed=77, smax=600, kmax=(2, 4), kmins=(1,))
+    res, _ = vs.vexpand_batch_results(G, qs, tile_words=8)
+    for q, r in zip(qs, res):
+        p, _ = vs.vexpand(G, q.sources, q.k_min, q.k_max, q.path, tile_words=8,
+                          keep_order=True)
+        for transpose in (False, True):
+            a, b = r.to_colmain(transpose=transpose), p.to_colmain(transpose=transpose)
+            assert np.array_equal(a.export(), b.export())
+            a.close()
+            b.close()
+        x, y = r.col_counts(order="desc"), p.col_counts(order="desc")
+        assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1])
+        assert np.array_equal(r.row_or(), p.row_or())
+        r.close()
+        p.close()
+
+
+def test_batch_results_replayed_graph_and_learned(g14):
+    # several replays of the captured graph rewrite the same result matrices
+    g, G = g14
+    qs = make_queries(g.V, 6, seed=13, smax=2000, kmax=(2, 5))
+    r1, _ = vs.vexpand_batch_results(G, qs, replays=3, use_graph=True)
+    r2, _ = vs.vexpand_batch_results(G, qs, use_graph=False, sched="pull")
+    for a, b in zip(r1, r2):
+        _same_result(a, b, g.V)
+        a.close()
+        b.close()

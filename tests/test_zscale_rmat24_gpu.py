@@ -98,6 +98,16 @@ def test_c2_rmat24_s4096_every_row_vs_oracle_b(orc, rmat24, blocks24):
         assert np.array_equal(c, counts) and np.array_equal(dg, digests), sched
         assert bst["te"] == te, sched
     log("batched engine: counts, digests and TE exact")
+    # the batched engine's result matrix (expanded in place, rows packed)
+    (r,), bst = vs.vexpand_batch_results(G, [vs.BatchQuery(src, 1, 4)])
+    assert r.k_actual_max == ka and bst["te"] == te
+    assert np.array_equal(r.row_counts(0), counts)
+    assert np.array_equal(r.row_digests(0), digests)
+    got = r.export(0, _lib.LAYOUT_DVSM)
+    r.close()
+    assert np.array_equal(got, dv), _first_diff(got, dv)
+    del got
+    log("batched engine: result matrix exact")
 
 
 def test_c2_rmat24_any_kmin2_vs_oracle_b(orc, rmat24, blocks24):

@@ -19,7 +19,8 @@ from ._lib import (DIR_BACKWARD, DIR_BOTH, DIR_FORWARD, LAYOUT_DVSM, LAYOUT_NATI
 __all__ = [
     "HostCSR", "Graph", "GraphStore", "store_write", "VExpandResult", "vexpand", "find_shortest", "mintersect",
     "ColMat", "mintersect_colmat", "pack_colmain", "unpack_colmain",
-    "sample_sources", "sample_paths3", "Scheduler", "vexpand_batch", "device_count",
+    "sample_sources", "sample_paths3", "Scheduler", "vexpand_batch",
+    "vexpand_batch_results", "BatchQuery", "device_count",
     "PATH_ANY", "PATH_SHORTEST", "PATH_FIND_SHORTEST", "DIR_FORWARD", "DIR_BACKWARD",
     "DIR_BOTH", "SCHED_HEURISTIC", "SCHED_PUSH", "SCHED_PULL", "SCHED_LEARNED",
     "LAYOUT_NATIVE", "LAYOUT_DVSM", "LAYOUT_ROWMAJOR", "VSError",
@@ -641,6 +642,48 @@ class BatchQuery:
     k_min: int
     k_max: int
     path: str = "shortest"
+
+
+def _batch_args(queries, tile_words, sched, use_graph, replays, mem_budget, direction, timing,
+                keep_order):
+    keep = [_u32(q.sources) for q in queries]
+    arr = (_lib.Query * len(queries))()
+    for i, q in enumerate(queries):
+        arr[i].sources = _ptr(keep[i], C.c_uint32)
+        arr[i].S = len(keep[i])
+        arr[i].k_min = q.k_min
+        arr[i].k_max = q.k_max
+        arr[i].path_type = _PATHS[q.path]
+    p = _lib.BatchParams(tile_words, _SCHED[sched], int(use_graph), replays, mem_budget,
+                         _DIRS[direction], (_lib.VX_TIMING if timing else 0) |
+                         (_lib.BATCH_KEEP_ORDER if keep_order else 0))
+    return keep, arr, p
+
+
+def _batch_stats(st) -> dict:
+    return {"te": st.te, "ms_total": st.ms_total, "ms_capture": st.ms_capture,
+            "num_tiles": st.num_tiles, "num_waves": st.num_waves, "launches": st.launches,
+            "push_decisions": st.push_decisions, "pull_decisions": st.pull_decisions,
+            "ms_expand": st.ms_expand, "n_expand": st.n_expand, "bytes_alg": st.bytes_alg,
+            "ms_wall": st.ms_wall, "ms_expand_sparse": st.ms_expand_sparse,
+            "n_expand_sparse": st.n_expand_sparse, "bytes_sparse": st.bytes_sparse}
+
+
+def vexpand_batch_results(graph: Graph, queries: list, tile_words: int = 0,
+                          sched: str = "heuristic", use_graph: bool = True, replays: int = 1,
+                          mem_budget: int = 0, direction: str = "forward",
+                          timing: bool = False, keep_order: bool = False):
+    """Runs many VExpand queries packed into shared tiles and returns
+    ([VExpandResult per query], stats): each result is what vexpand() returns
+    for that query (rows = its sources in its order)."""
+    keep, arr, p = _batch_args(queries, tile_words, sched, use_graph, replays, mem_budget,
+                               direction, timing, keep_order)
+    hs = (C.c_void_p * max(1, len(queries)))()
+    st = _lib.BatchStats()
+    _lib.call("vs_vexpand_batch_results", graph.handle, arr, len(queries), C.byref(p), hs,
+              C.byref(st))
+    return [VExpandResult(C.c_void_p(hs[i]), keep[i]) for i in range(len(queries))], \
+        _batch_stats(st)
 
 
 def vexpand_batch(graph: Graph, queries: list, tile_words: int = 0, sched: str = "heuristic",

@@ -150,6 +150,9 @@ SIGNATURES = {
     "vs_result_free": (C.c_int, [vp]),
     "vs_vexpand_batch": (C.c_int, [vp, C.POINTER(Query), C.c_uint32, C.POINTER(BatchParams),
                                    u64p, u64p, C.POINTER(BatchStats)]),
+    "vs_vexpand_batch_results": (C.c_int, [vp, C.POINTER(Query), C.c_uint32,
+                                           C.POINTER(BatchParams), C.POINTER(vp),
+                                           C.POINTER(BatchStats)]),
     "vs_mintersect": (C.c_int, [C.POINTER(vp), C.c_uint32, u32p, C.c_uint64, u32p, u64p, u32p,
                                 C.c_uint64, u64p, C.POINTER(MintersectStats)]),
     "vs_mintersect_device": (C.c_int, [C.POINTER(vp), C.c_uint32, vp, C.c_uint64, vp, vp, vp,

@@ -113,7 +113,24 @@ struct BatchConst {
   uint32_t has_heavy;     // some in-list spans several pull items
   uint32_t nhwords;       // bitmap words holding such vertices
   const uint32_t *hwords;
+  // Result matrices (vs_vexpand_batch_results), all nullptr otherwise:
+  //  res_base  a one-query group expands straight into its result: tile t's
+  //            result words (R, or Vis when no R is kept) live at
+  //            res_base + t * V * TW
+  //  wdest     a multi-query group copies each result word of global word w
+  //            to query wdest[w].x's matrix qbase[x], at its word wdest[w].y
+  //            (x == UINT32_MAX: a padding word)
+  //  tile_front  per tile, hop and word: the OR of the rows of L_h (a row's
+  //            bit is set when the row gained a vertex at hop h)
+  uint64_t *res_base;
+  const uint2 *wdest;
+  uint64_t *const *qbase;
+  uint64_t *tile_front;   // [ntiles][H + 1][TW]
+  uint32_t res_sub;       // results: source bits of sub_src rows are cleared
+                          // in the matrix, so counts and digests need no correction
 };
+
+constexpr uint32_t kNoRow = 0xFFFFFFFFu;  // src of a padding row (no source)
 
 __device__ __forceinline__ HopArgs slot_args(const BatchConst &c, const SlotDesc &sd,
                                              uint32_t hop) {
@@ -216,6 +233,11 @@ __global__ void k_bw_assign(BatchConst c) {
     const uint32_t t = wave * c.nslots + s;
     c.slots[s].tile = t < c.ntiles ? t : UINT32_MAX;
     c.slots[s].mode = 0;
+    if (c.res_base && t < c.ntiles) {
+      uint64_t *m = c.res_base + static_cast<size_t>(t) * c.V * c.TW;
+      if (c.use_r) c.slots[s].R = m;
+      else c.slots[s].Vis = m;
+    }
   }
   for (uint32_t i = threadIdx.x; i < c.nslots * (c.H + 1); i += blockDim.x) {
     HopCounters *h = &c.slots[i / (c.H + 1)].ctr[i % (c.H + 1)];
@@ -259,6 +281,7 @@ __global__ void k_bw_l0(BatchConst c) {
     if (lr >= ti.rows) continue;
     const uint32_t row = ti.row0 + lr;
     const uint32_t v = c.src[row];
+    if (v == kNoRow) continue;  // padding between queries' words
     const uint32_t j = lr / 64;
     const ull bit = 1ull << (lr % 64);
     const size_t wi = static_cast<size_t>(v) * c.TW + j;
@@ -650,6 +673,17 @@ __global__ void k_bw_finish(BatchConst c) {
           *pp = HopCounters{};
         }
     __syncthreads();
+    if (c.tile_front) {
+      const uint32_t n = c.nslots * (c.H + 1) * c.TW;
+      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint32_t sl = i / ((c.H + 1) * c.TW), rem = i % ((c.H + 1) * c.TW);
+        const SlotDesc &sd = c.slots[sl];
+        if (sd.tile == UINT32_MAX) continue;
+        const uint32_t h = rem / c.TW, j = rem % c.TW;
+        c.tile_front[(static_cast<size_t>(sd.tile) * (c.H + 1) + h) * c.TW + j] =
+            sd.front[static_cast<size_t>(h) * 64 + j];
+      }
+    }
   }
   const uint32_t rows_per_tile = c.TW * 64;
   const uint32_t total = c.nslots * rows_per_tile;
@@ -661,7 +695,7 @@ __global__ void k_bw_finish(BatchConst c) {
     const uint32_t lr = i % rows_per_tile;
     if (lr >= ti.rows) continue;
     const uint32_t row = ti.row0 + lr;
-    if (!c.use_r && c.sub_src[row]) {
+    if (!c.use_r && !c.res_sub && c.sub_src[row]) {
       atomicAdd(&c.out_counts[row], ~0ull);  // minus one
       if (c.out_digests) atomicAdd(&c.out_digests[row], 0ull - digest_hash(c.src[row]));
     }
@@ -695,6 +729,42 @@ __global__ void k_bw_finish(BatchConst c) {
     c.ctl->te += te;
     c.ctl->bytes += bytes;
     c.ctl->bytes_sparse += bytes_sparse;
+  }
+}
+
+// ---- results: the result of a Shortest row with k_min == 1 is its visited
+// set minus its source (vexpand.cpp:306-310 keeps the source for k_min = 0)
+__global__ void k_bw_subsrc(BatchConst c) {
+  const uint32_t rows_per_tile = c.TW * 64;
+  const uint32_t total = c.nslots * rows_per_tile;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += gridDim.x * blockDim.x) {
+    const SlotDesc &sd = c.slots[i / rows_per_tile];
+    if (sd.tile == UINT32_MAX) continue;
+    const TileInfo ti = c.tiles[sd.tile];
+    const uint32_t lr = i % rows_per_tile;
+    if (lr >= ti.rows || !c.sub_src[ti.row0 + lr]) continue;
+    const size_t wi = static_cast<size_t>(c.src[ti.row0 + lr]) * c.TW + lr / 64;
+    atomicAnd(reinterpret_cast<ull *>(&sd.Vis[wi]), ~(1ull << (lr % 64)));
+  }
+}
+
+// ---- results of a multi-query group: every result word of the wave's tiles
+// goes to its query's matrix (queries own whole words)
+template <int TW>
+__global__ __launch_bounds__(kBlock) void k_bw_export(BatchConst c) {
+  const uint64_t per = static_cast<uint64_t>(c.V) * TW;
+  const uint64_t total = static_cast<uint64_t>(c.nslots) * per;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < total;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const SlotDesc &sd = c.slots[i / per];
+    if (sd.tile == UINT32_MAX) continue;
+    const uint64_t k = i % per;  // vertex * TW + j
+    const uint32_t v = static_cast<uint32_t>(k / TW), j = static_cast<uint32_t>(k % TW);
+    const uint2 d = c.wdest[static_cast<size_t>(sd.tile) * TW + j];
+    if (d.x == UINT32_MAX) continue;
+    const uint64_t *M = c.use_r ? sd.R : sd.Vis;
+    c.qbase[d.x][(static_cast<size_t>(d.y / TW) * c.V + v) * TW + d.y % TW] = M[k];
   }
 }
 
@@ -772,9 +842,13 @@ static void enqueue_hop_tw(const BatchConst &c, bool shortest, uint32_t hop,
 template <int TW>
 static void enqueue_final_tw(const BatchConst &c, uint32_t grid, hipStream_t s) {
   hipLaunchKernelGGL((k_bw_clear<TW>), dim3(grid), dim3(kBlock), 0, s, c, c.H);
+  if (c.res_sub)
+    hipLaunchKernelGGL(k_bw_subsrc, dim3(std::max<uint32_t>(1, (c.nslots * TW * 64 + 255) / 256)),
+                       dim3(256), 0, s, c);
   // half the usual grid: each block accumulates two chunk groups in LDS
   hipLaunchKernelGGL((k_bw_rows<TW>), dim3(std::max<uint32_t>(1, grid / 2)), dim3(kBlock), 0, s,
                      c);
+  if (c.wdest) hipLaunchKernelGGL((k_bw_export<TW>), dim3(grid), dim3(kBlock), 0, s, c);
 }
 
 // Enqueues the whole launch sequence of one wave; returns the number of
@@ -801,7 +875,7 @@ static uint32_t enqueue_wave(const BatchConst &c, bool shortest, uint32_t grid,
   hipLaunchKernelGGL(k_bw_finish, dim3(std::max<uint32_t>(1, (c.nslots * c.TW * 64 + 255) / 256)),
                      dim3(256), 0, s, c);
   hipLaunchKernelGGL(k_bw_advance, dim3(1), dim3(64), 0, s, c.ctl);
-  n += 4;
+  n += 4 + (c.res_sub ? 1 : 0) + (c.wdest ? 1 : 0);
   return n;
 }
 
@@ -880,11 +954,25 @@ vs_status source_volumes(vs_graph_s *g, const DevCSR *push, const uint32_t *src,
   return st;
 }
 
-// One group = queries of one path type.
+// Result matrices of a group's queries (vs_vexpand_batch_results), owned
+// until handed out: returned to the pool otherwise.
+struct GroupResults {
+  vs_graph_s *g = nullptr;
+  std::vector<vs_result_s *> res;  // per query of the group, in group order
+  std::vector<Mat> mats;           // their matrices while not handed out
+  ~GroupResults() {
+    for (auto &m : mats)
+      if (m.d) g->pool.put(m.d, m.bytes, false);
+    for (auto *r : res) delete r;
+  }
+};
+
+// One group = queries of one path type.  With ro, every query also gets its
+// result matrix (rows packed per query; see the packing comment).
 static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &qs,
                            const vs_batch_params &p, bool shortest,
                            std::vector<uint64_t> &counts, std::vector<uint64_t> &digests,
-                           bool want_digests, vs_batch_stats &st) {
+                           bool want_digests, vs_batch_stats &st, GroupResults *ro = nullptr) {
   const DevCSR *push = nullptr, *pull = nullptr;
   VS_TRY(graph_csrs(g, p.direction, &push, &pull));
   const uint32_t V = g->V, nvb = (V + 63) / 64;
@@ -896,13 +984,62 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
     H = std::max(H, q->k_max);
     if (q->k_min >= 2) use_r = true;
   }
-  if (S64 == 0) return VS_OK;
-  if (S64 > 0xFFFFFFF0ull) { set_error("batch: too many rows"); return VS_ERR_INVALID; }
+  const bool results = ro != nullptr;
+  const uint32_t nq = static_cast<uint32_t>(qs.size());
+  if (results) {  // empty queries get empty results even when no row runs
+    ro->g = g;
+    ro->res.assign(nq, nullptr);
+    ro->mats.assign(nq, Mat{});
+  }
+  // packed row slots: with results every query starts at a word boundary,
+  // so a query owns whole words (the rest of its last word is padding)
+  uint64_t W64 = 0;
+  for (auto *q : qs) W64 += results ? (q->S + 63) / 64 : 0;
+  if (!results) W64 = (S64 + 63) / 64;
+  if (S64 > 0xFFFFFFF0ull || W64 * 64 > 0xFFFFFFF0ull) {
+    set_error("batch: too many rows");
+    return VS_ERR_INVALID;
+  }
   const uint32_t S = static_cast<uint32_t>(S64);
-  const uint32_t W = (S + 63) / 64;
+  const uint32_t W = static_cast<uint32_t>(W64);
+  const uint32_t Sp = results ? W * 64 : S;  // >= S
   const uint32_t TW = choose_tile_words(p.tile_words ? p.tile_words : g->tuning.tile_words, W);
-  const uint32_t T = (W + TW - 1) / TW;
+  const uint32_t T = W ? (W + TW - 1) / TW : 0;
   const uint32_t Wpad = T * TW;
+  const uint32_t RT = 64 * TW;
+  const size_t mat = static_cast<size_t>(V) * TW * 8;
+  // one query: its result tiles are the batch's tiles, expanded in place
+  const bool zero_copy = results && nq == 1;
+  std::vector<std::vector<uint32_t>> stored;  // per query: caller row -> stored row
+  std::vector<uint32_t> qword0;               // per query: its first packed word
+  if (results) {
+    stored.resize(nq);
+    qword0.assign(nq, 0);
+    for (uint32_t qi = 0; qi < nq; ++qi) {
+      const vs_query *q = qs[qi];
+      const uint32_t tq = (q->S + RT - 1) / RT;
+      ro->res[qi] = new vs_result_s;
+      vs_result_s *r = ro->res[qi];
+      r->g = g;
+      r->S = q->S;
+      r->V = V;
+      r->tw = TW;
+      r->ntiles = tq;
+      r->k_actual_max = 0;
+      if (q->S) r->row_map.assign(q->sources, q->sources + q->S);
+      if (q->S == 0 || zero_copy) continue;
+      Mat &m = ro->mats[qi];
+      m.bytes = static_cast<size_t>(tq) * mat;
+      // words past the query's own rows stay zero (no batch word feeds them)
+      VS_TRY(g->pool.get(m.bytes, true, g->stream, reinterpret_cast<void **>(&m.d)));
+    }
+    if (zero_copy && S) {
+      Mat &m = ro->mats[0];
+      m.bytes = static_cast<size_t>(T) * mat;
+      VS_TRY(g->pool.get(m.bytes, false, g->stream, reinterpret_cast<void **>(&m.d)));
+    }
+  }
+  if (S64 == 0) return VS_OK;
   // host tables
   std::vector<uint32_t> src(S);
   std::vector<uint8_t> sub(S, 0);
@@ -931,23 +1068,63 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   // become fully visited early (their lanes stop early).  The keys come from
   // one small kernel over the sources' lists.  pos[r] is the packed position
   // of group row r.
+  //
+  // With results, the packing is per query instead: queries in order of
+  // k_max (descending), each starting at a word boundary, its rows sorted
+  // by 2-hop volume within each tile of its own (64 * TW rows of its caller
+  // order), the plain engine's packing.  A query's words then hold only its
+  // rows, so its result is a word-for-word copy -- or, for a one-query
+  // group, the batch's own tiles -- and the result's stored-row map is the
+  // within-tile order.
   std::vector<uint32_t> pos(S);
   {
-    std::vector<uint32_t> order(S);
-    for (uint32_t r = 0; r < S; ++r) order[r] = r;
+    std::vector<uint64_t> key;
     if (!(p.flags & VS_BATCH_KEEP_ORDER)) {
-      std::vector<uint64_t> key(S);
+      key.resize(S);
       VS_TRY(source_volumes(g, push, src.data(), S, key.data()));
-      std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-        if (kmax[x] != kmax[y]) return kmax[x] > kmax[y];
-        return key[x] > key[y];
-      });
     }
-    std::vector<uint32_t> s2(S), kn(S), kx(S);
-    std::vector<uint8_t> sb(S);
-    for (uint32_t i = 0; i < S; ++i) {
-      const uint32_t r = order[i];
-      pos[r] = i;
+    if (!results) {
+      std::vector<uint32_t> order(S);
+      for (uint32_t r = 0; r < S; ++r) order[r] = r;
+      if (!key.empty())
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+          if (kmax[x] != kmax[y]) return kmax[x] > kmax[y];
+          return key[x] > key[y];
+        });
+      for (uint32_t i = 0; i < S; ++i) pos[order[i]] = i;
+    } else {
+      std::vector<uint32_t> qorder(nq), qrow0(nq, 0);
+      for (uint32_t qi = 0; qi < nq; ++qi) {
+        qorder[qi] = qi;
+        if (qi) qrow0[qi] = qrow0[qi - 1] + qs[qi - 1]->S;
+      }
+      std::stable_sort(qorder.begin(), qorder.end(),
+                       [&](uint32_t x, uint32_t y) { return qs[x]->k_max > qs[y]->k_max; });
+      uint32_t w0 = 0;
+      for (uint32_t qi : qorder) {
+        qword0[qi] = w0;
+        const uint32_t n = qs[qi]->S, r0 = qrow0[qi];
+        w0 += (n + 63) / 64;
+        std::vector<uint32_t> ord(n);
+        for (uint32_t i = 0; i < n; ++i) ord[i] = i;
+        if (!key.empty())
+          for (uint32_t t0 = 0; t0 < n; t0 += RT)
+            std::stable_sort(ord.begin() + t0, ord.begin() + std::min(n, t0 + RT),
+                             [&](uint32_t x, uint32_t y) { return key[r0 + x] > key[r0 + y]; });
+        stored[qi].resize(n);
+        bool ident = true;
+        for (uint32_t k = 0; k < n; ++k) {
+          stored[qi][ord[k]] = k;
+          ident &= ord[k] == k;
+          pos[r0 + ord[k]] = qword0[qi] * 64 + k;
+        }
+        if (ident) stored[qi].clear();
+      }
+    }
+    std::vector<uint32_t> s2(Sp, kNoRow), kn(Sp, 0), kx(Sp, 0);
+    std::vector<uint8_t> sb(Sp, 0);
+    for (uint32_t r = 0; r < S; ++r) {
+      const uint32_t i = pos[r];
       s2[i] = src[r];
       kn[i] = kmin[r];
       kx[i] = kmax[r];
@@ -961,7 +1138,8 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   std::vector<uint64_t> live(static_cast<size_t>(H + 2) * Wpad, 0);
   std::vector<uint64_t> racc;
   if (use_r) racc.assign(static_cast<size_t>(H + 1) * Wpad, 0);
-  for (uint32_t r = 0; r < S; ++r) {
+  for (uint32_t r = 0; r < Sp; ++r) {
+    if (src[r] == kNoRow) continue;
     const uint32_t w = r / 64;
     const uint64_t bit = 1ull << (r % 64);
     for (uint32_t i = 0; i <= std::min(kmax[r], H + 1); ++i)
@@ -973,7 +1151,7 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   std::vector<TileInfo> tiles(T);
   for (uint32_t t = 0; t < T; ++t) {
     tiles[t].row0 = t * TW * 64;
-    tiles[t].rows = std::min<uint32_t>(S - tiles[t].row0, TW * 64);
+    tiles[t].rows = std::min<uint32_t>(Sp - tiles[t].row0, TW * 64);
     tiles[t].word0 = t * TW;
     uint32_t h = 0;
     for (uint32_t r = tiles[t].row0; r < tiles[t].row0 + tiles[t].rows; ++r)
@@ -981,7 +1159,6 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
     tiles[t].hmax = h;
   }
   // slots that fit the memory budget
-  const size_t mat = static_cast<size_t>(V) * TW * 8;
   const size_t items_cap = static_cast<size_t>(V) + push->E / kPushChunk + 64;
   const size_t per_slot = mat * (shortest ? 4 : 3) + (use_r ? mat : 0) + 3 * nvb * 8 +
                           2 * static_cast<size_t>(V) * 8 +
@@ -1001,17 +1178,36 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   hipStream_t s = g->stream;
   // device arrays
   DevVec d_src, d_sub, d_live, d_racc, d_tiles, d_slots, d_ctl, d_counts, d_dig;
-  VS_TRY(d_src.alloc(S * 4ull));
-  VS_TRY(d_sub.alloc(S));
+  DevVec d_wdest, d_qbase, d_front;
+  VS_TRY(d_src.alloc(Sp * 4ull));
+  VS_TRY(d_sub.alloc(Sp));
   VS_TRY(d_live.alloc(live.size() * 8));
   if (use_r) VS_TRY(d_racc.alloc(racc.size() * 8));
   VS_TRY(d_tiles.alloc(T * sizeof(TileInfo)));
   VS_TRY(d_slots.alloc(nslots * sizeof(SlotDesc)));
   VS_TRY(d_ctl.alloc(sizeof(BatchCtl)));
-  VS_TRY(d_counts.alloc(S * 8ull));
-  if (want_digests) VS_TRY(d_dig.alloc(S * 8ull));
-  VS_HIP(hipMemcpyAsync(d_src.p, src.data(), S * 4ull, hipMemcpyHostToDevice, s));
-  VS_HIP(hipMemcpyAsync(d_sub.p, sub.data(), S, hipMemcpyHostToDevice, s));
+  VS_TRY(d_counts.alloc(Sp * 8ull));
+  if (want_digests) VS_TRY(d_dig.alloc(Sp * 8ull));
+  const size_t front_words = static_cast<size_t>(T) * (H + 1) * TW;
+  if (results) {
+    VS_TRY(d_front.alloc(front_words * 8));
+    if (!zero_copy) {
+      std::vector<uint2> wdest(Wpad, make_uint2(UINT32_MAX, 0));
+      std::vector<uint64_t *> qbase(nq, nullptr);
+      for (uint32_t qi = 0; qi < nq; ++qi) {
+        qbase[qi] = ro->mats[qi].d;
+        for (uint32_t w = 0; w < (qs[qi]->S + 63) / 64; ++w)
+          wdest[qword0[qi] + w] = make_uint2(qi, w);
+      }
+      VS_TRY(d_wdest.alloc(Wpad * sizeof(uint2)));
+      VS_TRY(d_qbase.alloc(nq * sizeof(uint64_t *)));
+      VS_HIP(hipMemcpyAsync(d_wdest.p, wdest.data(), Wpad * sizeof(uint2), hipMemcpyHostToDevice, s));
+      VS_HIP(hipMemcpyAsync(d_qbase.p, qbase.data(), nq * sizeof(uint64_t *), hipMemcpyHostToDevice, s));
+      VS_HIP(hipStreamSynchronize(s));  // wdest / qbase leave scope
+    }
+  }
+  VS_HIP(hipMemcpyAsync(d_src.p, src.data(), Sp * 4ull, hipMemcpyHostToDevice, s));
+  VS_HIP(hipMemcpyAsync(d_sub.p, sub.data(), Sp, hipMemcpyHostToDevice, s));
   VS_HIP(hipMemcpyAsync(d_live.p, live.data(), live.size() * 8, hipMemcpyHostToDevice, s));
   if (use_r) VS_HIP(hipMemcpyAsync(d_racc.p, racc.data(), racc.size() * 8, hipMemcpyHostToDevice, s));
   VS_HIP(hipMemcpyAsync(d_tiles.p, tiles.data(), T * sizeof(TileInfo), hipMemcpyHostToDevice, s));
@@ -1039,8 +1235,16 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
     VS_TRY(get(items_cap * 8, false, &q)); sd[i].items = static_cast<ull *>(q);
     sd[i].Vis = nullptr;
     sd[i].R = nullptr;
-    if (shortest) { VS_TRY(get(mat, false, &q)); sd[i].Vis = static_cast<uint64_t *>(q); }
-    if (use_r) { VS_TRY(get(mat, false, &q)); sd[i].R = static_cast<uint64_t *>(q); }
+    // with a one-query group's results the result matrix (R, or Vis when no
+    // R is kept) is the query's own tile, set per wave by k_bw_assign
+    if (shortest && !(zero_copy && !use_r)) {
+      VS_TRY(get(mat, false, &q));
+      sd[i].Vis = static_cast<uint64_t *>(q);
+    }
+    if (use_r && !zero_copy) {
+      VS_TRY(get(mat, false, &q));
+      sd[i].R = static_cast<uint64_t *>(q);
+    }
     VS_TRY(get((H + 1) * 64 * 8ull, true, &q));
     sd[i].front = static_cast<uint64_t *>(q);
     VS_TRY(get((H + 1) * sizeof(HopCounters), true, &q));
@@ -1055,8 +1259,8 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   VS_HIP(hipMemcpyAsync(d_slots.p, sd.data(), nslots * sizeof(SlotDesc), hipMemcpyHostToDevice, s));
   BatchCtl ctl0{};
   VS_HIP(hipMemcpyAsync(d_ctl.p, &ctl0, sizeof(ctl0), hipMemcpyHostToDevice, s));
-  VS_HIP(hipMemsetAsync(d_counts.p, 0, S * 8ull, s));
-  if (want_digests) VS_HIP(hipMemsetAsync(d_dig.p, 0, S * 8ull, s));
+  VS_HIP(hipMemsetAsync(d_counts.p, 0, Sp * 8ull, s));
+  if (want_digests) VS_HIP(hipMemsetAsync(d_dig.p, 0, Sp * 8ull, s));
 
   BatchConst c{};
   c.V = V;
@@ -1065,7 +1269,7 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   c.nslots = nslots;
   c.ntiles = T;
   c.Wpad = Wpad;
-  c.S = S;
+  c.S = Sp;
   c.sched = p.sched;
   c.alpha = g->tuning.push_alpha ? g->tuning.push_alpha : 16;
   c.sparse_pct = g->tuning.sparse_pct ? g->tuning.sparse_pct : kSparsePct;
@@ -1093,6 +1297,13 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   c.has_heavy = pull->has_heavy ? 1 : 0;
   c.nhwords = pull->nhwords;
   c.hwords = pull->hwords;
+  if (results) {
+    c.res_base = zero_copy ? ro->mats[0].d : nullptr;
+    c.wdest = zero_copy ? nullptr : static_cast<const uint2 *>(d_wdest.p);
+    c.qbase = zero_copy ? nullptr : static_cast<uint64_t *const *>(d_qbase.p);
+    c.tile_front = static_cast<uint64_t *>(d_front.p);
+    c.res_sub = use_r ? 0u : 1u;
+  }
   const uint32_t grid = static_cast<uint32_t>(g->num_cus) * 8;
 
   hipEvent_t e0, e1;
@@ -1131,8 +1342,8 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
       const auto w0 = std::chrono::steady_clock::now();
       if (rep > 0) {  // timing replays: restart from wave 0 with clean outputs
         re = hipMemcpyAsync(d_ctl.p, &ctl0, sizeof(ctl0), hipMemcpyHostToDevice, s);
-        if (re == hipSuccess) re = hipMemsetAsync(d_counts.p, 0, S * 8ull, s);
-        if (re == hipSuccess && want_digests) re = hipMemsetAsync(d_dig.p, 0, S * 8ull, s);
+        if (re == hipSuccess) re = hipMemsetAsync(d_counts.p, 0, Sp * 8ull, s);
+        if (re == hipSuccess && want_digests) re = hipMemsetAsync(d_dig.p, 0, Sp * 8ull, s);
       }
       if (re == hipSuccess) re = hipEventRecord(e0, s);
       for (uint32_t w = 0; w < nwaves && re == hipSuccess; ++w) re = hipGraphLaunch(exec, s);
@@ -1159,8 +1370,8 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
     for (uint32_t rep = 0; rep < replays; ++rep) {
       if (rep > 0) {
         VS_HIP(hipMemcpyAsync(d_ctl.p, &ctl0, sizeof(ctl0), hipMemcpyHostToDevice, s));
-        VS_HIP(hipMemsetAsync(d_counts.p, 0, S * 8ull, s));
-        if (want_digests) VS_HIP(hipMemsetAsync(d_dig.p, 0, S * 8ull, s));
+        VS_HIP(hipMemsetAsync(d_counts.p, 0, Sp * 8ull, s));
+        if (want_digests) VS_HIP(hipMemsetAsync(d_dig.p, 0, Sp * 8ull, s));
       }
       const bool last = rep + 1 == replays;
       VS_HIP(hipEventRecord(e0, s));
@@ -1183,11 +1394,40 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   st.ms_wall += replay_n ? replay_wall / replay_n : 0.0;
   BatchCtl ctl{};
   VS_HIP(hipMemcpy(&ctl, d_ctl.p, sizeof(ctl), hipMemcpyDeviceToHost));
-  std::vector<uint64_t> hc(S), hd;
-  VS_HIP(hipMemcpy(hc.data(), d_counts.p, S * 8ull, hipMemcpyDeviceToHost));
+  std::vector<uint64_t> hc(Sp), hd;
+  VS_HIP(hipMemcpy(hc.data(), d_counts.p, Sp * 8ull, hipMemcpyDeviceToHost));
   if (want_digests) {
-    hd.resize(S);
-    VS_HIP(hipMemcpy(hd.data(), d_dig.p, S * 8ull, hipMemcpyDeviceToHost));
+    hd.resize(Sp);
+    VS_HIP(hipMemcpy(hd.data(), d_dig.p, Sp * 8ull, hipMemcpyDeviceToHost));
+  }
+  if (results) {
+    // k_actual_max per query as VExpand::after_child sets it
+    // (vexpand.cpp:222-233): Shortest stops at the first hop that adds no
+    // vertex to any of its rows (that hop counts), Any runs to k_max
+    std::vector<uint64_t> front(front_words);
+    VS_HIP(hipMemcpy(front.data(), d_front.p, front_words * 8, hipMemcpyDeviceToHost));
+    for (uint32_t qi = 0; qi < nq; ++qi) {
+      const vs_query *q = qs[qi];
+      vs_result_s *r = ro->res[qi];
+      if (q->S == 0) continue;
+      uint32_t ka = q->k_max;
+      if (shortest)
+        for (uint32_t h = 1; h <= q->k_max; ++h) {
+          uint64_t any = 0;
+          for (uint32_t w = 0; w < (q->S + 63) / 64; ++w) {
+            const uint32_t gw = qword0[qi] + w;
+            const uint32_t n = std::min<uint32_t>(64, q->S - w * 64);
+            const uint64_t mask = n == 64 ? ~0ull : ((1ull << n) - 1);
+            any |= front[(static_cast<size_t>(gw / TW) * (H + 1) + h) * TW + gw % TW] & mask;
+          }
+          if (!any) {
+            ka = h;
+            break;
+          }
+        }
+      r->k_actual_max = ka;
+      r->stored.swap(stored[qi]);
+    }
   }
   // back from packed positions to group row order
   for (uint32_t r = 0; r < S; ++r) counts.push_back(hc[pos[r]]);
@@ -1211,30 +1451,30 @@ using namespace vs;
 
 extern "C" {
 
-vs_status vs_vexpand_batch(vs_graph_t g, const vs_query *queries, uint32_t nq,
-                           const vs_batch_params *p, uint64_t *row_counts,
-                           uint64_t *row_digests, vs_batch_stats *stats) try {
-  clear_error();
+static vs_status batch_impl(const char *fn, vs_graph_t g, const vs_query *queries, uint32_t nq,
+                            const vs_batch_params *p, uint64_t *row_counts,
+                            uint64_t *row_digests, vs_result_t *results,
+                            vs_batch_stats *stats) {
   if (!g || !p || (nq > 0 && !queries)) {
-    set_error("vs_vexpand_batch: null argument");
+    set_error("%s: null argument", fn);
     return VS_ERR_INVALID;
   }
   if (p->tile_words > 64 || (p->tile_words & (p->tile_words - 1))) {
-    set_error("vs_vexpand_batch: tile_words %u is not a power of two <= 64", p->tile_words);
+    set_error("%s: tile_words %u is not a power of two <= 64", fn, p->tile_words);
     return VS_ERR_INVALID;
   }
   if (p->sched < VS_SCHED_HEURISTIC || p->sched > VS_SCHED_LEARNED) {
-    set_error("vs_vexpand_batch: unknown sched %d", p->sched);
+    set_error("%s: unknown sched %d", fn, p->sched);
     return VS_ERR_INVALID;
   }
   for (uint32_t q = 0; q < nq; ++q) {
     const vs_query &Q = queries[q];
     if (Q.path_type != VS_PATH_ANY && Q.path_type != VS_PATH_SHORTEST) {
-      set_error("vs_vexpand_batch: query %u: path_type %d not batchable", q, Q.path_type);
+      set_error("%s: query %u: path_type %d not batchable", fn, q, Q.path_type);
       return VS_ERR_INVALID;
     }
     if (Q.k_max >= VS_MAX_HOPS || (Q.S > 0 && !Q.sources)) {
-      set_error("vs_vexpand_batch: query %u: bad k_max or null sources", q);
+      set_error("%s: query %u: bad k_max or null sources", fn, q);
       return VS_ERR_INVALID;
     }
   }
@@ -1244,6 +1484,9 @@ vs_status vs_vexpand_batch(vs_graph_t g, const vs_query *queries, uint32_t nq,
   std::vector<uint64_t> row_off(nq + 1, 0);
   for (uint32_t q = 0; q < nq; ++q) row_off[q + 1] = row_off[q] + queries[q].S;
   const bool want_d = row_digests != nullptr;
+  // results are handed out only when every group has succeeded
+  std::vector<std::unique_ptr<GroupResults>> held;
+  std::vector<std::vector<uint32_t>> held_ids;
   for (int path : {VS_PATH_SHORTEST, VS_PATH_ANY}) {
     std::vector<const vs_query *> qs;
     std::vector<uint32_t> ids;
@@ -1252,7 +1495,13 @@ vs_status vs_vexpand_batch(vs_graph_t g, const vs_query *queries, uint32_t nq,
     if (qs.empty()) continue;
     out_c.clear();
     out_d.clear();
-    VS_TRY(run_group(g, qs, *p, path == VS_PATH_SHORTEST, out_c, out_d, want_d, st));
+    GroupResults *ro = nullptr;
+    if (results) {
+      held.emplace_back(new GroupResults);
+      held_ids.push_back(ids);
+      ro = held.back().get();
+    }
+    VS_TRY(run_group(g, qs, *p, path == VS_PATH_SHORTEST, out_c, out_d, want_d, st, ro));
     uint64_t r = 0;
     for (uint32_t id : ids) {
       for (uint32_t i = 0; i < queries[id].S; ++i, ++r) {
@@ -1261,9 +1510,41 @@ vs_status vs_vexpand_batch(vs_graph_t g, const vs_query *queries, uint32_t nq,
       }
     }
   }
+  for (size_t k = 0; k < held.size(); ++k) {
+    GroupResults &gr = *held[k];
+    for (size_t qi = 0; qi < held_ids[k].size(); ++qi) {
+      vs_result_s *res = gr.res[qi];
+      res->mats.push_back(gr.mats[qi]);
+      gr.mats[qi] = Mat{};
+      gr.res[qi] = nullptr;
+      g->refs++;
+      results[held_ids[k][qi]] = res;
+    }
+  }
   if (stats) *stats = st;
   return VS_OK;
+}
+
+vs_status vs_vexpand_batch(vs_graph_t g, const vs_query *queries, uint32_t nq,
+                           const vs_batch_params *p, uint64_t *row_counts,
+                           uint64_t *row_digests, vs_batch_stats *stats) try {
+  clear_error();
+  return batch_impl("vs_vexpand_batch", g, queries, nq, p, row_counts, row_digests, nullptr,
+                    stats);
 } VS_CATCH("vs_vexpand_batch")
+
+vs_status vs_vexpand_batch_results(vs_graph_t g, const vs_query *queries, uint32_t nq,
+                                   const vs_batch_params *p, vs_result_t *results,
+                                   vs_batch_stats *stats) try {
+  clear_error();
+  if (nq > 0 && !results) {
+    set_error("vs_vexpand_batch_results: null results");
+    return VS_ERR_INVALID;
+  }
+  for (uint32_t q = 0; q < nq; ++q) results[q] = nullptr;
+  return batch_impl("vs_vexpand_batch_results", g, queries, nq, p, nullptr, nullptr, results,
+                    stats);
+} VS_CATCH("vs_vexpand_batch_results")
 
 // Candidate storage tile widths x push thresholds, timed on the caller's
 // sources (Shortest or Any, hops 1..k_max), best kept in the graph tuning.
