@@ -5,15 +5,19 @@ d=.05, self-loops and duplicates removed), 4096 seeded uniform sources, Shortest
 hops 1..4; frontier / visited / result bit-matrices resident in HBM; the CSR is
 uploaded once before timing.  A step is one full VExpand over the 4096 sources.
 
-Two GPU legs are measured on the same workload:
+Three GPU legs are measured on the same workload, each returning the VExpand
+result (the merged bit matrix, resident in HBM, behind a result handle):
   * plain: the single-query engine (vs_vexpand), density-heuristic push/pull;
-  * batched+learned: the batched engine (vs_vexpand_batch: tiles packed into one
-    persistent expand kernel per hop, the launch sequence captured once in a
-    hipGraph and replayed), with the autotuned tile width and the learned
-    scheduler when its weights are present (else the heuristic).
-Both must give the same per-source counts and per-row digests; the plain leg's
-full matrix is also compared bit-for-bit with the CPU baseline on a sample.
-`value` is the batched+learned GTEPS; the plain GTEPS is reported next to it.
+  * batched+heuristic and batched+learned: the batched engine
+    (vs_vexpand_batch_results: tiles expanded by one persistent kernel per hop,
+    the launch sequence captured once in a hipGraph and replayed), with the
+    tuning the autotuner picks for it, under the density heuristic and under
+    the learned scheduler (when its weights are present), so batching and
+    learning are measured separately.
+All legs must give the same per-source counts, per-row digests, TE and
+k_actual, the value leg the same full matrix as the plain one; the plain
+engine's matrix is also compared bit-for-bit with the CPU baseline on a sample.
+`value` is the batched+learned GTEPS; the other legs are reported next to it.
 
 The CPU baseline is the reference algorithm restated in C++ (oracle B: 1024-row
 stacks, destination-partitioned Hilbert edge blocks, OpenMP) timed on a bounded
@@ -65,7 +69,7 @@ def parse():
     ap.add_argument("--tile-words", type=int, default=0, help="plain leg tile width")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--weights", default=str(ROOT / "vertexsurge_amd" / "sched_weights.npz"))
-    ap.add_argument("--cpu-sources", type=int, default=1024)
+    ap.add_argument("--cpu-sources", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-queries", type=int, default=256)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -118,32 +122,47 @@ def load_scheduler(vs, G, path):
 
 
 def batched_leg(vs, np, G, queries, args, sched, calls=3):
-    """Timed replays produce the per-source counts (the result); the per-row
-    digests for the parity check come from one more, untimed, call.
+    """The batched engine returning the operator's result (vs_vexpand_batch_results:
+    one VExpand result per query, rows packed, expanded in place for one query).
 
-    Each call allocates its matrices afresh, and the same replay has measured
-    88.7-102.7 ms at R-MAT 24 from one call to the next in one process (the
-    kernels and their work are identical; only where the buffers land
-    differs).  The step time is the median over `calls` independent calls;
-    every call's time is reported."""
+    A step is one replay of the captured launch sequence, which recomputes the
+    result matrix; the host clock around a replay (reset, launch, synchronize)
+    is the step time.  Each call allocates its matrices afresh, and the same
+    replay has measured 88.7-102.7 ms at R-MAT 24 from one call to the next in
+    one process (identical kernels and work; only where the buffers land
+    differs), so the step time is the median over `calls` independent calls,
+    each call's time reported.  Counts and digests for the parity check are
+    read from the last call's results."""
     reps = args.warmup + args.steps
-    per_call, c, st = [], None, None
-    for _ in range(calls):
-        ci, _, sti = vs.vexpand_batch(G, queries, sched=sched, replays=max(1, reps),
-                                      want_digests=False)
-        # a step = one replay of the captured launch sequence; the host clock
-        # around it (reset, launch, synchronize) is the step time, the device
-        # events are logged next to it
+    per_call, st = [], None
+    for k in range(calls):
+        res, sti = vs.vexpand_batch_results(G, queries, sched=sched, replays=max(1, reps))
         per_call.append(max(sti["ms_wall"], sti["ms_total"]))
-        if c is not None and not np.array_equal(c, ci):
-            log("batched counts differ between timed calls")
-        c, st = ci, sti
+        if k + 1 < calls:
+            for r in res:
+                r.close()
+        st = sti
     ms = float(np.median(per_call))
-    c2, d, _ = vs.vexpand_batch(G, queries, sched=sched, replays=1, want_digests=True)
-    if not np.array_equal(c, c2):
-        log("batched counts differ between the timed and the digest run")
-    return c, d, st, {"ms": ms, "te": st["te"], "gteps": st["te"] / (ms / 1e3) / 1e9,
-                      "calls_ms": [round(x, 3) for x in per_call]}
+    return res, st, {"ms": ms, "te": st["te"], "gteps": st["te"] / (ms / 1e3) / 1e9,
+                     "calls_ms": [round(x, 3) for x in per_call]}
+
+
+def host_cpu():
+    """Model, sockets, cores and threads of this host (lscpu)."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except Exception:
+        pass
+    return {"model": info.get("Model name", "unknown"),
+            "sockets": int(info.get("Socket(s)", "0") or 0),
+            "cores_per_socket": int(info.get("Core(s) per socket", "0") or 0),
+            "threads_per_core": int(info.get("Thread(s) per core", "0") or 0),
+            "nproc": os.cpu_count()}
 
 
 def main() -> int:
@@ -178,34 +197,51 @@ def main() -> int:
                        args.tile_words)
     log(f"plain: {plain['ms']:.2f} ms/step, {plain['gteps']:.1f} GTEPS (TE={plain['te']})")
 
-    tuning = G.get_tuning()
-    if not args.no_autotune:
-        import ctypes as C
-        t = vs._lib.Tuning()
-        best_ms = C.c_double()
-        vs._lib.call("vs_autotune_vexpand", G.handle, src.ctypes.data_as(vs._lib.u32p),
-                     len(src), args.k_max, vs._PATHS[args.path], C.byref(t), C.byref(best_ms))
-        tuning = {"tile_words": t.tile_words, "push_alpha": t.push_alpha}
-        log(f"autotuned: {tuning} ({best_ms.value:.2f} ms)")
-    sched = load_scheduler(vs, G, args.weights)
-    bsched = "learned" if sched is not None else "heuristic"
     queries = [vs.BatchQuery(src, args.k_min, args.k_max, args.path)]
-    bc, bd, bst, batched = batched_leg(vs, np, G, queries, args, bsched)
-    log(f"batched+{bsched}: {batched['ms']:.2f} ms/step, {batched['gteps']:.1f} GTEPS "
-        f"(tiles {bst['num_tiles']}, push/pull decisions {bst['push_decisions']}/"
-        f"{bst['pull_decisions']})")
+    tuning = G.get_tuning()
+    tune_ms = None
+    if not args.no_autotune:
+        tuning, best_ms, start_ms = vs.autotune_batch(G, queries)
+        tune_ms = {"start": round(start_ms, 3), "best": round(best_ms, 3)}
+        log(f"autotuned (batched engine): {tuning} ({start_ms:.2f} -> {best_ms:.2f} ms)")
+    sched = load_scheduler(vs, G, args.weights)
+    legs = {}
+    for bsched in ["heuristic"] + (["learned"] if sched is not None else []):
+        res, bst, leg = batched_leg(vs, np, G, queries, args, bsched)
+        r = res[0]
+        leg.update(sched=bsched, counts=r.row_counts(0), digests=r.row_digests(0),
+                   k_actual=r.k_actual_max, push=bst["push_decisions"],
+                   pull=bst["pull_decisions"], tiles=bst["num_tiles"],
+                   launches=bst["launches"], ms_device_last_call=bst["ms_total"],
+                   ms_host_clock_last_call=bst["ms_wall"], result=r)
+        legs[bsched] = leg
+        log(f"batched+{bsched}: {leg['ms']:.2f} ms/step, {leg['gteps']:.1f} GTEPS (calls "
+            f"{leg['calls_ms']}; tiles {bst['num_tiles']}, push/pull decisions "
+            f"{bst['push_decisions']}/{bst['pull_decisions']})")
+    bsched = "learned" if "learned" in legs else "heuristic"
+    batched = legs[bsched]
     if args.profile_only:
         return 0
 
-    # parity of the two legs: per-source counts and digests
+    # parity: every leg's result against the plain result (row counts,
+    # digests, TE, k_actual), and the value leg's full matrix
     res, pst = vs.vexpand(G, src, args.k_min, args.k_max, args.path, sched=args.sched,
                           tile_words=plain["tile_words"], timing=True)
     pc, pd = res.row_counts(0), res.row_digests(0)
+    exact_legs = {}
+    for name, leg in legs.items():
+        exact_legs[name] = bool(np.array_equal(pc, leg["counts"]) and
+                                np.array_equal(pd, leg["digests"]) and
+                                pst["te"] == leg["te"] and pst["k_actual_max"] == leg["k_actual"])
+    full = bool(np.array_equal(res.export(0, vs.LAYOUT_DVSM),
+                               batched["result"].export(0, vs.LAYOUT_DVSM)))
     res.close()
-    legs_exact = bool(np.array_equal(pc, bc) and np.array_equal(pd, bd)
-                      and pst["te"] == bst["te"])
-    log(f"batched == plain (counts, digests, TE): {legs_exact}; plain modes per hop "
-        f"{pst['mode_hop']} ms per hop {[round(x, 2) for x in pst['ms_hop']]}")
+    for leg in legs.values():
+        leg["result"].close()
+    legs_exact = all(exact_legs.values()) and full
+    log(f"batched == plain (counts, digests, TE, k_actual per leg {exact_legs}; full matrix of "
+        f"the {bsched} leg {full}); plain modes per hop {pst['mode_hop']} ms per hop "
+        f"{[round(x, 2) for x in pst['ms_hop']]}")
 
     # roofline of the measured leg's dominant kernel, k_bw_expand (per-launch
     # HIP events on the engine's stream).  It runs as two template variants
@@ -271,15 +307,17 @@ def main() -> int:
             "k_max": args.k_max, "path": args.path, "te": plain["te"],
             "k_actual_max": pst["k_actual_max"], "reachable_pairs": int(pc.sum()),
             "value_leg": f"batched+{bsched}" if use_batched else "plain",
-            "batched": {"gteps": round(batched["gteps"], 2), "ms": round(batched["ms"], 3),
-                        "ms_calls": batched["calls_ms"],
-                        "ms_device_last_call": round(bst["ms_total"], 3),
-                        "ms_host_clock_last_call": round(bst["ms_wall"], 3),
-                        "sched": bsched, "tuning": tuning, "tiles": bst["num_tiles"],
-                        "graph_launches": bst["launches"],
-                        "push_decisions": bst["push_decisions"],
-                        "pull_decisions": bst["pull_decisions"],
-                        "bit_exact_vs_plain": legs_exact},
+            "legs_return": "VExpand result handles (the merged matrix in HBM) in every leg",
+            "tuning": tuning, "autotune_ms": tune_ms,
+            "batched": {name: {"gteps": round(l["gteps"], 2), "ms": round(l["ms"], 3),
+                               "ms_calls": l["calls_ms"],
+                               "ms_device_last_call": round(l["ms_device_last_call"], 3),
+                               "ms_host_clock_last_call": round(l["ms_host_clock_last_call"], 3),
+                               "tiles": l["tiles"], "graph_launches": l["launches"],
+                               "push_decisions": l["push"], "pull_decisions": l["pull"],
+                               "bit_exact_vs_plain": exact_legs[name]}
+                        for name, l in legs.items()},
+            "bit_exact_all_legs": legs_exact,
             "plain_gpu": {"gteps": round(plain["gteps"], 2), "ms": round(plain["ms"], 3),
                           "sched": args.sched, "tile_words": pst["tile_words"],
                           "dominant_kernel": f"k_{p_dom}",
@@ -300,27 +338,41 @@ def main() -> int:
 
 
 def cpu_baseline(vs, np, csr, G, src, args):
+    """Oracle B (the reference's CPU algorithm restated) on the first
+    --cpu-sources sources (2048 = two 1024-row stacks): one warm-up run, then
+    the median of three; its matrix is compared with the GPU's."""
     import oracle
     ncpu = min(args.cpu_sources, len(src))
     sample = src[:ncpu]
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    log(f"CPU baseline: building destination-partitioned Hilbert blocks ({threads} threads)")
+    cpu = host_cpu()
+    log(f"CPU baseline on {cpu}: building destination-partitioned Hilbert blocks "
+        f"({threads} threads)")
     tb = time.time()
     blocks = oracle.EdgeBlocks(csr.offsets, csr.adj, nblocks=1000)
-    log(f"blocks built in {time.time() - tb:.1f}s; running oracle B on {ncpu} sources")
-    dv, ka_cpu, _, cpu_ms = blocks.vexpand(sample, args.k_min, args.k_max, args.path)
+    log(f"blocks built in {time.time() - tb:.1f}s; oracle B on {ncpu} sources, warm-up + 3 runs")
+    runs = []
+    for k in range(4):
+        dv, ka_cpu, _, cpu_ms = blocks.vexpand(sample, args.k_min, args.k_max, args.path)
+        if k:
+            runs.append(cpu_ms)
+        if k < 3:
+            del dv
     rs, sst = vs.vexpand(G, sample, args.k_min, args.k_max, args.path)
     gpu_dv = rs.export(0, vs.LAYOUT_DVSM)
     exact = bool(np.array_equal(gpu_dv, dv)) and ka_cpu == sst["k_actual_max"]
     rs.close()
     del gpu_dv, dv
     blocks.close()
+    cpu_ms = float(np.median(runs))
     cpu_gteps = sst["te"] / (cpu_ms / 1e3) / 1e9
-    log(f"CPU {cpu_ms / 1e3:.2f}s for {ncpu} sources (TE={sst['te']}): {cpu_gteps:.2f} GTEPS;"
-        f" GPU==CPU bit-exact: {exact}")
+    log(f"CPU {[round(x / 1e3, 2) for x in runs]} s for {ncpu} sources (TE={sst['te']}): "
+        f"{cpu_gteps:.2f} GTEPS (median); GPU==CPU bit-exact: {exact}")
     return {"value": round(cpu_gteps, 3), "unit": "GTEPS", "cores": threads, "kind": "port",
-            "sample": f"{ncpu} of {len(src)} sources, hops {args.k_min}..{args.k_max}, same "
-                      f"graph; TE of the sample / CPU time",
+            "sample": f"{ncpu} of {len(src)} sources ({(ncpu + 1023) // 1024} stacks of 1024), "
+                      f"hops {args.k_min}..{args.k_max}, same graph; TE of the sample / median "
+                      f"of 3 runs after a warm-up",
+            "runs_s": [round(x / 1e3, 3) for x in runs], "host": cpu,
             "bit_exact_vs_gpu": exact}
 
 
@@ -366,6 +418,10 @@ def run_batch_workload(vs, np, csr, G, args):
     vs._lib.call("vs_device_synchronize")
     plain_wall = time.perf_counter() - t
     plain_s = plain_dev_ms / 1e3
+    tuning = G.get_tuning()
+    if not args.no_autotune:  # tuned on every 16th query (the whole batch takes seconds)
+        tuning, best_ms, start_ms = vs.autotune_batch(G, qs[::16])
+        log(f"autotuned on {len(qs[::16])} queries: {tuning} ({start_ms:.1f} -> {best_ms:.1f} ms)")
     sched = load_scheduler(vs, G, args.weights)
     bsched = "learned" if sched is not None else "heuristic"
     # timed call: the per-row counts (the result), as the plain leg's device
@@ -391,6 +447,7 @@ def run_batch_workload(vs, np, csr, G, args):
                       "plain_wall_s": round(plain_wall, 3),
                       "batched_s": round(st["ms_total"] / 1e3, 3),
                       "batched_wall_s": round(batch_wall, 3), "sched": bsched,
+                      "tuning": tuning,
                       "bit_exact_vs_plain": exact, "stats": st}}
     print(json.dumps(out), flush=True)
     return 0

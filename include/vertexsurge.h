@@ -561,8 +561,19 @@ vs_status vs_mintersect_colmat(const vs_colmat_t *deps, uint32_t nd,
                                uint64_t *total, vs_mintersect_stats *stats);
 
 /* -------------------------------------------------------------- autotuner */
-/* Times candidate tile widths and push thresholds on the given sources and
- * stores the fastest in the graph's tuning. */
+/* Tunes the batched engine on the caller's queries and stores the result in
+ * the graph's tuning: coordinate descent over tile_words (the widest the
+ * rows allow and two narrower), push_alpha (4, 16, 64), sparse_pct (35, 50,
+ * 65) and pull_hubs (32768, 131072, 524288, at most V), each candidate timed
+ * by vs_vexpand_batch (device time, the mean of two replays of the captured
+ * graph after a warm one).  The starting tuning (the graph's, or
+ * p->tile_words when set) is measured first (*start_ms) and a candidate is
+ * kept only when faster, so *best_ms <= *start_ms.  Scheduling knobs never
+ * change a result bit.  best, best_ms and start_ms may be NULL. */
+vs_status vs_autotune_batch(vs_graph_t g, const vs_query *queries, uint32_t nq,
+                            const vs_batch_params *p, vs_tuning *best, double *best_ms,
+                            double *start_ms);
+/* vs_autotune_batch over one query: the given sources, hops 1..k_max. */
 vs_status vs_autotune_vexpand(vs_graph_t g, const uint32_t *sources, uint32_t S,
                               uint32_t k_max, int32_t path_type,
                               vs_tuning *best, double *best_ms);

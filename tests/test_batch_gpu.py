@@ -215,3 +215,21 @@ def test_batch_results_replayed_graph_and_learned(g14):
         _same_result(a, b, g.V)
         a.close()
         b.close()
+
+
+def test_autotune_batch_exact_and_no_slower(g14):
+    g, G = g14
+    saved = G.get_tuning()
+    qs = make_queries(g.V, 8, seed=41, smax=3000, kmax=(2, 5))
+    c0, d0, _ = vs.vexpand_batch(G, qs)
+    tuning, best, start = vs.autotune_batch(G, qs)
+    try:
+        # the starting tuning is one of the candidates: never slower by the
+        # tuner's own clock; the pick is stored in the graph
+        assert 0 < best <= start
+        assert G.get_tuning() == tuning
+        assert tuning["sparse_pct"] in (35, 50, 65) and tuning["push_alpha"] in (4, 16, 64)
+        c1, d1, _ = vs.vexpand_batch(G, qs)
+        assert np.array_equal(c0, c1) and np.array_equal(d0, d1)
+    finally:
+        G.set_tuning(**saved)

@@ -20,7 +20,7 @@ __all__ = [
     "HostCSR", "Graph", "GraphStore", "store_write", "VExpandResult", "vexpand", "find_shortest", "mintersect",
     "ColMat", "mintersect_colmat", "pack_colmain", "unpack_colmain",
     "sample_sources", "sample_paths3", "Scheduler", "vexpand_batch",
-    "vexpand_batch_results", "BatchQuery", "device_count",
+    "vexpand_batch_results", "autotune_batch", "BatchQuery", "device_count",
     "PATH_ANY", "PATH_SHORTEST", "PATH_FIND_SHORTEST", "DIR_FORWARD", "DIR_BACKWARD",
     "DIR_BOTH", "SCHED_HEURISTIC", "SCHED_PUSH", "SCHED_PULL", "SCHED_LEARNED",
     "LAYOUT_NATIVE", "LAYOUT_DVSM", "LAYOUT_ROWMAJOR", "VSError",
@@ -667,6 +667,19 @@ def _batch_stats(st) -> dict:
             "ms_expand": st.ms_expand, "n_expand": st.n_expand, "bytes_alg": st.bytes_alg,
             "ms_wall": st.ms_wall, "ms_expand_sparse": st.ms_expand_sparse,
             "n_expand_sparse": st.n_expand_sparse, "bytes_sparse": st.bytes_sparse}
+
+
+def autotune_batch(graph: Graph, queries: list, tile_words: int = 0, sched: str = "heuristic",
+                   direction: str = "forward"):
+    """Tunes the batched engine on these queries (vs_autotune_batch); the
+    best tuning is stored in the graph.  Returns (tuning dict, best ms, start ms)."""
+    keep, arr, p = _batch_args(queries, tile_words, sched, True, 1, 0, direction, False, False)
+    t = _lib.Tuning()
+    best, start = C.c_double(), C.c_double()
+    _lib.call("vs_autotune_batch", graph.handle, arr, len(queries), C.byref(p), C.byref(t),
+              C.byref(best), C.byref(start))
+    return ({"tile_words": t.tile_words, "push_alpha": t.push_alpha,
+             "sparse_pct": t.sparse_pct, "pull_hubs": t.pull_hubs}, best.value, start.value)
 
 
 def vexpand_batch_results(graph: Graph, queries: list, tile_words: int = 0,

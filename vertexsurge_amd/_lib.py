@@ -185,6 +185,8 @@ SIGNATURES = {
                                  f32p]),
     "vs_sched_predict": (C.c_int, [vp, f32p, C.c_uint32, f32p]),
     "vs_graph_set_scheduler": (C.c_int, [vp, vp]),
+    "vs_autotune_batch": (C.c_int, [vp, C.POINTER(Query), C.c_uint32, C.POINTER(BatchParams),
+                                    C.POINTER(Tuning), f64p, f64p]),
     "vs_autotune_vexpand": (C.c_int, [vp, u32p, C.c_uint32, C.c_uint32, C.c_int32,
                                       C.POINTER(Tuning), f64p]),
 }

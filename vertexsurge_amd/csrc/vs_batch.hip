@@ -1166,6 +1166,7 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
                           (H + 1) * kParts * (kPartStride + 64 * 8);
   size_t freeb = 0, totalb = 0;
   VS_HIP(hipMemGetInfo(&freeb, &totalb));
+  freeb += g->pool.cached();  // the slot buffers come from the graph's pool
   const size_t budget = p.mem_budget ? p.mem_budget : freeb / 10 * 7;
   uint32_t nslots = static_cast<uint32_t>(std::min<size_t>(budget / per_slot, T));
   nslots = std::min<uint32_t>(nslots, kMaxSlots);
@@ -1211,15 +1212,21 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   VS_HIP(hipMemcpyAsync(d_live.p, live.data(), live.size() * 8, hipMemcpyHostToDevice, s));
   if (use_r) VS_HIP(hipMemcpyAsync(d_racc.p, racc.data(), racc.size() * 8, hipMemcpyHostToDevice, s));
   VS_HIP(hipMemcpyAsync(d_tiles.p, tiles.data(), T * sizeof(TileInfo), hipMemcpyHostToDevice, s));
-  // slot buffers (clean: zero once, kept clean by the kernels)
-  std::vector<DevVec> bufs;
-  bufs.reserve(nslots * 16);  // 13 buffers per slot; moves are safe anyway
+  // Slot buffers come from the graph's pool (zeroed once per call, kept
+  // clean by the kernels during it), so repeated calls -- and the plain
+  // engine, whose matrices have the same size -- reuse the same device
+  // memory; they go back to the pool when the call ends.
+  struct SlotBufs {
+    Pool &pool;
+    std::vector<std::pair<void *, size_t>> v;
+    ~SlotBufs() {
+      for (auto &x : v) pool.put(x.first, x.second, false);
+    }
+  } bufs{g->pool, {}};
   std::vector<SlotDesc> sd(nslots);
   auto get = [&](size_t bytes, bool zero, void **out) -> vs_status {
-    bufs.emplace_back();
-    VS_TRY(bufs.back().alloc(bytes));
-    if (zero) VS_HIP(hipMemsetAsync(bufs.back().p, 0, bytes, s));
-    *out = bufs.back().p;
+    VS_TRY(g->pool.get(bytes, zero, s, out));
+    bufs.v.emplace_back(*out, bytes);
     return VS_OK;
   };
   for (uint32_t i = 0; i < nslots; ++i) {
@@ -1546,8 +1553,95 @@ vs_status vs_vexpand_batch_results(vs_graph_t g, const vs_query *queries, uint32
                     stats);
 } VS_CATCH("vs_vexpand_batch_results")
 
-// Candidate storage tile widths x push thresholds, timed on the caller's
-// sources (Shortest or Any, hops 1..k_max), best kept in the graph tuning.
+// Autotuner of the batched engine.  Coordinate descent over the graph's
+// tuning: tile width, then the push threshold alpha, the word-mask threshold
+// sparse_pct and the hub count pull_hubs, each knob over a few candidates
+// with the others fixed; every candidate is timed by the batched engine on
+// the caller's queries (device time: the mean of `reps` replays of the
+// captured launch sequence after one warm replay).  The starting tuning is
+// measured first and a candidate replaces the current choice only when it
+// is faster, so the result is never slower than the start by the tuner's own
+// clock.  No choice changes a result bit.
+static vs_status autotune_batch(vs_graph_t g, const vs_query *queries, uint32_t nq,
+                                const vs_batch_params &p0, vs_tuning *best, double *best_ms,
+                                double *start_ms) {
+  uint64_t rows = 0;
+  for (uint32_t q = 0; q < nq; ++q) rows += queries[q].S;
+  if (rows == 0) {
+    set_error("vs_autotune_batch: no source rows");
+    return VS_ERR_INVALID;
+  }
+  vs_batch_params p = p0;
+  p.use_graph = 1;
+  p.replays = 3;  // one warm replay, the mean of the other two
+  p.flags &= ~VS_VX_TIMING;
+  const vs_tuning saved = g->tuning;
+  vs_tuning cur = saved;
+  const uint32_t tw_max = choose_tile_words(64, static_cast<uint32_t>((rows + 63) / 64));
+  cur.tile_words = choose_tile_words(p.tile_words ? p.tile_words : cur.tile_words,
+                                     static_cast<uint32_t>((rows + 63) / 64));
+  p.tile_words = 0;  // the graph's tuning decides
+  if (!cur.push_alpha) cur.push_alpha = 16;
+  if (!cur.sparse_pct) cur.sparse_pct = kSparsePct;
+  if (!cur.pull_hubs) cur.pull_hubs = kPullHubs;
+  auto measure = [&](const vs_tuning &t, double *ms) -> vs_status {
+    g->tuning = t;
+    vs_batch_stats st{};
+    const vs_status s = batch_impl("vs_autotune_batch", g, queries, nq, &p, nullptr, nullptr,
+                                   nullptr, &st);
+    *ms = st.ms_total;
+    return s;
+  };
+  double bms = 0.0;
+  vs_status s = measure(cur, &bms);
+  if (s != VS_OK) { g->tuning = saved; return s; }
+  if (start_ms) *start_ms = bms;
+  auto try_knob = [&](uint32_t vs_tuning::*knob, std::vector<uint32_t> cands) -> vs_status {
+    const uint32_t keep = cur.*knob;
+    for (uint32_t v : cands) {
+      if (v == keep) continue;
+      vs_tuning t = cur;
+      t.*knob = v;
+      double ms = 0.0;
+      VS_TRY(measure(t, &ms));
+      if (ms < bms) {
+        bms = ms;
+        cur = t;
+      }
+    }
+    return VS_OK;
+  };
+  std::vector<uint32_t> tws;
+  for (uint32_t tw = tw_max; tw >= 1 && tws.size() < 3; tw /= 2) tws.push_back(tw);
+  const uint32_t V = g->V;
+  std::vector<uint32_t> hubs;
+  for (uint32_t h : {32768u, 131072u, 524288u}) hubs.push_back(std::min(h, std::max(V, 1u)));
+  s = try_knob(&vs_tuning::tile_words, tws);
+  if (s == VS_OK) s = try_knob(&vs_tuning::push_alpha, {4u, 16u, 64u});
+  if (s == VS_OK) s = try_knob(&vs_tuning::sparse_pct, {35u, 50u, 65u});
+  if (s == VS_OK) s = try_knob(&vs_tuning::pull_hubs, hubs);
+  if (s != VS_OK) {
+    g->tuning = saved;
+    return s;
+  }
+  g->tuning = cur;
+  if (best) *best = cur;
+  if (best_ms) *best_ms = bms;
+  return VS_OK;
+}
+
+vs_status vs_autotune_batch(vs_graph_t g, const vs_query *queries, uint32_t nq,
+                            const vs_batch_params *p, vs_tuning *best, double *best_ms,
+                            double *start_ms) try {
+  clear_error();
+  if (!g || !p || !queries || nq == 0) {
+    set_error("vs_autotune_batch: need a graph, parameters and queries");
+    return VS_ERR_INVALID;
+  }
+  return autotune_batch(g, queries, nq, *p, best, best_ms, start_ms);
+} VS_CATCH("vs_autotune_batch")
+
+// One query (hops 1..k_max) through the batched engine's tuner.
 vs_status vs_autotune_vexpand(vs_graph_t g, const uint32_t *sources, uint32_t S,
                               uint32_t k_max, int32_t path_type, vs_tuning *best,
                               double *best_ms) try {
@@ -1556,39 +1650,11 @@ vs_status vs_autotune_vexpand(vs_graph_t g, const uint32_t *sources, uint32_t S,
     set_error("vs_autotune_vexpand: need a graph, sources and k_max >= 1");
     return VS_ERR_INVALID;
   }
-  const uint32_t W = (S + 63) / 64;
-  const uint32_t tw_max = choose_tile_words(64, W);
-  vs_tuning saved = g->tuning, bt = g->tuning;
-  double bms = 1e300;
-  for (uint32_t tw = std::max<uint32_t>(1, tw_max / 4); tw <= tw_max; tw *= 2) {
-    for (uint32_t alpha : {4u, 16u, 64u}) {
-      vs_tuning t{tw, alpha, saved.sparse_pct, saved.pull_hubs};
-      g->tuning = t;
-      vs_vexpand_params pp{};
-      pp.k_min = 1;
-      pp.k_max = k_max;
-      pp.path_type = path_type;
-      pp.direction = VS_DIR_FORWARD;
-      pp.sched = VS_SCHED_HEURISTIC;
-      double ms = 0;
-      for (int rep = 0; rep < 2; ++rep) {  // first run warms the pool
-        vs_vexpand_stats st;
-        vs_result_t r = nullptr;
-        vs_status s = vs_vexpand(g, sources, S, &pp, &r, &st);
-        if (s != VS_OK) { g->tuning = saved; return s; }
-        vs_result_free(r);
-        ms = st.ms_total;
-      }
-      if (ms < bms) {
-        bms = ms;
-        bt = t;
-      }
-    }
-  }
-  g->tuning = bt;
-  if (best) *best = bt;
-  if (best_ms) *best_ms = bms;
-  return VS_OK;
+  vs_query q{sources, S, 1, k_max, path_type};
+  vs_batch_params p{};
+  p.sched = VS_SCHED_HEURISTIC;
+  p.direction = VS_DIR_FORWARD;
+  return autotune_batch(g, &q, 1, p, best, best_ms, nullptr);
 } VS_CATCH("vs_autotune_vexpand")
 
 } // extern "C"

@@ -49,6 +49,12 @@ class Pool {
   vs_status get(size_t bytes, bool zero, hipStream_t s, void **out);
   void put(void *p, size_t bytes, bool clean);
   void trim();
+  // bytes of the cached (free) buffers
+  size_t cached() const {
+    size_t n = 0;
+    for (const auto &kv : free_) n += kv.first;
+    return n;
+  }
   ~Pool() { trim(); }
 
  private:
