@@ -66,7 +66,10 @@ typedef enum {
 /* Per-hop expansion scheduling.  The reference's ExpandStrategy::Dense runs
  * every edge every hop (expand.cpp:263-311); Adaptive switches on a density
  * ratio (expand.cpp:346-414).  On the GPU every choice below yields the same
- * bits; only the time differs. */
+ * bits; only the time differs.  VS_SCHED_LEARNED also picks the storage tile
+ * width of a query (or of a batch group) when none is forced (tile_words 0
+ * in the call and in the graph tuning): the analogue of the reference's
+ * stack-height choice decide_k0 (expand.cpp:416-449). */
 typedef enum {
   VS_SCHED_HEURISTIC = 0, /* push/pull by frontier-edge density */
   VS_SCHED_PUSH = 1,      /* always sparse scatter */
@@ -265,6 +268,14 @@ typedef struct {
   uint64_t frontier_act[VS_MAX_HOPS]; /* vertices with a nonzero frontier row */
   uint64_t frontier_deg[VS_MAX_HOPS]; /* their out-degree sum */
   double vis_frac[VS_MAX_HOPS];       /* nonzero visited words / all words */
+  /* query features (filled when rows are packed, the learned scheduler picks
+   * the tile width, or VS_VX_TIMING is set; else 0): out-degree sum of the
+   * sources and their 2-hop volume sum_{s} sum_{u in N(s)} deg(u) */
+  uint64_t src_degsum;
+  uint64_t src_vol2;
+  /* 1 when the learned scheduler picked tile_words */
+  uint32_t tile_words_learned;
+  uint32_t reserved2;
 } vs_vexpand_stats;
 
 typedef struct vs_result_s *vs_result_t;
@@ -470,6 +481,13 @@ vs_status vs_sched_train(vs_sched_t s, const float *X, const float *Y,
                          const float *M, uint32_t n, uint32_t epochs, float lr,
                          float *loss_curve);
 vs_status vs_sched_predict(vs_sched_t s, const float *X, uint32_t n, float *Y);
+/* The feature domain in which the scheduler may overrule the density
+ * heuristic: n feature rows (VS_MLP_IN floats each; normally its training
+ * samples, at most VS_SCHED_MAX_DOMAIN) and a radius.  A decision whose
+ * feature vector lies farther than `radius` (Euclidean) from every row is the
+ * heuristic's.  n == 0 removes the guard. */
+#define VS_SCHED_MAX_DOMAIN 1024
+vs_status vs_sched_set_domain(vs_sched_t s, const float *rows, uint32_t n, float radius);
 /* Attach (or detach with NULL) a scheduler used by VS_SCHED_LEARNED.  The
  * graph holds its own reference: vs_sched_free may be called while the
  * scheduler is attached, and the scheduler lives until the last graph using

@@ -16,67 +16,98 @@ HELD = [{"kind": "rmat", "scale": 17, "ef": 16, "a": 0.57, "seed": 99},
         {"kind": "er", "V": 1 << 17, "pairs": 16 << 17, "seed": 99}]
 
 
-def heuristic_choice(X, E, alpha=16):
-    # feature 2 is log2(1 + actdeg) / 40; heuristic: push iff actdeg * alpha < E
-    actdeg = np.exp2(X[:, 2].astype(np.float64) * 40.0) - 1.0
-    return np.where(actdeg * alpha < E, 0, 1)
-
-
 @pytest.fixture(scope="module")
 def trained():
-    X, Y, M = sched_train.collect(TRAIN, source_sizes=(64, 512, 2048), k_max=5, log=print)
-    assert len(X) >= 20
-    s = sched_train.train(X, Y, M, epochs=800, lr=3e-3)
+    X, Y, M = sched_train.collect(TRAIN, source_sizes=(64, 512, 2048), k_max=5, log=print,
+                                  tw_sizes=(2048, 4096, 8192), tw_hops=(3, 5))
+    assert len(X) >= 20 and (~sched_train.hop_rows(X)).sum() >= 8
+    s = sched_train.train(X, Y, M, epochs=1500, lr=3e-3)
     assert s.curve[-1] < s.curve[0]
-    return s, X, Y
+    return s, X, Y, M
 
 
 def test_training_fits_the_traces(trained):
-    s, X, Y = trained
+    s, X, Y, M = trained
     acc = sched_train.accuracy(s, X, Y)
     # on its own traces the MLP must pick the faster option most of the time
     assert acc["agree"] >= 0.7, acc
     assert acc["ms_mlp"] <= min(acc["ms_push"], acc["ms_pull"]) * 1.05, acc
+    tw = sched_train.tw_accuracy(s, X, Y, M)
+    assert tw["ms_mlp"] <= tw["ms_fallback"] * 1.0001, tw
 
 
-def test_learned_vs_heuristic_on_held_out(trained):
-    s, _, _ = trained
-    gains = []
+def test_learned_gains_on_held_out_graphs(trained):
+    """The learned decisions, judged on measured traces of graphs the MLP never
+    saw: for each decision the same measured table prices the learned choice
+    and the method without it (density heuristic for push/pull, the widest
+    tile for the tile width), so no timing noise separates them.  The learned
+    scheduler must not lose on any held-out graph."""
+    s = trained[0]
+    rows = sched_train.heldout_traces(s, HELD, source_sizes=(128, 1024, 4096), k_max=5,
+                                      tw_sizes=(2048, 4096, 8192), tw_hops=(3, 5), log=print)
+    for r in rows:
+        gain = 1 - r["ms_learned"] / r["ms_heuristic"]
+        tw_gain = 1 - r["tw_ms_learned"] / r["tw_ms_default"]
+        print(f"{r['graph']}: push/pull gain {100 * gain:.2f}% ({r['changed']} of {r['hops']} "
+              f"decisions changed), tile width gain {100 * tw_gain:.2f}% ({r['tw_changed']} of "
+              f"{r['queries']} changed)")
+        assert r["ms_learned"] <= r["ms_heuristic"] * 1.0001, r
+        assert r["tw_ms_learned"] <= r["tw_ms_default"] * 1.0001, r
+
+
+def test_learned_is_bit_exact(trained):
+    s = trained[0]
     for spec in HELD:
         csr = sched_train.make_graph(spec)
         G = vs.Graph.from_csr(csr)
-        X, Y, M = [], [], []
-        for q, S in enumerate((128, 1024, 4096)):
-            src = vs.sample_sources(csr.V, S, 500 + q)
-            x, y, m = sched_train.trace_query(G, csr.V, csr.E, src, 5)
-            X += x
-            Y += y
-        X, Y = np.array(X), np.array(Y)
-        P = s.predict(X)
-        mlp = np.where(P[:, 0] < P[:, 1], 0, 1)
-        heur = heuristic_choice(X, csr.E)
-        t = lambda pick: float(np.sum(np.exp2(Y[np.arange(len(Y)), pick])))
-        t_mlp, t_heur, t_best = t(mlp), t(heur), float(np.sum(np.exp2(Y.min(1))))
-        gains.append((spec["kind"], t_heur, t_mlp, t_best))
-        print(f"{spec}: heuristic {t_heur:.2f} ms, learned {t_mlp:.2f} ms, best {t_best:.2f} ms")
-        # bit-exactness under the learned scheduler, host path
         G.set_scheduler(s)
-        src = vs.sample_sources(csr.V, 1024, 7)
+        src = vs.sample_sources(csr.V, 4096, 7)
         r1, s1 = vs.vexpand(G, src, 1, 4, sched="learned")
         r2, s2 = vs.vexpand(G, src, 1, 4, sched="heuristic")
         assert np.array_equal(r1.export(0, vs.LAYOUT_DVSM), r2.export(0, vs.LAYOUT_DVSM))
         assert s1["te"] == s2["te"]
+        print(f"{spec}: learned tile width {s1['tile_words']} (heuristic {s2['tile_words']})")
+        b1, _ = vs.vexpand_batch_results(G, [vs.BatchQuery(src, 1, 4)], sched="learned")
+        assert np.array_equal(b1[0].export(0, vs.LAYOUT_DVSM), r2.export(0, vs.LAYOUT_DVSM))
+        for x in (r1, r2, b1[0]):
+            x.close()
         G.set_scheduler(None)
-    # the learned choices must not lose against the heuristic by more than
-    # timing noise on the held-out graphs, summed
-    t_heur = sum(g[1] for g in gains)
-    t_mlp = sum(g[2] for g in gains)
-    assert t_mlp <= 1.10 * t_heur, gains
 
 
 def test_weights_roundtrip(tmp_path, trained):
-    s, X, _ = trained
+    s, X, _, _ = trained
     p = tmp_path / "w.npz"
     sched_train.save(s, p, {"note": "test"})
     s2 = sched_train.load(p)
     assert np.array_equal(s.predict(X[:16]), s2.predict(X[:16]))
+
+
+def test_domain_guard_returns_decisions_to_the_heuristic():
+    """An MLP that always predicts push 2^10 times faster overrides every
+    decision inside its domain, and none outside it -- on the host path and
+    inside the replayed batched graph -- without changing a bit."""
+    g = vs.HostCSR.rmat(15, 16, seed=12)
+    G = vs.Graph.from_csr(g)
+    s = vs.Scheduler(seed=3)
+    p = np.zeros(s.num_params, dtype=np.float32)
+    p[-8] = -5.0   # b3[push]: log2 ms predicted for push
+    p[-7] = 5.0    # b3[pull]
+    s.set_params(p)
+    G.set_scheduler(s)
+    src = vs.sample_sources(g.V, 1024, seed=4)
+    ref, rst = vs.vexpand(G, src, 1, 4, sched="heuristic")
+    assert 2 in rst["mode_hop"]  # the heuristic pulls somewhere
+    want = ref.export(0, vs.LAYOUT_DVSM)
+    for rows, radius, forced in ((np.zeros((0, 8)), 0.0, True),       # no domain
+                                 (np.full((1, 8), 5.0), 0.01, False)):  # far from every decision
+        s.set_domain(rows, radius)
+        r, st = vs.vexpand(G, src, 1, 4, sched="learned")
+        assert np.array_equal(r.export(0, vs.LAYOUT_DVSM), want)
+        assert st["mode_hop"] == ([1] * len(st["mode_hop"]) if forced else rst["mode_hop"])
+        (b,), bst = vs.vexpand_batch_results(G, [vs.BatchQuery(src, 1, 4)], sched="learned")
+        assert np.array_equal(b.export(0, vs.LAYOUT_DVSM), want)
+        assert (bst["pull_decisions"] == 0) == forced
+        r.close()
+        b.close()
+    ref.close()
+    G.set_scheduler(None)

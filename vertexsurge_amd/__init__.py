@@ -253,6 +253,8 @@ def stats_dict(st: _lib.VexpandStats) -> dict:
         "n_push": st.n_push, "bytes_pull": st.bytes_pull, "bytes_push": st.bytes_push,
         "frontier_act": list(st.frontier_act[:k]), "frontier_deg": list(st.frontier_deg[:k]),
         "vis_frac": list(st.vis_frac[:k]),
+        "src_degsum": st.src_degsum, "src_vol2": st.src_vol2,
+        "tile_words_learned": bool(st.tile_words_learned),
     }
 
 
@@ -589,6 +591,7 @@ class Scheduler:
         n = C.c_size_t()
         _lib.call("vs_sched_num_params", C.byref(n))
         self.num_params = n.value
+        self.domain = None  # (rows, radius) once set
 
     @property
     def handle(self):
@@ -617,6 +620,14 @@ class Scheduler:
         _lib.call("vs_sched_train", self._h, _ptr(X, C.c_float), _ptr(Y, C.c_float),
                   _ptr(M, C.c_float), len(X), epochs, lr, _ptr(curve, C.c_float))
         return curve
+
+    def set_domain(self, rows, radius: float):
+        """Feature rows (normally the training samples) and a radius: the MLP
+        overrules the heuristic only for decisions within radius of a row."""
+        rows = np.ascontiguousarray(rows, dtype=np.float32).reshape(-1, _lib.MLP_IN)
+        _lib.call("vs_sched_set_domain", self._h, _ptr(rows, C.c_float), len(rows),
+                  float(radius))
+        self.domain = (rows.copy(), float(radius))
 
     def predict(self, X) -> np.ndarray:
         X = np.ascontiguousarray(X, dtype=np.float32)

@@ -32,6 +32,7 @@ GRAPH_SYMMETRIC = 0x1
 MAX_HOPS = 64
 MLP_IN, MLP_HIDDEN, MLP_OUT = 8, 64, 8
 NOALLOC_MAX_TUPLES = 4718592  # VS_NOALLOC_MAX_TUPLES
+SCHED_MAX_DOMAIN = 1024  # VS_SCHED_MAX_DOMAIN
 
 u32p = C.POINTER(C.c_uint32)
 u64p = C.POINTER(C.c_uint64)
@@ -59,7 +60,9 @@ class VexpandStats(C.Structure):
                 ("n_pull", C.c_uint64), ("n_push", C.c_uint64),
                 ("bytes_pull", C.c_uint64), ("bytes_push", C.c_uint64),
                 ("frontier_act", C.c_uint64 * MAX_HOPS), ("frontier_deg", C.c_uint64 * MAX_HOPS),
-                ("vis_frac", C.c_double * MAX_HOPS)]
+                ("vis_frac", C.c_double * MAX_HOPS),
+                ("src_degsum", C.c_uint64), ("src_vol2", C.c_uint64),
+                ("tile_words_learned", C.c_uint32), ("reserved2", C.c_uint32)]
 
 
 class Tuning(C.Structure):
@@ -184,6 +187,7 @@ SIGNATURES = {
     "vs_sched_train": (C.c_int, [vp, f32p, f32p, f32p, C.c_uint32, C.c_uint32, C.c_float,
                                  f32p]),
     "vs_sched_predict": (C.c_int, [vp, f32p, C.c_uint32, f32p]),
+    "vs_sched_set_domain": (C.c_int, [vp, f32p, C.c_uint32, C.c_float]),
     "vs_graph_set_scheduler": (C.c_int, [vp, vp]),
     "vs_autotune_batch": (C.c_int, [vp, C.POINTER(Query), C.c_uint32, C.POINTER(BatchParams),
                                     C.POINTER(Tuning), f64p, f64p]),

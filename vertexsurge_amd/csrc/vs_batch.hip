@@ -348,10 +348,12 @@ __global__ __launch_bounds__(64) void k_bw_decide(BatchConst c, uint32_t hop) {
                            : 0.f;
           }
           wave_sync();
-          float h1, h2, y;
-          mlp_forward_wave(c.mlp, s_x, s_h, h1, h2, y);
-          const float tp = __shfl(y, kOptPush), tl = __shfl(y, kOptPull);
-          if (isfinite(tp) && isfinite(tl) && fabsf(tp - tl) >= 0.1f) mode = tp < tl ? 1 : 2;
+          if (mlp_in_domain_wave(c.mlp, s_x)) {  // else the heuristic decides
+            float h1, h2, y;
+            mlp_forward_wave(c.mlp, s_x, s_h, h1, h2, y);
+            const float tp = __shfl(y, kOptPush), tl = __shfl(y, kOptPull);
+            if (isfinite(tp) && isfinite(tl) && fabsf(tp - tl) >= 0.1f) mode = tp < tl ? 1 : 2;
+          }
         }
       }
     }
@@ -1003,7 +1005,40 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   const uint32_t S = static_cast<uint32_t>(S64);
   const uint32_t W = static_cast<uint32_t>(W64);
   const uint32_t Sp = results ? W * 64 : S;  // >= S
-  const uint32_t TW = choose_tile_words(p.tile_words ? p.tile_words : g->tuning.tile_words, W);
+  // 2-hop volumes of the group's sources (group row order): the packing key,
+  // and with the learned scheduler the features of the tile-width choice
+  std::vector<uint64_t> key;
+  const bool learn_tw = p.sched == VS_SCHED_LEARNED && g->sched && p.tile_words == 0 &&
+                        g->tuning.tile_words == 0;
+  if (S && (learn_tw || !(p.flags & VS_BATCH_KEEP_ORDER))) {
+    std::vector<uint32_t> all(S);
+    uint32_t r = 0;
+    for (auto *q : qs)
+      for (uint32_t i = 0; i < q->S; ++i, ++r) {
+        if (q->sources[i] >= V) {
+          set_error("batch: source %u out of range (V=%u)", q->sources[i], V);
+          return VS_ERR_INVALID;
+        }
+        all[r] = q->sources[i];
+      }
+    key.resize(S);
+    VS_TRY(source_volumes(g, push, all.data(), S, key.data()));
+  }
+  uint32_t TW = choose_tile_words(p.tile_words ? p.tile_words : g->tuning.tile_words, W);
+  if (learn_tw && S) {
+    QueryFeatures qf;
+    qf.S = S;
+    qf.k_max = H;
+    qf.V = V;
+    qf.E = push->E;
+    uint32_t r = 0;
+    for (auto *q : qs)
+      for (uint32_t i = 0; i < q->S; ++i, ++r) {
+        qf.vol2 += key[r];
+        qf.degsum += push->hoff[q->sources[i] + 1] - push->hoff[q->sources[i]];
+      }
+    TW = sched_tile_words(g->sched, qf, choose_tile_words(64, W), TW);
+  }
   const uint32_t T = W ? (W + TW - 1) / TW : 0;
   const uint32_t Wpad = T * TW;
   const uint32_t RT = 64 * TW;
@@ -1078,11 +1113,7 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   // within-tile order.
   std::vector<uint32_t> pos(S);
   {
-    std::vector<uint64_t> key;
-    if (!(p.flags & VS_BATCH_KEEP_ORDER)) {
-      key.resize(S);
-      VS_TRY(source_volumes(g, push, src.data(), S, key.data()));
-    }
+    if (p.flags & VS_BATCH_KEEP_ORDER) key.clear();  // computed for the tile width only
     if (!results) {
       std::vector<uint32_t> order(S);
       for (uint32_t r = 0; r < S; ++r) order[r] = r;

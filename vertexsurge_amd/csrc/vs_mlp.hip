@@ -24,6 +24,9 @@ struct vs_sched_s {
   uint32_t step = 0;
   uint32_t grid = 64;
   hipStream_t stream = nullptr;
+  // decision domain (host copy of what follows the parameters on the device)
+  std::vector<float> domain;
+  float radius = 0.f;
 };
 
 namespace vs {
@@ -189,21 +192,77 @@ static vs_status sched_predict_dev(vs_sched_s *s, const float *d_x, uint32_t n,
 
 const float *sched_params(vs_sched_s *s) { return s ? s->d_params : nullptr; }
 
+void query_features(const QueryFeatures &f, float *x) {
+  x[0] = static_cast<float>(std::log2(1.0 + f.S) / 32.0);
+  x[1] = static_cast<float>(std::log2(1.0 + f.vol2) / 48.0);
+  x[2] = static_cast<float>(std::log2(1.0 + f.degsum) / 40.0);
+  x[3] = 0.f;
+  x[4] = 0.f;
+  x[5] = static_cast<float>(std::log2(1.0 + f.E) / 40.0);
+  x[6] = static_cast<float>(std::log2(1.0 + f.V) / 32.0);
+  x[7] = static_cast<float>(f.k_max / 8.0);
+}
+
+// Whether x lies within the scheduler's decision domain (no domain: yes).
+static bool in_domain(const vs_sched_s *s, const float *x) {
+  const size_t n = s->domain.size() / kIn;
+  if (n == 0) return true;
+  const float r2 = s->radius * s->radius;
+  for (size_t i = 0; i < n; ++i) {
+    float d = 0.f;
+    for (int k = 0; k < kIn; ++k) {
+      const float t = s->domain[i * kIn + k] - x[k];
+      d += t * t;
+    }
+    if (d <= r2) return true;
+  }
+  return false;
+}
+
+// One forward pass on the scheduler's device; false if anything fails.
+static bool predict_one(vs_sched_s *s, const float *x, float *y) {
+  if (hipSetDevice(s->device) != hipSuccess) { (void)hipGetLastError(); return false; }
+  // scratch: 2 * 64 floats after the loss slot
+  float *d_x = s->d_loss + 1, *d_y = s->d_loss + 1 + kIn;
+  if (hipMemcpyAsync(d_x, x, kIn * sizeof(float), hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+      sched_predict_dev(s, d_x, 1, d_y) != VS_OK ||
+      hipMemcpyAsync(y, d_y, kOut * sizeof(float), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+      hipStreamSynchronize(s->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return true;
+}
+
+uint32_t sched_tile_words(vs_sched_s *s, const QueryFeatures &f, uint32_t tw_max,
+                          uint32_t fallback) {
+  if (!s || tw_max < 16) return fallback;
+  float x[kIn], y[kOut];
+  query_features(f, x);
+  if (!in_domain(s, x) || !predict_one(s, x, y)) return fallback;
+  auto slot = [](uint32_t tw) { return tw == 16 ? kOptTW16 : tw == 32 ? kOptTW32 : kOptTW64; };
+  if (fallback != 16 && fallback != 32 && fallback != 64) return fallback;
+  uint32_t best = fallback;
+  float bp = y[slot(fallback)];
+  if (!std::isfinite(bp)) return fallback;
+  for (uint32_t tw = 16; tw <= tw_max && tw <= 64; tw *= 2) {
+    const float p = y[slot(tw)];
+    if (std::isfinite(p) && p < bp) {
+      bp = p;
+      best = tw;
+    }
+  }
+  // keep the fallback unless the pick is predicted clearly faster
+  if (best != fallback && y[slot(fallback)] - bp < 0.1f) return fallback;
+  return best;
+}
+
 int sched_decide(vs_sched_s *s, const SchedFeatures &f, const vs_tuning &t) {
   const int fallback = heuristic_decide(f, t);
   if (!s) return fallback;
   float x[kIn], y[kOut];
   sched_features(f, x);
-  if (hipSetDevice(s->device) != hipSuccess) { (void)hipGetLastError(); return fallback; }
-  // scratch: 2 * 64 floats after the loss slot
-  float *d_x = s->d_loss + 1, *d_y = s->d_loss + 1 + kIn;
-  if (hipMemcpyAsync(d_x, x, sizeof(x), hipMemcpyHostToDevice, s->stream) != hipSuccess ||
-      sched_predict_dev(s, d_x, 1, d_y) != VS_OK ||
-      hipMemcpyAsync(y, d_y, sizeof(y), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
-      hipStreamSynchronize(s->stream) != hipSuccess) {
-    (void)hipGetLastError();
-    return fallback;
-  }
+  if (!in_domain(s, x) || !predict_one(s, x, y)) return fallback;
   const float tp = y[kOptPush], tl = y[kOptPull];
   if (!std::isfinite(tp) || !std::isfinite(tl)) return fallback;
   // within 0.1 log2 units (~7%) the prediction cannot separate the options:
@@ -237,7 +296,8 @@ vs_status vs_sched_create(uint64_t seed, vs_sched_t *out) try {
   std::unique_ptr<vs_sched_s> s(new vs_sched_s);
   VS_HIP(hipGetDevice(&s->device));
   VS_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-  VS_HIP(hipMalloc(&s->d_params, kNumParams * 4));
+  VS_HIP(hipMalloc(&s->d_params, kDeviceFloats * 4));
+  VS_HIP(hipMemset(s->d_params, 0, kDeviceFloats * 4));  // no domain
   VS_HIP(hipMalloc(&s->d_m, kNumParams * 4));
   VS_HIP(hipMalloc(&s->d_v, kNumParams * 4));
   VS_HIP(hipMalloc(&s->d_grad, kNumParams * 4));
@@ -305,6 +365,25 @@ vs_status vs_sched_get_params(vs_sched_t s, float *params, size_t n) try {
   VS_HIP(hipMemcpy(params, s->d_params, kNumParams * 4, hipMemcpyDeviceToHost));
   return VS_OK;
 } VS_CATCH("vs_sched_get_params")
+
+vs_status vs_sched_set_domain(vs_sched_t s, const float *rows, uint32_t n, float radius) try {
+  clear_error();
+  if (!s || (n && !rows) || n > kMaxDomain || !(radius >= 0.f)) {
+    set_error("vs_sched_set_domain: need a scheduler, at most %u rows and a radius >= 0",
+              kMaxDomain);
+    return VS_ERR_INVALID;
+  }
+  std::vector<float> dev(2 + static_cast<size_t>(n) * kIn);
+  dev[0] = static_cast<float>(n);
+  dev[1] = radius;
+  if (n) std::memcpy(dev.data() + 2, rows, static_cast<size_t>(n) * kIn * 4);
+  VS_HIP(hipSetDevice(s->device));
+  VS_HIP(hipStreamSynchronize(s->stream));
+  VS_HIP(hipMemcpy(s->d_params + kOffDomain, dev.data(), dev.size() * 4, hipMemcpyHostToDevice));
+  s->domain.assign(rows, rows + static_cast<size_t>(n) * kIn);
+  s->radius = radius;
+  return VS_OK;
+} VS_CATCH("vs_sched_set_domain")
 
 vs_status vs_sched_set_params(vs_sched_t s, const float *params, size_t n) try {
   clear_error();

@@ -288,19 +288,48 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
   const bool with_distance = !find_shortest && p.with_distance;
   const uint32_t V = g->V, nvb = (V + 63) / 64;
   const uint32_t W = (S + 63) / 64;
-  const uint32_t TW = choose_tile_words(p.tile_words ? p.tile_words : g->tuning.tile_words, W);
-  const uint32_t T = S ? (W + TW - 1) / TW : 0;
   const uint32_t k_min = p.k_min, k_max = p.k_max;
+  const bool timing = (p.flags & VS_VX_TIMING) != 0;
+  // Row packing (below) and the learned tile-width choice read the sources'
+  // 2-hop volumes sum_{u in N(s)} deg(u), from one small kernel.
+  const bool pack = !(p.flags & VS_VX_KEEP_ORDER) && W > 1 &&
+                    k_max >= (find_shortest ? 2u : 3u);
+  const bool learn_tw = !find_shortest && p.sched == VS_SCHED_LEARNED && g->sched &&
+                        p.tile_words == 0 && g->tuning.tile_words == 0;
+  std::vector<uint64_t> key;
+  QueryFeatures qf;
+  if (S && (pack || learn_tw || timing)) {
+    key.resize(S);
+    VS_TRY(source_volumes(g, push, sources, S, key.data()));
+    qf.S = S;
+    qf.k_max = k_max;
+    qf.V = g->V;
+    qf.E = push->E;
+    for (uint32_t r = 0; r < S; ++r) {
+      qf.vol2 += key[r];
+      qf.degsum += push->hoff[sources[r] + 1] - push->hoff[sources[r]];
+    }
+  }
+  uint32_t TW = choose_tile_words(p.tile_words ? p.tile_words : g->tuning.tile_words, W);
+  bool tw_learned = false;
+  if (learn_tw && S) {
+    const uint32_t pick = sched_tile_words(g->sched, qf, choose_tile_words(64, W), TW);
+    tw_learned = pick != TW;
+    TW = pick;
+  }
+  const uint32_t T = S ? (W + TW - 1) / TW : 0;
   const size_t tile_words = static_cast<size_t>(V) * TW;
   const size_t mat_bytes = static_cast<size_t>(T) * tile_words * 8;
   const size_t bm_bytes = static_cast<size_t>(T) * nvb * 8;
-  const bool timing = (p.flags & VS_VX_TIMING) != 0;
   hipStream_t s = g->stream;
 
   if (st) {
     std::memset(st, 0, sizeof(*st));
     st->tile_words = TW;
     st->num_tiles = T;
+    st->src_degsum = qf.degsum;
+    st->src_vol2 = qf.vol2;
+    st->tile_words_learned = tw_learned ? 1u : 0u;
   }
   // Row packing: the rows run in packed order, sources and targets permuted
   // to match; pos[r] is caller row r's packed row (same tile).  The result
@@ -309,13 +338,9 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
   // on (fewer needed words per pull): used from k_max = 3, or 2 for
   // FindShortest, whose per-row distances are permuted on the host.
   const uint32_t *const caller_sources = sources;
-  const bool pack = !(p.flags & VS_VX_KEEP_ORDER) && W > 1 &&
-                    k_max >= (find_shortest ? 2u : 3u);
   const uint32_t RT = 64 * TW;
   std::vector<uint32_t> pk_src, pk_tgt, pos;
   if (pack) {
-    std::vector<uint64_t> key(S);
-    VS_TRY(source_volumes(g, push, sources, S, key.data()));
     std::vector<uint32_t> order(S);
     for (uint32_t r = 0; r < S; ++r) order[r] = r;
     for (uint32_t t0 = 0; t0 < S; t0 += RT) {

@@ -36,6 +36,41 @@ def features(hop, rows, V, E, pop, act, actdeg, vis_frac) -> np.ndarray:
     return np.array(x, dtype=np.float32)
 
 
+def query_features(S, vol2, degsum, k_max, V, E) -> np.ndarray:
+    """Same formula as query_features() in vs_mlp.hip: the query-level row of
+    the tile-width decision (hop feature 0)."""
+    x = [math.log2(1.0 + S) / 32.0, math.log2(1.0 + vol2) / 48.0,
+         math.log2(1.0 + degsum) / 40.0, 0.0, 0.0, math.log2(1.0 + E) / 40.0,
+         math.log2(1.0 + V) / 32.0, k_max / 8.0]
+    return np.array(x, dtype=np.float32)
+
+
+TW_SLOT = {16: 2, 32: 3, 64: 4}
+
+
+def trace_tile_widths(graph: Graph, V: int, E: int, sources, k_max: int, reps: int = 2,
+                      path="shortest"):
+    """One query-level sample: the whole query's device time at every tile
+    width it allows (16, 32, 64; median of reps), heuristic push/pull."""
+    W = (len(sources) + 63) // 64
+    widths = [tw for tw in (16, 32, 64) if tw <= max(1, 1 << (W - 1).bit_length())]
+    if len(widths) < 2:
+        return [], [], []
+    y = np.zeros(MLP_OUT, dtype=np.float32)
+    m = np.zeros(MLP_OUT, dtype=np.float32)
+    st = None
+    for tw in widths:
+        ms = []
+        for _ in range(reps):
+            r, st = vexpand(graph, sources, 1, k_max, path, tile_words=tw, timing=True)
+            r.close()
+            ms.append(st["ms_total"])
+        y[TW_SLOT[tw]] = math.log2(float(np.median(ms)))
+        m[TW_SLOT[tw]] = 1.0
+    x = query_features(len(sources), st["src_vol2"], st["src_degsum"], k_max, V, E)
+    return [x], [y], [m]
+
+
 def trace_query(graph: Graph, V: int, E: int, sources, k_max: int, path="shortest"):
     """Per-hop samples of one query: (X [n, IN], Y [n, OUT], M [n, OUT])."""
     _, sp = vexpand(graph, sources, 1, k_max, path, sched="push", timing=True)
@@ -61,7 +96,9 @@ def trace_query(graph: Graph, V: int, E: int, sources, k_max: int, path="shortes
     return X, Y, M
 
 
-def collect(graph_specs, source_sizes=(256, 1024, 4096), k_max=5, seed=1, log=print):
+def collect(graph_specs, source_sizes=(256, 1024, 4096), k_max=5, seed=1, log=print,
+            tw_sizes=(2048, 4096, 8192), tw_hops=(3, 4, 5)):
+    """Push/pull samples per (hop, tile) and tile-width samples per query."""
     X, Y, M = [], [], []
     for spec in graph_specs:
         t0 = time.time()
@@ -73,6 +110,15 @@ def collect(graph_specs, source_sizes=(256, 1024, 4096), k_max=5, seed=1, log=pr
             X += x
             Y += y
             M += m
+            log(f"  push/pull traces of {S} sources: {len(x)} samples")
+        for qi, S in enumerate(spec.get("tw_sources", tw_sizes)):
+            for k in tw_hops:
+                src = sample_sources(csr.V, min(S, csr.V), seed * 104729 + qi * 31 + k)
+                x, y, m = trace_tile_widths(G, csr.V, csr.E, src, k)
+                X += x
+                Y += y
+                M += m
+            log(f"  tile-width traces of {S} sources done")
         log(f"traced {spec}: {len(X)} samples so far ({time.time() - t0:.1f}s)")
         G.close()
         csr.close()
@@ -80,7 +126,7 @@ def collect(graph_specs, source_sizes=(256, 1024, 4096), k_max=5, seed=1, log=pr
 
 
 def make_graph(spec: dict) -> HostCSR:
-    spec = {k: v for k, v in spec.items() if k != "sources"}
+    spec = {k: v for k, v in spec.items() if k not in ("sources", "tw_sources")}
     if spec["kind"] == "rmat":
         a = spec.get("a", 0.57)
         bc = (1.0 - a - 0.05) / 2
@@ -90,16 +136,53 @@ def make_graph(spec: dict) -> HostCSR:
     raise ValueError(spec)
 
 
+def nn_distance(A, B, leave_one_out=False):
+    """Euclidean distance of every row of A to its nearest row of B."""
+    A = np.asarray(A, dtype=np.float64)
+    B = np.asarray(B, dtype=np.float64)
+    d = np.sqrt(((A[:, None, :] - B[None, :, :]) ** 2).sum(-1))
+    if leave_one_out:
+        np.fill_diagonal(d, np.inf)
+    return d.min(axis=1)
+
+
+def domain_of(X, max_rows=1024):
+    """The decision domain of a model trained on X: its samples (every k-th
+    when there are more than max_rows) and the largest leave-one-out
+    nearest-neighbour distance among them, so every training sample lies
+    inside and a decision farther from all of them than the training samples
+    are from each other falls back to the heuristic."""
+    X = np.asarray(X, dtype=np.float32)
+    rows = X[:: max(1, -(-len(X) // max_rows))]
+    radius = float(nn_distance(rows, rows, leave_one_out=True).max()) if len(rows) > 1 else 0.0
+    return rows, radius
+
+
 def train(X, Y, M, epochs=1500, lr=3e-3, seed=1) -> Scheduler:
     s = Scheduler(seed=seed)
     curve = s.train(X, Y, M, epochs=epochs, lr=lr)
     s.curve = curve
+    s.set_domain(*domain_of(X))
     return s
 
 
+def in_domain(X, domain):
+    if domain is None or len(domain[0]) == 0:
+        return np.ones(len(X), dtype=bool)
+    return nn_distance(X, domain[0]) <= domain[1] + 1e-6
+
+
+def hop_rows(X):
+    """Rows of per-hop push/pull samples (hop feature > 0)."""
+    return X[:, 3] > 0
+
+
 def accuracy(s: Scheduler, X, Y) -> dict:
-    """Fraction of samples where the MLP picks the faster option, and the time
-    lost against the oracle choice, next to the same for always-push/pull."""
+    """Fraction of push/pull samples where the MLP picks the faster option,
+    and the time lost against the oracle choice, next to the same for
+    always-push/pull."""
+    h = hop_rows(X)
+    X, Y = X[h], Y[h]
     P = s.predict(X)
     best = np.minimum(Y[:, 0], Y[:, 1])
     pick = np.where(P[:, 0] < P[:, 1], Y[:, 0], Y[:, 1])
@@ -109,14 +192,65 @@ def accuracy(s: Scheduler, X, Y) -> dict:
             "ms_pull": t(Y[:, 1])}
 
 
+def decide_push_pull(P, X, E, alpha=16, domain=None):
+    """The library's decision (sched_decide): the MLP's faster option unless
+    the sample lies outside the decision domain or the two predictions are
+    within 0.1 log2 units, then the density heuristic (push iff frontier
+    out-edges * alpha < E).  0 = push, 1 = pull."""
+    actdeg = np.exp2(X[:, 2].astype(np.float64) * 40.0) - 1.0
+    heur = np.where(actdeg * alpha < E, 0, 1)
+    mlp = np.where(P[:, 0] < P[:, 1], 0, 1)
+    keep = (np.abs(P[:, 0] - P[:, 1]) < 0.1) | ~in_domain(X, domain)
+    return np.where(keep, heur, mlp), heur
+
+
+def decide_tile_width(P, M, fallback=64, X=None, domain=None):
+    """The library's decision (sched_tile_words) over the widths a sample
+    allows (M): the lowest prediction, unless the sample lies outside the
+    decision domain or the pick is not 0.1 log2 units below the fallback's.
+    Returns output slots."""
+    slots = [TW_SLOT[tw] for tw in (16, 32, 64)]
+    fb = np.array([TW_SLOT[min(fallback, max(tw for tw in (16, 32, 64) if m[TW_SLOT[tw]]))]
+                   for m in M])
+    pick = fb.copy()
+    inside = in_domain(X, domain) if X is not None else np.ones(len(P), dtype=bool)
+    for n in range(len(P)):
+        if not inside[n]:
+            continue
+        allowed = [k for k in slots if M[n, k]]
+        best = min(allowed, key=lambda k: P[n, k])
+        if P[n, fb[n]] - P[n, best] >= 0.1:
+            pick[n] = best
+    return pick, fb
+
+
+def tw_accuracy(s: Scheduler, X, Y, M) -> dict:
+    """Tile-width samples: time of the learned choice against the fallback
+    (the widest the query allows) and the best width."""
+    q = ~hop_rows(X)
+    if not q.any():
+        return {"n": 0}
+    X, Y, M = X[q], Y[q], M[q]
+    pick, fb = decide_tile_width(s.predict(X), M, X=X, domain=s.domain)
+    r = np.arange(len(X))
+    t = lambda v: float(np.sum(np.exp2(v)))
+    best = np.array([min(Y[n, k] for k in (2, 3, 4) if M[n, k]) for n in r])
+    return {"n": int(len(X)), "ms_mlp": t(Y[r, pick]), "ms_fallback": t(Y[r, fb]),
+            "ms_best": t(best), "changed": int(np.sum(pick != fb))}
+
+
 def save(s: Scheduler, path: Path = WEIGHTS, meta: dict | None = None):
-    np.savez(path, params=s.get_params(), meta=np.array(json.dumps(meta or {})))
+    rows, radius = s.domain if s.domain is not None else (np.zeros((0, MLP_IN), np.float32), 0.0)
+    np.savez(path, params=s.get_params(), domain=rows, radius=np.array([radius], np.float32),
+             meta=np.array(json.dumps(meta or {})))
 
 
 def load(path: Path = WEIGHTS, seed=1) -> Scheduler:
     d = np.load(path, allow_pickle=False)
     s = Scheduler(seed=seed)
     s.set_params(d["params"])
+    if "domain" in d.files:
+        s.set_domain(d["domain"], float(d["radius"][0]))
     return s
 
 
@@ -136,12 +270,71 @@ C5_SPECS = ([{"kind": "rmat", "scale": 22, "ef": 32, "a": a, "seed": 300 + i,
             [{"kind": "rmat", "scale": 23, "ef": 32, "a": a, "seed": 310 + i,
               "sources": (256, 1024, 4096)} for i, a in enumerate((0.45, 0.55))] +
             [{"kind": "rmat", "scale": 24, "ef": 32, "a": a, "seed": 320 + i,
-              "sources": (256, 1024, 4096)} for i, a in enumerate((0.5, 0.6))])
+              "sources": (256, 1024, 4096)} for i, a in enumerate((0.5, 0.6))] +
+            [{"kind": "rmat", "scale": 25, "ef": 32, "a": 0.5, "seed": 330,
+              "sources": (256, 1024), "tw_sources": (2048, 4096)}])
 # held out: R-MAT scale 25 and an Erdos-Renyi graph with the same V and pairs
 C5_HELDOUT = [{"kind": "rmat", "scale": 25, "ef": 32, "a": 0.57, "seed": 777},
               {"kind": "er", "V": 1 << 25, "pairs": 32 << 25, "seed": 777}]
 PRESETS = {"small": (TRAIN_SPECS, HELDOUT_SPECS), "large": (LARGE_SPECS, LARGE_HELDOUT),
            "c5": (C5_SPECS, C5_HELDOUT)}
+
+
+def heldout_traces(s: Scheduler, specs, source_sizes=(128, 1024, 4096), k_max=5,
+                   tw_sizes=(2048, 4096, 8192), tw_hops=(3, 4, 5), log=print, save=None):
+    """Both decisions judged on measured traces of held-out graphs: per graph,
+    the summed time of the library's learned choices against the density
+    heuristic (push/pull) and against the default width (tile width), all on
+    the same measured table, so no timing noise separates them."""
+    out = []
+    for spec in specs:
+        csr = make_graph(spec)
+        G = Graph.from_csr(csr)
+        X, Y, M = collect_graph(G, csr, source_sizes, k_max, tw_sizes, tw_hops, seed=500)
+        G.close()
+        if save:
+            np.savez(f"{save}_{len(out)}.npz", X=X, Y=Y, M=M, E=np.array([csr.E]))
+        P = s.predict(X)
+        h = hop_rows(X)
+        row = {"graph": spec}
+        if h.any():
+            pick, heur = decide_push_pull(P[h], X[h], csr.E, domain=s.domain)
+            r = np.arange(int(h.sum()))
+            t = lambda k: float(np.sum(np.exp2(Y[h][r, k])))
+            row.update(hops=int(h.sum()), ms_heuristic=t(heur), ms_learned=t(pick),
+                       ms_best=float(np.sum(np.exp2(Y[h][:, :2].min(axis=1)))),
+                       changed=int(np.sum(pick != heur)))
+        q = ~h
+        if q.any():
+            pick, fb = decide_tile_width(P[q], M[q], X=X[q], domain=s.domain)
+            r = np.arange(int(q.sum()))
+            row.update(queries=int(q.sum()), tw_ms_default=float(np.sum(np.exp2(Y[q][r, fb]))),
+                       tw_ms_learned=float(np.sum(np.exp2(Y[q][r, pick]))),
+                       tw_changed=int(np.sum(pick != fb)))
+        log(json.dumps(row))
+        out.append(row)
+        csr.close()
+    return out
+
+
+def collect_graph(G, csr, source_sizes, k_max, tw_sizes, tw_hops, seed):
+    X, Y, M = [], [], []
+    for qi, S in enumerate(source_sizes):
+        x, y, m = trace_query(G, csr.V, csr.E, sample_sources(csr.V, min(S, csr.V), seed + qi),
+                              k_max)
+        X += x
+        Y += y
+        M += m
+        print(f"[sched_train]   held-out push/pull traces of {S} sources", file=sys.stderr,
+              flush=True)
+    for qi, S in enumerate(tw_sizes):
+        for k in tw_hops:
+            src = sample_sources(csr.V, min(S, csr.V), seed * 7 + qi * 31 + k)
+            x, y, m = trace_tile_widths(G, csr.V, csr.E, src, k)
+            X += x
+            Y += y
+            M += m
+    return np.array(X), np.array(Y), np.array(M)
 
 
 def evaluate(s: Scheduler, specs, sources=(1024, 4096), k_max=4, reps=3, log=print):
@@ -156,9 +349,12 @@ def evaluate(s: Scheduler, specs, sources=(1024, 4096), k_max=4, reps=3, log=pri
             src = sample_sources(csr.V, min(S, csr.V), 900 + qi)
             t = {}
             res = {}
+            tw_learned = None
             for sched in ("heuristic", "learned", "heuristic", "learned"):
                 r, st = vexpand(G, src, 1, k_max, sched=sched)
                 res[sched] = (r.row_counts(0), r.row_digests(0), st["te"])
+                if sched == "learned":
+                    tw_learned = st["tile_words"]
                 r.close()
             for sched in ("heuristic", "learned"):
                 ms = []
@@ -172,7 +368,7 @@ def evaluate(s: Scheduler, specs, sources=(1024, 4096), k_max=4, reps=3, log=pri
                 res["heuristic"][2] == res["learned"][2]
             row = {"graph": spec, "sources": int(S), "ms_heuristic": t["heuristic"],
                    "ms_learned": t["learned"], "bit_exact": bool(same),
-                   "te": int(res["learned"][2])}
+                   "te": int(res["learned"][2]), "tile_words_learned": tw_learned}
             log(json.dumps(row))
             out.append(row)
         G.set_scheduler(None)
@@ -192,15 +388,19 @@ def main(argv=None):
                          "c5: 22-24 / held-out R-MAT 25 + an Erdos-Renyi graph of its size")
     ap.add_argument("--eval-only", action="store_true", help="evaluate the saved weights")
     ap.add_argument("--no-eval", action="store_true", help="train and save only")
+    ap.add_argument("--no-timed-eval", action="store_true",
+                    help="judge on held-out traces only (skip the whole-query timings)")
     ap.add_argument("--traces", help="also save the traces (X, Y, M) to this .npz")
+    ap.add_argument("--heldout-traces", help="save each held-out graph's traces to PREFIX_i.npz")
     args = ap.parse_args(argv)
     preset = "large" if args.large else args.preset
     specs, held = PRESETS[preset]
     log = lambda m: print(f"[sched_train] {m}", file=sys.stderr, flush=True)
     if args.eval_only:
         s = load(Path(args.out))
-        rows = evaluate(s, held, log=log)
-        print(json.dumps({"heldout": rows}))
+        traces = heldout_traces(s, held, log=log, save=args.heldout_traces)
+        rows = [] if args.no_timed_eval else evaluate(s, held, log=log)
+        print(json.dumps({"heldout_traces": traces, "heldout": rows}))
         return
     if args.quick:
         specs = specs[::4]
@@ -211,6 +411,7 @@ def main(argv=None):
     s = train(X, Y, M, epochs=args.epochs)
     log(f"loss {s.curve[0]:.4f} -> {s.curve[-1]:.4f}")
     acc = accuracy(s, X, Y)
+    acc["tile_width"] = tw_accuracy(s, X, Y, M)
     log(f"train accuracy {acc}")
     save(s, Path(args.out), {"samples": int(len(X)), "train_acc": acc,
                              "specs": specs, "epochs": args.epochs, "preset": preset})
@@ -218,9 +419,10 @@ def main(argv=None):
         print(json.dumps({"samples": int(len(X)), "final_loss": float(s.curve[-1]),
                           "train": acc}))
         return
-    rows = evaluate(s, held, log=log)
+    traces = heldout_traces(s, held, log=log, save=args.heldout_traces)
+    rows = [] if args.no_timed_eval else evaluate(s, held, log=log)
     print(json.dumps({"samples": int(len(X)), "final_loss": float(s.curve[-1]),
-                      "train": acc, "heldout": rows}))
+                      "train": acc, "heldout_traces": traces, "heldout": rows}))
 
 
 if __name__ == "__main__":
