@@ -68,6 +68,8 @@ def parse():
                     help="scheduling of the plain leg")
     ap.add_argument("--tile-words", type=int, default=0, help="plain leg tile width")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", default="",
+                    help="tile_words,push_alpha,sparse_pct,pull_hubs for the graph (no autotune)")
     ap.add_argument("--weights", default=str(ROOT / "vertexsurge_amd" / "sched_weights.npz"))
     ap.add_argument("--cpu-sources", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
@@ -198,9 +200,12 @@ def main() -> int:
     log(f"plain: {plain['ms']:.2f} ms/step, {plain['gteps']:.1f} GTEPS (TE={plain['te']})")
 
     queries = [vs.BatchQuery(src, args.k_min, args.k_max, args.path)]
+    if args.tuning:
+        tw, alpha, sparse, hubs = (int(x) for x in args.tuning.split(","))
+        G.set_tuning(tile_words=tw, push_alpha=alpha, sparse_pct=sparse, pull_hubs=hubs)
     tuning = G.get_tuning()
     tune_ms = None
-    if not args.no_autotune:
+    if not args.no_autotune and not args.tuning:
         tuning, best_ms, start_ms = vs.autotune_batch(G, queries)
         tune_ms = {"start": round(start_ms, 3), "best": round(best_ms, 3)}
         log(f"autotuned (batched engine): {tuning} ({start_ms:.2f} -> {best_ms:.2f} ms)")

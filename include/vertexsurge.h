@@ -586,7 +586,7 @@ vs_status vs_mintersect_colmat(const vs_colmat_t *deps, uint32_t nd,
  * by vs_vexpand_batch (device time, the mean of two replays of the captured
  * graph after a warm one).  The starting tuning (the graph's, or
  * p->tile_words when set) is measured first (*start_ms) and a candidate is
- * kept only when faster, so *best_ms <= *start_ms.  Scheduling knobs never
+ * kept only when at least 2% faster, so *best_ms <= *start_ms.  Scheduling knobs never
  * change a result bit.  best, best_ms and start_ms may be NULL. */
 vs_status vs_autotune_batch(vs_graph_t g, const vs_query *queries, uint32_t nq,
                             const vs_batch_params *p, vs_tuning *best, double *best_ms,

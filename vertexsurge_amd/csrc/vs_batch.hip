@@ -1591,8 +1591,10 @@ vs_status vs_vexpand_batch_results(vs_graph_t g, const vs_query *queries, uint32
 // the caller's queries (device time: the mean of `reps` replays of the
 // captured launch sequence after one warm replay).  The starting tuning is
 // measured first and a candidate replaces the current choice only when it
-// is faster, so the result is never slower than the start by the tuner's own
-// clock.  No choice changes a result bit.
+// is at least 2% faster, so the result is never slower than the start by the
+// tuner's own clock.  No choice changes a result bit.
+constexpr double kTuneGain = 0.02;  // a candidate replaces the current choice only 2% faster
+
 static vs_status autotune_batch(vs_graph_t g, const vs_query *queries, uint32_t nq,
                                 const vs_batch_params &p0, vs_tuning *best, double *best_ms,
                                 double *start_ms) {
@@ -1635,7 +1637,9 @@ static vs_status autotune_batch(vs_graph_t g, const vs_query *queries, uint32_t 
       t.*knob = v;
       double ms = 0.0;
       VS_TRY(measure(t, &ms));
-      if (ms < bms) {
+      // a candidate must be clearly faster: differences inside the
+      // replay-to-replay spread would make the choice follow the noise
+      if (ms < bms * (1.0 - kTuneGain)) {
         bms = ms;
         cur = t;
       }
