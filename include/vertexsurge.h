@@ -578,6 +578,27 @@ vs_status vs_mintersect_colmat(const vs_colmat_t *deps, uint32_t nd,
                                uint64_t *offsets, uint32_t *lists, uint64_t list_cap,
                                uint64_t *total, vs_mintersect_stats *stats);
 
+/* Mintersect with a VarRowMatrixTable base (Mintersect::run_internal's
+ * VarRow-base branch, mintersect.cpp:244-333): every set bit of matrix i of a
+ * VExpand result is one base tuple (start vertex = the row's source, target
+ * vertex = the column's vertex; the table's iterators,
+ * bit_matrix_table.cpp:165-188), and dependency d intersects the column bound
+ * to base variable var_idx[d] (0 = start, 1 = target).  The dependencies are
+ * ColMain tables (mat_deps, as vs_mintersect_colmat: column lookup through
+ * the column map, missing column = empty, new vertices through deps[0]'s row
+ * map) or edge relations (graph_deps, as vs_mintersect: the bound vertex's
+ * adjacency; the reference's flat dependencies), exactly one of the two
+ * non-NULL.  Output: one (start, target, new vertex) tuple per element,
+ * ordered by base tuple (row, then target ascending) and new vertex
+ * ascending (the reference's order follows its threads).  The base tuples
+ * and intersections stay on the device.  With start == NULL only *n is set
+ * (the only_count total, mintersect.cpp:336-340); VS_ERR_CAPACITY when
+ * cap < *n. */
+vs_status vs_mintersect_varrow(vs_result_t base, uint32_t i, const vs_colmat_t *mat_deps,
+                               const vs_graph_t *graph_deps, uint32_t nd, const uint32_t *var_idx,
+                               uint32_t *start, uint32_t *target, uint32_t *nv, uint64_t cap,
+                               uint64_t *n, vs_mintersect_stats *stats);
+
 /* -------------------------------------------------------------- autotuner */
 /* Tunes the batched engine on the caller's queries and stores the result in
  * the graph's tuning: coordinate descent over tile_words (the widest the

@@ -20,12 +20,17 @@ col_map None or uint32 arrays of vertex ids (MatrixTable, matrix_table.hh:4-37).
 * mintersect_colmain -- Mintersect, flat base and ColMain dependencies
                         (mintersect.cpp:48-243 with tackle_di_for_base_vid,
                         :139-156; find_col_idx, bit_matrix_table.cpp:26-57).
+* mintersect_varrow  -- Mintersect, VarRow base (mintersect.cpp:244-333):
+                        base pairs from the set bits (varrow_tuples), ColMain
+                        or edge-relation dependencies.
 
 Pinned by: the reference's own MintersectFlatMatMat fixture (3 tuples,
 mintersect.cpp:411-512), a hand-computed word layout, the reference's
 TableConvert properties (big_operators2.cpp:215-297: m(i,j) == c(i,j), and
 == c(j,i) transposed), ColMain-of-adjacency intersections checked against the
-sorted-merge C oracle, and (count, id) ordering with ties worked by hand.
+sorted-merge C oracle, and (count, id) ordering with ties worked by hand;
+mintersect_varrow by a hand-worked triangle case and a brute-force triple loop
+(start, target, w) over every vertex w on small random graphs.
 """
 from __future__ import annotations
 
@@ -115,6 +120,70 @@ def mintersect_colmain(deps, tuples):
     np.cumsum(counts, out=offsets[1:])
     lists = np.concatenate(lists) if lists else np.zeros(0, dtype=np.uint32)
     return counts, offsets, lists
+
+
+def varrow_tuples(M, row_map):
+    """The set bits of a VarRow matrix as (row vertex, column vertex) pairs in
+    (row, column) order; the row vertex goes through row_map
+    (VarRowMatrixTable::Iterator::next, bit_matrix_table.cpp:147-160)."""
+    r, c = np.nonzero(np.asarray(M, dtype=bool))
+    if row_map is not None:
+        r = np.asarray(row_map, dtype=np.int64)[r]
+    return np.stack([r, c], axis=1).astype(np.uint32)
+
+
+def mintersect_varrow(M, row_map, deps, var_idx, adjacency=None):
+    """Mintersect with a VarRow base (mintersect.cpp:244-333): for every base
+    pair (start, target) dependency d intersects the column bound to
+    base variable var_idx[d]; deps are ColMain triples (as mintersect_colmain)
+    or, with adjacency = [(offsets, adj), ...], edge relations whose bound
+    vertex's neighbours form the set.  Returns (start, target, new) arrays,
+    ordered by base pair, then new vertex."""
+    base = varrow_tuples(M, row_map)
+    nd = len(var_idx)
+    tup = base[:, np.asarray(var_idx, dtype=np.int64)].reshape(-1, nd)
+    if adjacency is None:
+        # mintersect_colmain over chunks of tuples at once: row t of `ir` is
+        # the AND of the columns tuple t names (all False for a missing one)
+        R = np.asarray(deps[0][0]).shape[0]
+        cols = []
+        for d, (Md, _, cmap) in enumerate(deps):
+            Md = np.asarray(Md, dtype=bool)
+            lut = {}
+            ids = range(Md.shape[1]) if cmap is None else np.asarray(cmap).tolist()
+            for c, x in enumerate(ids):
+                lut.setdefault(int(x), c)
+            cols.append((Md.T, np.array([lut.get(int(x), -1) for x in tup[:, d]],
+                                        dtype=np.int64)))
+        counts, lists = [], []
+        for a in range(0, len(tup), 2048):
+            b = min(len(tup), a + 2048)
+            ir = np.ones((b - a, R), dtype=bool)
+            for MT, ci in cols:
+                ok = ci[a:b] >= 0
+                ir[~ok] = False
+                ir[ok] &= MT[ci[a:b][ok]]
+            t_i, r_i = np.nonzero(ir)
+            if deps[0][1] is not None:
+                r_i = np.asarray(deps[0][1])[r_i]
+            counts.append(np.bincount(t_i, minlength=b - a))
+            lists.append(r_i.astype(np.uint32))
+        counts = np.concatenate(counts) if counts else np.zeros(0, dtype=np.int64)
+        lists = np.concatenate(lists) if lists else np.zeros(0, dtype=np.uint32)
+    else:
+        counts, lists = [], []
+        for t in tup:
+            s = None
+            for d, (off, adj) in enumerate(adjacency):
+                v = int(t[d])
+                nb = np.asarray(adj[off[v]:off[v + 1]], dtype=np.uint32)
+                s = nb if s is None else np.intersect1d(s, nb)
+            counts.append(len(s))
+            lists.append(np.sort(s).astype(np.uint32))
+        counts = np.array(counts, dtype=np.int64)
+        lists = np.concatenate(lists) if lists else np.zeros(0, dtype=np.uint32)
+    rep = np.repeat(np.arange(len(base)), np.asarray(counts, dtype=np.int64))
+    return base[rep, 0], base[rep, 1], np.asarray(lists, dtype=np.uint32)
 
 
 def adjacency_colmain(offsets, adj, V: int):

@@ -160,3 +160,83 @@ def test_colmat_upload_clears_padding(g12):
     assert np.all(out[:, 0] == np.uint64(0xFFFFFFFFFFFFFFFF))
     assert np.all(out[:, 1] == np.uint64(0x3F)) and np.all(out[:, 2:] == 0)
     assert m.get(69, 2) and m.get(0, 0)
+
+
+@pytest.fixture(scope="module")
+def adj12(g12):
+    g, G = g12
+    A = ot.adjacency_colmain(g.offsets, g.adj, g.V)
+    cm = vs.ColMat.from_bool(G, A)
+    yield A, cm
+    cm.close()
+
+
+@pytest.mark.parametrize("tile_words,keep_order", [(4, False), (4, True), (64, False)])
+def test_mintersect_varrow_vs_oracle(orc, g12, adj12, tile_words, keep_order):
+    """Mintersect with a VarRow base: every set bit of a 1-hop VExpand result
+    is a (start, target) edge; ColMain adjacency and edge-relation
+    dependencies over the start or target vertex, against the oracle (rows
+    packed or in caller order, several tiles)."""
+    g, G = g12
+    A, cm = adj12
+    src = vs.sample_sources(g.V, 700, 31).astype(np.uint32)
+    src[::40] = src[2]  # repeated sources
+    r, _ = vs.vexpand(G, src, 1, 1, tile_words=tile_words, keep_order=keep_order)
+    M = _layer(orc, g, src, 1, 1)
+    for var_idx in ([0, 1], [1], [1, 0, 1]):
+        nd = len(var_idx)
+        es, et, ew = ot.mintersect_varrow(M, src, [(A, None, None)] * nd, var_idx)
+        for deps in ([cm] * nd, [G] * nd):
+            s, t, w = vs.mintersect_varrow(r, deps, var_idx)
+            assert len(s) == len(es) > 0, (var_idx, type(deps[0]))
+            assert np.array_equal(s, es) and np.array_equal(t, et), var_idx
+            assert np.array_equal(w, ew), var_idx
+            assert vs.mintersect_varrow(r, deps, var_idx, with_lists=False) == len(es)
+    r.close()
+
+
+def test_mintersect_varrow_row_and_col_maps(orc, g12):
+    """ColMain dependencies with a row map (new vertices go through it) and a
+    column map (a bound vertex with no column gives the empty set): the
+    transposed TableConvert of another VExpand."""
+    g, G = g12
+    src = vs.sample_sources(g.V, 250, 41)
+    r, _ = vs.vexpand(G, src, 1, 1, tile_words=2)
+    M = _layer(orc, g, src, 1, 1)
+    src2 = vs.sample_sources(g.V, 900, 42)
+    r2, _ = vs.vexpand(G, src2, 1, 2, path="any", tile_words=8)
+    M2 = _layer(orc, g, src2, 1, 2, "any")
+    cm = r2.to_colmain(transpose=True)  # rows: reached vertices, columns: src2
+    E, rm, cmap = ot.table_convert(M2, src2, None, True)
+    for var_idx in ([0], [1, 0]):
+        nd = len(var_idx)
+        es, et, ew = ot.mintersect_varrow(M, src, [(E, rm, cmap)] * nd, var_idx)
+        s, t, w = vs.mintersect_varrow(r, [cm] * nd, var_idx)
+        assert np.array_equal(s, es) and np.array_equal(t, et) and np.array_equal(w, ew)
+    cm.close()
+    r2.close()
+    r.close()
+
+
+def test_mintersect_varrow_capacity_and_errors(g12, adj12):
+    import ctypes as C
+    from vertexsurge_amd import _lib
+    g, G = g12
+    _, cm = adj12
+    r, _ = vs.vexpand(G, vs.sample_sources(g.V, 64, 5), 1, 1)
+    n = vs.mintersect_varrow(r, [G, G], [0, 1], with_lists=False)
+    assert n > 1
+    out = [np.zeros(n, dtype=np.uint32) for _ in range(3)]
+    got = C.c_uint64()
+    vi = np.array([0, 1], dtype=np.uint32)
+    arr = (C.c_void_p * 2)(G.handle, G.handle)
+    rc = _lib.load().vs_mintersect_varrow(r._h, 0, None, arr, 2,
+                                         vi.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                         *[o.ctypes.data_as(C.POINTER(C.c_uint32)) for o in out],
+                                         n - 1, C.byref(got), None)
+    assert rc == _lib.VS_ERR_CAPACITY and got.value == n
+    with pytest.raises(ValueError):
+        vs.mintersect_varrow(r, [cm, G], [0, 1])
+    with pytest.raises(RuntimeError):
+        vs.mintersect_varrow(r, [G], [2])
+    r.close()

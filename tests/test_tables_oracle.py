@@ -1,5 +1,6 @@
 """Pins of the table-operator oracle (oracle/tables.py) on the CPU."""
 import numpy as np
+import pytest
 
 from oracle import tables as ot
 
@@ -104,3 +105,50 @@ def test_planted_mistake_is_caught(orc):
     c_ref, _, _ = orc.mintersect_sorted([(off, adj)] * 2, tup)
     assert np.array_equal(c_ok, c_ref)
     assert not np.array_equal(c_bad, c_ref)
+
+
+def test_varrow_triangle_by_hand():
+    """Path 0-1-2 plus edge 0-2 (a triangle) and a pendant 2-3.  Base: rows
+    are sources {0, 2} (row map), bit (r, c) = c adjacent to the source.
+    Intersecting N(start) and N(target) lists the third corner of each
+    triangle through the base edge."""
+    off = np.array([0, 2, 4, 7, 8])
+    adj = np.array([1, 2, 0, 2, 0, 1, 3, 2], dtype=np.uint32)
+    M = np.zeros((2, 4), dtype=bool)
+    M[0, [1, 2]] = True  # N(0)
+    M[1, [0, 1, 3]] = True  # N(2)
+    s, t, w = ot.mintersect_varrow(M, np.array([0, 2]), None, [0, 1],
+                                   adjacency=[(off, adj)] * 2)
+    got = list(zip(s.tolist(), t.tolist(), w.tolist()))
+    assert got == [(0, 1, 2), (0, 2, 1), (2, 0, 1), (2, 1, 0)]
+    # the same with ColMain adjacency dependencies
+    A = ot.adjacency_colmain(off, adj, 4)
+    s2, t2, w2 = ot.mintersect_varrow(M, np.array([0, 2]), [(A, None, None)] * 2, [0, 1])
+    assert np.array_equal(s, s2) and np.array_equal(t, t2) and np.array_equal(w, w2)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_varrow_brute_force(orc, seed):
+    """Every (start, target, w) with bit (start, target) set and w in the sets
+    the var_idx choice names, by a triple loop over all vertices."""
+    import vertexsurge_amd as vs
+    g = vs.HostCSR.rmat(6, 4, seed=seed)
+    off, adj = np.asarray(g.offsets), np.asarray(g.adj)
+    rng = np.random.default_rng(seed)
+    src = rng.choice(g.V, size=9, replace=False).astype(np.uint32)
+    M = rng.random((9, g.V)) < 0.2
+    A = ot.adjacency_colmain(off, adj, g.V)
+    for var_idx in ([0, 1], [1, 1, 0], [1]):
+        exp = []
+        for r in range(9):
+            for c in range(g.V):
+                if not M[r, c]:
+                    continue
+                bound = [(int(src[r]), c)[v] for v in var_idx]
+                for w in range(g.V):
+                    if all(A[w, b] for b in bound):
+                        exp.append((int(src[r]), c, w))
+        for kw in (dict(deps=[(A, None, None)] * len(var_idx)),
+                   dict(deps=None, adjacency=[(off, adj)] * len(var_idx))):
+            s, t, w = ot.mintersect_varrow(M, src, var_idx=var_idx, **kw)
+            assert list(zip(s.tolist(), t.tolist(), w.tolist())) == exp

@@ -18,7 +18,7 @@ from ._lib import (DIR_BACKWARD, DIR_BOTH, DIR_FORWARD, LAYOUT_DVSM, LAYOUT_NATI
 
 __all__ = [
     "HostCSR", "Graph", "GraphStore", "store_write", "VExpandResult", "vexpand", "find_shortest", "mintersect",
-    "ColMat", "mintersect_colmat", "pack_colmain", "unpack_colmain",
+    "ColMat", "mintersect_colmat", "mintersect_varrow", "pack_colmain", "unpack_colmain",
     "sample_sources", "sample_paths3", "Scheduler", "vexpand_batch",
     "vexpand_batch_results", "autotune_batch", "BatchQuery", "device_count",
     "PATH_ANY", "PATH_SHORTEST", "PATH_FIND_SHORTEST", "DIR_FORWARD", "DIR_BACKWARD",
@@ -580,6 +580,35 @@ def mintersect_colmat(deps: list, tuples, with_lists: bool = True):
                   _ptr(lists, C.c_uint32), len(lists), C.byref(total), C.byref(st))
         lists = lists[:total.value]
     return counts, offsets, lists, {"ms": st.ms, "total": st.total, "bytes_alg": st.bytes_alg}
+
+
+def mintersect_varrow(base: "VExpandResult", deps: list, var_idx, i: int = 0,
+                      with_lists: bool = True):
+    """Mintersect over a VExpand result as the base table (the reference's
+    VarRow-base branch): dependency d intersects the column bound to the start
+    (var_idx[d] == 0) or target (1) vertex of every set bit.  deps: all ColMat
+    or all Graph.  Returns (start, target, new) arrays ordered by base tuple
+    and new vertex, or the total when with_lists is False."""
+    nd = len(deps)
+    vi = _u32(var_idx)
+    if len(vi) != nd:
+        raise ValueError("one base variable per dependency")
+    arr = (C.c_void_p * nd)(*[d.handle for d in deps])
+    mats = all(isinstance(d, ColMat) for d in deps)
+    if not mats and not all(isinstance(d, Graph) for d in deps):
+        raise ValueError("dependencies must be all ColMat or all Graph")
+    n = C.c_uint64()
+    st = _lib.MintersectStats()
+    args = (arr, None) if mats else (None, arr)
+    _lib.call("vs_mintersect_varrow", base._h, i, *args, nd, _ptr(vi, C.c_uint32), None, None,
+              None, 0, C.byref(n), C.byref(st))
+    if not with_lists:
+        return n.value
+    out = [np.zeros(max(1, n.value), dtype=np.uint32) for _ in range(3)]
+    if n.value:
+        _lib.call("vs_mintersect_varrow", base._h, i, *args, nd, _ptr(vi, C.c_uint32),
+                  *[_ptr(o, C.c_uint32) for o in out], n.value, C.byref(n), C.byref(st))
+    return tuple(o[:n.value] for o in out)
 
 
 class Scheduler:

@@ -176,6 +176,9 @@ SIGNATURES = {
     "vs_colmat_col_counts": (C.c_int, [vp, C.c_int32, u32p, i32p]),
     "vs_result_row_or": (C.c_int, [vp, C.c_uint32, u32p, C.c_uint32, u32p]),
     "vs_colmat_row_or": (C.c_int, [vp, u32p, C.c_uint32, u32p]),
+    "vs_mintersect_varrow": (C.c_int, [vp, C.c_uint32, C.POINTER(vp), C.POINTER(vp), C.c_uint32,
+                                       u32p, u32p, u32p, u32p, C.c_uint64, u64p,
+                                       C.POINTER(MintersectStats)]),
     "vs_mintersect_colmat": (C.c_int, [C.POINTER(vp), C.c_uint32, u32p, C.c_uint64, u32p, u64p,
                                        u32p, C.c_uint64, u64p, C.POINTER(MintersectStats)]),
     "vs_sched_create": (C.c_int, [C.c_uint64, C.POINTER(vp)]),

@@ -312,6 +312,18 @@ class CallerOrder {
   size_t bytes_ = 0;
 };
 
+// The flat pairs of matrix `which` of r (the union of all when UINT32_MAX)
+// in (row, target) order into device buffers from the graph's pool (the
+// caller returns them, `bytes` each); with want false only *n (vs_flat.hip).
+vs_status flat_pairs_device(vs_result_s *r, uint32_t which, bool want, uint32_t **d_start,
+                            uint32_t **d_target, size_t *bytes, uint64_t *n);
+// Multi-way intersection over edge relations with every array in device
+// memory (vs_mintersect_device, vs_mintersect.hip).
+vs_status mintersect_impl(vs_graph_s *const *deps, uint32_t arity, const uint32_t *d_tuples,
+                          uint64_t T, uint32_t *d_counts, uint64_t *d_offsets,
+                          uint32_t *d_lists, uint64_t list_cap, uint64_t *total,
+                          vs_mintersect_stats *st);
+
 // Device table of the stored -> caller row map of r (S uint32, pooled; the
 // destructor returns it), or nullptr when the rows are in caller order.
 class DestRows {

@@ -150,6 +150,105 @@ static void launch_flat_tw(uint32_t tw, bool fill, uint32_t grid, hipStream_t s,
   }
 }
 
+
+// The flat pairs of matrix `which` of r (every matrix's union when which is
+// UINT32_MAX), in (row, target) order, into device buffers from the graph's
+// pool that the caller returns: d_start / d_target hold *n entries (when
+// want is false only *n is computed).  Distances are not produced.
+vs_status flat_pairs_device(vs_result_s *r, uint32_t which, bool want, uint32_t **d_start,
+                            uint32_t **d_target, size_t *bytes, uint64_t *n) {
+  *n = 0;
+  *d_start = *d_target = nullptr;
+  *bytes = 0;
+  const size_t nm = which == UINT32_MAX ? r->mats.size() : 1;
+  if (nm == 0 || r->S == 0 || r->V == 0) return VS_OK;
+  if (nm > kFlatMaxLayers) {
+    set_error("flat pairs: %zu matrices (at most %u)", nm, kFlatMaxLayers);
+    return VS_ERR_UNSUPPORTED;
+  }
+  vs_graph_s *g = r->g;
+  VS_HIP(hipSetDevice(g->device));
+  hipStream_t s = g->stream;
+  FlatArgs f{};
+  CallerOrder co[kFlatMaxLayers];
+  for (size_t k = 0; k < nm; ++k) {
+    const uint32_t mi = which == UINT32_MAX ? static_cast<uint32_t>(k) : which;
+    VS_TRY(co[k].open(r, mi));
+    f.m[k] = co[k].d;
+    f.dist[k] = static_cast<int32_t>(r->mats[mi].distance);
+  }
+  f.nm = static_cast<uint32_t>(nm);
+  f.S = r->S;
+  f.V = r->V;
+  f.ntiles = r->ntiles;
+  f.nch = (r->V + kFlatChunk - 1) / kFlatChunk;
+  const size_t ncnt = static_cast<size_t>(r->S) * f.nch;
+  struct Buf { void *p = nullptr; size_t b = 0; };
+  std::vector<Buf> bufs;
+  struct Back {
+    Pool &p; std::vector<Buf> &v;
+    ~Back() { for (auto &x : v) if (x.p) p.put(x.p, x.b, false); }
+  } back{g->pool, bufs};
+  auto get = [&](size_t bytes, void **out) -> vs_status {
+    bufs.push_back({nullptr, bytes});
+    VS_TRY(g->pool.get(bytes, false, s, &bufs.back().p));
+    *out = bufs.back().p;
+    return VS_OK;
+  };
+  void *pc = nullptr, *po = nullptr;
+  VS_TRY(get(ncnt * 4, &pc));
+  VS_TRY(get((ncnt + 1) * 8, &po));
+  f.cnt = static_cast<uint32_t *>(pc);
+  f.off = static_cast<const uint64_t *>(po);
+  const uint32_t grid = grid_for(static_cast<uint64_t>(r->ntiles) * r->tw * f.nch, g->num_cus);
+  launch_flat_tw(r->tw, false, grid, s, f);
+  VS_HIP(hipGetLastError());
+  size_t tmp_bytes = 0;
+  VS_HIP(rocprim::exclusive_scan(nullptr, tmp_bytes, f.cnt, static_cast<uint64_t *>(po),
+                                 static_cast<uint64_t>(0), ncnt, rocprim::plus<uint64_t>(), s));
+  void *tmp = nullptr;
+  VS_TRY(get(std::max<size_t>(tmp_bytes, 8), &tmp));
+  VS_HIP(rocprim::exclusive_scan(tmp, tmp_bytes, f.cnt, static_cast<uint64_t *>(po),
+                                 static_cast<uint64_t>(0), ncnt, rocprim::plus<uint64_t>(), s));
+  uint64_t last_off = 0;
+  uint32_t last_cnt = 0;
+  VS_HIP(hipMemcpyAsync(&last_off, static_cast<uint64_t *>(po) + ncnt - 1, 8,
+                        hipMemcpyDeviceToHost, s));
+  VS_HIP(hipMemcpyAsync(&last_cnt, f.cnt + ncnt - 1, 4, hipMemcpyDeviceToHost, s));
+  VS_HIP(hipStreamSynchronize(s));
+  const uint64_t total = last_off + last_cnt;
+  *n = total;
+  if (!want || total == 0) return VS_OK;
+  void *rm = nullptr, *cm = nullptr, *ds = nullptr, *dt = nullptr;
+  if (!r->row_map.empty()) {
+    VS_TRY(get(r->row_map.size() * 4, &rm));
+    VS_HIP(hipMemcpyAsync(rm, r->row_map.data(), r->row_map.size() * 4, hipMemcpyHostToDevice, s));
+  }
+  if (!r->col_map.empty()) {
+    VS_TRY(get(r->col_map.size() * 4, &cm));
+    VS_HIP(hipMemcpyAsync(cm, r->col_map.data(), r->col_map.size() * 4, hipMemcpyHostToDevice, s));
+  }
+  // the outputs are the caller's: not in the scoped list
+  VS_TRY(g->pool.get(total * 4, false, s, &ds));
+  vs_status st = g->pool.get(total * 4, false, s, &dt);
+  if (st != VS_OK) {
+    g->pool.put(ds, total * 4, false);
+    return st;
+  }
+  *d_start = static_cast<uint32_t *>(ds);
+  *d_target = static_cast<uint32_t *>(dt);
+  *bytes = total * 4;
+  f.start = *d_start;
+  f.target = *d_target;
+  f.distout = nullptr;
+  f.row_map = static_cast<const uint32_t *>(rm);
+  f.col_map = static_cast<const uint32_t *>(cm);
+  launch_flat_tw(r->tw, true, grid, s, f);
+  VS_HIP(hipGetLastError());
+  VS_HIP(hipStreamSynchronize(s));  // the maps leave the pool on return
+  return VS_OK;
+}
+
 }  // namespace vs
 
 using namespace vs;

@@ -549,7 +549,7 @@ static uint64_t stage_limit_bytes() {
   return static_cast<uint64_t>(mb) << 20;
 }
 
-static vs_status mintersect_impl(const vs_graph_t *deps, uint32_t arity,
+vs_status mintersect_impl(const vs_graph_t *deps, uint32_t arity,
                                  const uint32_t *d_tuples, uint64_t T,
                                  uint32_t *d_counts, uint64_t *d_offsets,
                                  uint32_t *d_lists, uint64_t list_cap,

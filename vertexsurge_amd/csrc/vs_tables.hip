@@ -16,6 +16,7 @@
 // intersection streams columns through waves in fixed chunks (count pass ->
 // scan -> fill pass, outputs in ascending row order), every pass HBM-bound.
 #include <algorithm>
+#include <functional>
 #include <cstring>
 #include <memory>
 
@@ -496,12 +497,22 @@ static vs_status mc_count(vs_graph_s *g, const McArgs &a, uint64_t t0, uint64_t 
   return VS_OK;
 }
 
-static vs_status mintersect_colmat_impl(const vs_colmat_t *deps, uint32_t nd,
-                                        const uint32_t *tuples, uint64_t T, uint32_t *counts,
-                                        uint64_t *offsets, uint32_t *lists, uint64_t list_cap,
-                                        uint64_t *total, vs_mintersect_stats *st) {
-  if (st) std::memset(st, 0, sizeof(*st));
-  if (total) *total = 0;
+__global__ void k_mc_last(const uint32_t *counts, uint64_t *offsets, uint64_t T) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) offsets[T] = offsets[T - 1] + counts[T - 1];
+}
+
+// Device core of the ColMain intersection: tuples, counts and offsets (T + 1,
+// the exclusive scan of counts and the total) in device memory.  Once the
+// total is known, lists_for(total) gives the device list buffer (nullptr:
+// counts and offsets only; it may also fail, which ends the call with its
+// status).  *nfound counts the tuples whose columns all resolved.
+static vs_status mc_device(const vs_colmat_t *deps, uint32_t nd, const uint32_t *d_tuples,
+                           uint64_t T, uint32_t *d_counts, uint64_t *d_offsets,
+                           const std::function<vs_status(uint64_t, uint32_t **)> &lists_for,
+                           uint64_t *total, uint64_t *nfound, float *ms) {
+  *total = 0;
+  *nfound = 0;
+  *ms = 0.f;
   if (!deps || nd == 0 || nd > kMaxColmatDeps) {
     set_error("vs_mintersect_colmat: %u dependencies (1..%d)", nd, kMaxColmatDeps);
     return VS_ERR_INVALID;
@@ -520,18 +531,18 @@ static vs_status mintersect_colmat_impl(const vs_colmat_t *deps, uint32_t nd,
       return VS_ERR_INVALID;
     }
   }
-  if (lists && !offsets) { set_error("vs_mintersect_colmat: lists need offsets"); return VS_ERR_INVALID; }
-  if (T == 0) {
-    if (offsets) offsets[0] = 0;
-    return VS_OK;
-  }
-  if (!tuples || !counts) { set_error("vs_mintersect_colmat: null tuples/counts"); return VS_ERR_INVALID; }
   vs_graph_s *g = deps[0]->g;
   VS_HIP(hipSetDevice(g->device));
   for (uint32_t d = 0; d < nd; ++d)
     if (deps[d]->g != g) VS_HIP(hipStreamSynchronize(deps[d]->g->stream));
   hipStream_t s = g->stream;
-
+  if (T == 0) {
+    const uint64_t z = 0;
+    VS_HIP(hipMemcpyAsync(d_offsets, &z, 8, hipMemcpyHostToDevice, s));
+    VS_HIP(hipStreamSynchronize(s));
+    uint32_t *dl = nullptr;
+    return lists_for(0, &dl);
+  }
   McArgs a{};
   a.nd = nd;
   a.rows = deps[0]->rows;
@@ -547,16 +558,11 @@ static vs_status mintersect_colmat_impl(const vs_colmat_t *deps, uint32_t nd,
   }
   // target row map: the first ColMain dependency's (mintersect.cpp:115-122)
   a.row_map = deps[0]->d_row_map;
-
-  Scratch dt(g), dc(g), found(g), part(g), dcnt(g);
-  VS_TRY(dt.get(T * nd * 4, false));
+  a.tuples = d_tuples;
+  Scratch dc(g), found(g), part(g), tmp(g), pscan(g);
   VS_TRY(dc.get(T * nd * 4, false));
   VS_TRY(found.get(8, true));
-  VS_TRY(dcnt.get(T * 4, false));
-  VS_HIP(hipMemcpyAsync(dt.p, tuples, T * nd * 4, hipMemcpyHostToDevice, s));
-  a.tuples = dt.as<uint32_t>();
   a.colidx = dc.as<uint32_t>();
-
   hipEvent_t e0, e1;
   VS_HIP(hipEventCreate(&e0));
   VS_HIP(hipEventCreate(&e1));
@@ -574,63 +580,268 @@ static vs_status mintersect_colmat_impl(const vs_colmat_t *deps, uint32_t nd,
     const uint64_t nt = std::min(TB, T - t0);
     VS_TRY(mc_count(g, a, t0, nt, part.as<uint32_t>()));
     hipLaunchKernelGGL(k_mc_sum, dim3(grid_for((nt + 255) / 256, g->num_cus)), dim3(256), 0, s,
-                       part.as<uint32_t>(), a.nchunk, nt, dcnt.as<uint32_t>() + t0);
+                       part.as<uint32_t>(), a.nchunk, nt, d_counts + t0);
     VS_HIP(hipGetLastError());
   }
-  VS_HIP(hipMemcpyAsync(counts, dcnt.p, T * 4, hipMemcpyDeviceToHost, s));
-  uint64_t nfound = 0;
-  VS_HIP(hipMemcpyAsync(&nfound, found.p, 8, hipMemcpyDeviceToHost, s));
+  // offsets: exclusive scan of the counts, the total at T
+  size_t tb = 0;
+  VS_HIP(rocprim::exclusive_scan(nullptr, tb, d_counts, d_offsets, static_cast<uint64_t>(0),
+                                 static_cast<size_t>(T), rocprim::plus<uint64_t>(), s));
+  const size_t pitems = TB * a.nchunk;
+  size_t tb2 = 0;
+  VS_HIP(rocprim::exclusive_scan(nullptr, tb2, part.as<uint32_t>(), static_cast<uint64_t *>(nullptr),
+                                 static_cast<uint64_t>(0), pitems, rocprim::plus<uint64_t>(), s));
+  VS_TRY(tmp.get(std::max(tb, tb2), false));
+  VS_HIP(rocprim::exclusive_scan(tmp.p, tb, d_counts, d_offsets, static_cast<uint64_t>(0),
+                                 static_cast<size_t>(T), rocprim::plus<uint64_t>(), s));
+  hipLaunchKernelGGL(k_mc_last, dim3(1), dim3(64), 0, s, d_counts, d_offsets, T);
+  VS_HIP(hipGetLastError());
+  uint64_t tot = 0, nf = 0;
+  VS_HIP(hipMemcpyAsync(&tot, d_offsets + T, 8, hipMemcpyDeviceToHost, s));
+  VS_HIP(hipMemcpyAsync(&nf, found.p, 8, hipMemcpyDeviceToHost, s));
   VS_HIP(hipStreamSynchronize(s));
-  uint64_t tot = 0;
-  for (uint64_t t = 0; t < T; ++t) {
-    if (offsets) offsets[t] = tot;
-    tot += counts[t];
-  }
-  if (offsets) offsets[T] = tot;
-  if (total) *total = tot;
-  if (lists) {
-    if (tot > list_cap) {
-      set_error("vs_mintersect_colmat: list capacity %llu < total %llu",
-                (unsigned long long)list_cap, (unsigned long long)tot);
-      return VS_ERR_CAPACITY;
-    }
-    Scratch dl(g), pscan(g), tmp(g);
-    VS_TRY(dl.get(tot * 4, false));
-    VS_TRY(pscan.get(TB * a.nchunk * 8, false));
-    size_t tb = 0;
-    VS_HIP(rocprim::exclusive_scan(nullptr, tb, part.as<uint32_t>(), pscan.as<uint64_t>(),
-                                   static_cast<uint64_t>(0),
-                                   static_cast<size_t>(TB * a.nchunk),
-                                   rocprim::plus<uint64_t>(), s));
-    VS_TRY(tmp.get(tb, false));
+  *total = tot;
+  *nfound = nf;
+  uint32_t *d_lists = nullptr;
+  VS_TRY(lists_for(tot, &d_lists));
+  if (d_lists && tot) {
+    VS_TRY(pscan.get(pitems * 8, false));
     for (uint64_t t0 = 0; t0 < T; t0 += TB) {
       const uint64_t nt = std::min(TB, T - t0);
       const uint64_t nitems = nt * a.nchunk;
+      uint64_t base = 0;
+      VS_HIP(hipMemcpyAsync(&base, d_offsets + t0, 8, hipMemcpyDeviceToHost, s));
       VS_TRY(mc_count(g, a, t0, nt, part.as<uint32_t>()));
-      VS_HIP(rocprim::exclusive_scan(tmp.p, tb, part.as<uint32_t>(), pscan.as<uint64_t>(),
+      VS_HIP(rocprim::exclusive_scan(tmp.p, tb2, part.as<uint32_t>(), pscan.as<uint64_t>(),
                                      static_cast<uint64_t>(0), static_cast<size_t>(nitems),
                                      rocprim::plus<uint64_t>(), s));
+      VS_HIP(hipStreamSynchronize(s));  // base is read below
       hipLaunchKernelGGL(k_mc_fill, dim3(grid_for(nitems, g->num_cus)), dim3(kBlock), 0, s, a,
-                         t0, nitems, pscan.as<uint64_t>(), offsets[t0], dl.as<uint32_t>());
+                         t0, nitems, pscan.as<uint64_t>(), base, d_lists);
       VS_HIP(hipGetLastError());
     }
-    VS_HIP(hipEventRecord(e1, s));
-    if (tot) VS_HIP(hipMemcpyAsync(lists, dl.p, tot * 4, hipMemcpyDeviceToHost, s));
-    VS_HIP(hipStreamSynchronize(s));
-  } else {
-    VS_HIP(hipEventRecord(e1, s));
-    VS_HIP(hipEventSynchronize(e1));
+  }
+  VS_HIP(hipEventRecord(e1, s));
+  VS_HIP(hipEventSynchronize(e1));
+  VS_HIP(hipEventElapsedTime(ms, e0, e1));
+  return VS_OK;
+}
+
+static vs_status mintersect_colmat_impl(const vs_colmat_t *deps, uint32_t nd,
+                                        const uint32_t *tuples, uint64_t T, uint32_t *counts,
+                                        uint64_t *offsets, uint32_t *lists, uint64_t list_cap,
+                                        uint64_t *total, vs_mintersect_stats *st) {
+  if (st) std::memset(st, 0, sizeof(*st));
+  if (total) *total = 0;
+  if (!deps || nd == 0 || nd > kMaxColmatDeps || !deps[0]) {
+    set_error("vs_mintersect_colmat: %u dependencies (1..%d)", nd, kMaxColmatDeps);
+    return VS_ERR_INVALID;
+  }
+  if (lists && !offsets) { set_error("vs_mintersect_colmat: lists need offsets"); return VS_ERR_INVALID; }
+  if (T == 0) {
+    if (offsets) offsets[0] = 0;
+    return VS_OK;
+  }
+  if (!tuples || !counts) { set_error("vs_mintersect_colmat: null tuples/counts"); return VS_ERR_INVALID; }
+  vs_graph_s *g = deps[0]->g;
+  VS_HIP(hipSetDevice(g->device));
+  hipStream_t s = g->stream;
+  Scratch dt(g), dcnt(g), doff(g), dl(g);
+  VS_TRY(dt.get(T * nd * 4, false));
+  VS_TRY(dcnt.get(T * 4, false));
+  VS_TRY(doff.get((T + 1) * 8, false));
+  VS_HIP(hipMemcpyAsync(dt.p, tuples, T * nd * 4, hipMemcpyHostToDevice, s));
+  bool over = false;
+  auto lists_for = [&](uint64_t tot, uint32_t **out) -> vs_status {
+    *out = nullptr;
+    if (!lists || tot == 0) return VS_OK;
+    if (tot > list_cap) {
+      over = true;
+      return VS_OK;
+    }
+    VS_TRY(dl.get(tot * 4, false));
+    *out = dl.as<uint32_t>();
+    return VS_OK;
+  };
+  uint64_t tot = 0, nfound = 0;
+  float ms = 0.f;
+  VS_TRY(mc_device(deps, nd, dt.as<uint32_t>(), T, dcnt.as<uint32_t>(), doff.as<uint64_t>(),
+                   lists_for, &tot, &nfound, &ms));
+  VS_HIP(hipMemcpyAsync(counts, dcnt.p, T * 4, hipMemcpyDeviceToHost, s));
+  if (offsets) VS_HIP(hipMemcpyAsync(offsets, doff.p, (T + 1) * 8, hipMemcpyDeviceToHost, s));
+  if (lists && !over && tot) VS_HIP(hipMemcpyAsync(lists, dl.p, tot * 4, hipMemcpyDeviceToHost, s));
+  VS_HIP(hipStreamSynchronize(s));
+  if (total) *total = tot;
+  if (over) {
+    set_error("vs_mintersect_colmat: list capacity %llu < total %llu",
+              (unsigned long long)list_cap, (unsigned long long)tot);
+    return VS_ERR_CAPACITY;
   }
   if (st) {
-    float ms = 0.f;
-    VS_HIP(hipEventElapsedTime(&ms, e0, e1));
     st->ms = ms;
     st->total = tot;
     // compulsory bytes: the columns of every fully resolved tuple, the
     // tuples, counts, offsets and lists
-    st->bytes_alg = nfound * nd * a.used * 8 + 4ull * nd * T + 4 * T +
+    const uint64_t used = (static_cast<uint64_t>(deps[0]->rows) + 63) / 64;
+    st->bytes_alg = nfound * nd * used * 8 + 4ull * nd * T + 4 * T +
                     (offsets ? 8 * (T + 1) : 0) + (lists ? 4 * tot : 0);
   }
+  return VS_OK;
+}
+
+// ------------------------------------------- Mintersect, VarRow base table
+// Base tuple t of the VarRow-base branch: column d is the pair's start
+// vertex, or its target when bit d of `sel` is set.
+__global__ void k_vr_tuples(const uint32_t *__restrict__ start, const uint32_t *__restrict__ target,
+                            uint64_t P, uint32_t nd, uint32_t sel, uint32_t *__restrict__ tuples) {
+  for (uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; t < P;
+       t += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    for (uint32_t d = 0; d < nd; ++d) tuples[t * nd + d] = ((sel >> d) & 1) ? target[t] : start[t];
+}
+
+// (start, target) of base tuple t repeated over its output run
+// offsets[t] .. + counts[t]: a thread writes a short run itself, the wave
+// writes the long runs of its lanes together.
+__global__ __launch_bounds__(kBlock) void k_vr_spread(const uint32_t *__restrict__ start,
+                                                     const uint32_t *__restrict__ target,
+                                                     const uint32_t *__restrict__ counts,
+                                                     const uint64_t *__restrict__ offsets,
+                                                     uint64_t P, uint32_t *__restrict__ out_s,
+                                                     uint32_t *__restrict__ out_t) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  const uint64_t lim = (P + 63) / 64 * 64;  // whole waves stay in the loop
+  for (uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; t < lim;
+       t += stride) {
+    const uint32_t n = t < P ? counts[t] : 0;
+    const uint64_t o = n ? offsets[t] : 0;
+    const uint32_t a = n ? start[t] : 0, b = n ? target[t] : 0;
+    const bool big = n > 16;
+    if (!big)
+      for (uint32_t k = 0; k < n; ++k) {
+        out_s[o + k] = a;
+        out_t[o + k] = b;
+      }
+    for (uint64_t m = __ballot(big); m; m &= m - 1) {
+      const int l = __ffsll(static_cast<long long>(m)) - 1;
+      const uint32_t nl = __shfl(n, l), al = __shfl(a, l), bl = __shfl(b, l);
+      const uint64_t ol = shfl64(o, l);
+      for (uint32_t k = lane_id(); k < nl; k += 64) {
+        out_s[ol + k] = al;
+        out_t[ol + k] = bl;
+      }
+    }
+  }
+}
+
+static vs_status mintersect_varrow(vs_result_t base, uint32_t i, const vs_colmat_t *mat_deps,
+                                   const vs_graph_t *graph_deps, uint32_t nd,
+                                   const uint32_t *var_idx, uint32_t *start, uint32_t *target,
+                                   uint32_t *nv, uint64_t cap, uint64_t *n,
+                                   vs_mintersect_stats *st) {
+  const char *fn = "vs_mintersect_varrow";
+  if (st) std::memset(st, 0, sizeof(*st));
+  if (!n) { set_error("%s: null count", fn); return VS_ERR_INVALID; }
+  *n = 0;
+  VS_TRY(result_mat(base, i));
+  if ((mat_deps == nullptr) == (graph_deps == nullptr)) {
+    set_error("%s: give either ColMain or edge-relation dependencies", fn);
+    return VS_ERR_INVALID;
+  }
+  if (nd == 0 || nd > kMaxColmatDeps || !var_idx) {
+    set_error("%s: %u dependencies (1..%d) and their base variables", fn, nd, kMaxColmatDeps);
+    return VS_ERR_INVALID;
+  }
+  if (start && (!target || !nv)) { set_error("%s: null target / new-vertex output", fn); return VS_ERR_INVALID; }
+  uint32_t sel = 0;
+  for (uint32_t d = 0; d < nd; ++d) {
+    if (var_idx[d] > 1) {
+      set_error("%s: base variable %u of dependency %u (0 = start, 1 = target)", fn, var_idx[d], d);
+      return VS_ERR_INVALID;
+    }
+    const int dev = mat_deps ? (mat_deps[d] ? mat_deps[d]->g->device : -1)
+                             : (graph_deps[d] ? graph_deps[d]->device : -1);
+    if (dev != base->g->device) {
+      set_error("%s: dependency %u missing or on another device", fn, d);
+      return VS_ERR_INVALID;
+    }
+    sel |= var_idx[d] << d;
+  }
+  vs_graph_s *g = base->g;
+  VS_HIP(hipSetDevice(g->device));
+  hipStream_t s = g->stream;
+  // base tuples: the set bits of the base matrix, in (row, target) order
+  // (VarRowMatrixTable iterators, bit_matrix_table.cpp:165-188)
+  uint32_t *ps = nullptr, *pt = nullptr;
+  size_t pbytes = 0;
+  uint64_t P = 0;
+  VS_TRY(flat_pairs_device(base, i, true, &ps, &pt, &pbytes, &P));
+  struct Pairs {
+    Pool &pool; void *a, *b; size_t n;
+    ~Pairs() { if (a) pool.put(a, n, false); if (b) pool.put(b, n, false); }
+  } pairs{g->pool, ps, pt, pbytes};
+  if (P == 0) return VS_OK;
+  Scratch tup(g), cnt(g), off(g), lst(g), os(g), ot(g);
+  VS_TRY(tup.get(P * nd * 4, false));
+  VS_TRY(cnt.get(P * 4, false));
+  VS_TRY(off.get((P + 1) * 8, false));
+  hipLaunchKernelGGL(k_vr_tuples, dim3(grid_for((P + 255) / 256, g->num_cus)), dim3(256), 0, s,
+                     ps, pt, P, nd, sel, tup.as<uint32_t>());
+  VS_HIP(hipGetLastError());
+  uint64_t tot = 0;
+  float ms = 0.f;
+  const bool want = start != nullptr;
+  bool over = false;
+  uint32_t *d_lists = nullptr;
+  if (mat_deps) {
+    uint64_t nfound = 0;
+    auto lists_for = [&](uint64_t total, uint32_t **out) -> vs_status {
+      *out = nullptr;
+      if (!want || total == 0) return VS_OK;
+      if (total > cap) { over = true; return VS_OK; }
+      VS_TRY(lst.get(total * 4, false));
+      *out = d_lists = lst.as<uint32_t>();
+      return VS_OK;
+    };
+    VS_TRY(mc_device(mat_deps, nd, tup.as<uint32_t>(), P, cnt.as<uint32_t>(), off.as<uint64_t>(),
+                     lists_for, &tot, &nfound, &ms));
+  } else {
+    std::vector<vs_graph_s *> gd(graph_deps, graph_deps + nd);
+    vs_mintersect_stats ms1{}, ms2{};
+    VS_TRY(mintersect_impl(gd.data(), nd, tup.as<uint32_t>(), P, cnt.as<uint32_t>(),
+                           off.as<uint64_t>(), nullptr, 0, &tot, &ms1));
+    ms = static_cast<float>(ms1.ms);
+    if (want && tot && tot <= cap) {
+      VS_TRY(lst.get(tot * 4, false));
+      d_lists = lst.as<uint32_t>();
+      VS_TRY(mintersect_impl(gd.data(), nd, tup.as<uint32_t>(), P, cnt.as<uint32_t>(),
+                             off.as<uint64_t>(), d_lists, tot, &tot, &ms2));
+      ms += static_cast<float>(ms2.ms);
+    } else if (want && tot > cap) {
+      over = true;
+    }
+  }
+  *n = tot;
+  if (st) {
+    st->ms = ms;
+    st->total = tot;
+  }
+  if (!want) return VS_OK;
+  if (over) {
+    set_error("%s: capacity %llu < %llu tuples", fn, static_cast<unsigned long long>(cap),
+              static_cast<unsigned long long>(tot));
+    return VS_ERR_CAPACITY;
+  }
+  if (tot == 0) return VS_OK;
+  VS_TRY(os.get(tot * 4, false));
+  VS_TRY(ot.get(tot * 4, false));
+  hipLaunchKernelGGL(k_vr_spread, dim3(grid_for((P + 63) / 64, g->num_cus)), dim3(kBlock), 0, s,
+                     ps, pt, cnt.as<uint32_t>(), off.as<uint64_t>(), P, os.as<uint32_t>(),
+                     ot.as<uint32_t>());
+  VS_HIP(hipGetLastError());
+  VS_HIP(hipMemcpyAsync(start, os.p, tot * 4, hipMemcpyDeviceToHost, s));
+  VS_HIP(hipMemcpyAsync(target, ot.p, tot * 4, hipMemcpyDeviceToHost, s));
+  VS_HIP(hipMemcpyAsync(nv, d_lists, tot * 4, hipMemcpyDeviceToHost, s));
+  VS_HIP(hipStreamSynchronize(s));
   return VS_OK;
 }
 
@@ -892,6 +1103,15 @@ vs_status vs_colmat_row_or(vs_colmat_t m, uint32_t *ids, uint32_t cap, uint32_t 
   VS_TRY(colmat_col_counts_dev(m, cnt));
   return finish_row_or(m->g, cnt.as<uint32_t>(), m->cols, m->col_map, ids, cap, n);
 } VS_CATCH("vs_colmat_row_or")
+
+vs_status vs_mintersect_varrow(vs_result_t base, uint32_t i, const vs_colmat_t *mat_deps,
+                               const vs_graph_t *graph_deps, uint32_t nd, const uint32_t *var_idx,
+                               uint32_t *start, uint32_t *target, uint32_t *nv, uint64_t cap,
+                               uint64_t *n, vs_mintersect_stats *stats) try {
+  clear_error();
+  return mintersect_varrow(base, i, mat_deps, graph_deps, nd, var_idx, start, target, nv, cap, n,
+                           stats);
+} VS_CATCH("vs_mintersect_varrow")
 
 vs_status vs_mintersect_colmat(const vs_colmat_t *deps, uint32_t nd, const uint32_t *tuples,
                                uint64_t T, uint32_t *counts, uint64_t *offsets, uint32_t *lists,
