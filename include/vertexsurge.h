@@ -215,13 +215,24 @@ vs_status vs_store_csr(vs_store_t s, const uint64_t **offsets, const uint32_t **
 vs_status vs_store_graph(vs_store_t s, vs_graph_t *out);
 vs_status vs_store_close(vs_store_t s);
 /* Writes a CSR as a store the reference can load (gen_and_write_one_edge,
- * graph_loader.hh:152-200, for one relation of vertex_type to itself): the
- * forward CSR files, the reverse CSR files when directed, and schema.json
- * with no flat edge files listed.  An undirected relation's adj must hold
- * both directions of every edge.  dir must exist. */
+ * graph_loader.hh:152-276, for one relation of vertex_type to itself): the
+ * forward CSR files, the reverse CSR files when directed, and schema.json.
+ * With flat_partitions = P > 0 it also writes, for each stored direction, the
+ * flat edge files the reference's importers list (store_info flats and
+ * partitioned_flats {src_dst, hilbert}, partition_counts {P, P};
+ * graph_loader.cpp:216-223): "<name>_src_dst" (edges by source, then
+ * destination), "<name>_hilbert" (Peano-Hilbert order of (src, dst) on the
+ * next power-of-two grid, peano.hh:73-89), and per destination partition i
+ * (partition_by_dst, graph_loader.hh:111-139) "<name>_src_dst_<i>" and
+ * "<name>_hilbert_<i>" -- the files the reference's Dense Hilbert and
+ * BlockHilbert expansions and its ColMain expansion open (expand.cpp:197,
+ * 221, 266; schema.cpp:382-389).  Backward files hold the pairs reversed.
+ * With P = 0 no flat files are written or listed.  An undirected relation's
+ * adj must hold both directions of every edge.  dir must exist. */
 vs_status vs_store_write(const char *dir, const char *graph_name, const char *vertex_type,
                          const char *edge_type, int32_t directed, const uint64_t *offsets,
-                         const uint32_t *adj, uint32_t num_vertices, uint64_t num_edges);
+                         const uint32_t *adj, uint32_t num_vertices, uint64_t num_edges,
+                         uint32_t flat_partitions);
 
 /* ---------------------------------------------------------------- VExpand */
 /* Parameters of VExpand (big_operators.hh:101-129, vexpand.cpp:7-101). */

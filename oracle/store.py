@@ -16,11 +16,22 @@ Only tests/ use this module; the product path never imports it.
 * store_names -- file names (schema.cpp:11-49).
 * read_store -- what opening a relation yields: the source-side CSR with
   each list made an ascending set, plus the counts the library reports.
+* hilbert_key -- position on the Peano-Hilbert curve (XYToIndex with
+  PeanoDirection::RA, peano.hh:73-89), here as the textbook rotate-and-reflect
+  walk over the coordinate bits.
+* partition_by_dst -- destination partitions (graph_loader.hh:111-139).
+* flat_files -- the flat edge files of one relation (gen_and_write_one_edge,
+  graph_loader.hh:186-276, with flats / partitioned_flats {src_dst, hilbert}).
 
 Pinned by: a hand-derived fixture (tests/golden/store/, made by
 tests/golden/make_store_fixture.py from edge lists worked through the importer
 steps by hand, including the adjacent-only duplicate removal), and by
-round trips through the library's own writer.
+round trips through the library's own writer; hilbert_key by the 4 x 4 curve
+worked by hand from the reference's quadrant orders, by the reference's
+quadrant-order / sub-curve tables walked cell by cell up to 64 x 64, and by
+the reference's own Hilbert test property (consecutive cells of a full grid
+share a row or a column, gtest.cpp:526-573); partition_by_dst by hand-worked
+cuts.
 """
 from __future__ import annotations
 
@@ -138,3 +149,78 @@ def read_store(directory, edge: str | None = None):
     adj = (np.concatenate(out) if out else np.zeros(0)).astype(np.uint32)
     info["num_edges"] = len(adj)
     return V, np.array(offs, dtype=np.uint64), adj, info
+
+
+def hilbert_key(n: int, x, y) -> np.ndarray:
+    """Curve position of (x, y) on an n x n grid, n a power of two."""
+    x = np.asarray(x, dtype=np.uint64).copy()
+    y = np.asarray(y, dtype=np.uint64).copy()
+    d = np.zeros(x.shape, dtype=np.uint64)
+    s = np.uint64(n // 2)
+    while s > 0:
+        rx = ((x & s) > 0).astype(np.uint64)
+        ry = ((y & s) > 0).astype(np.uint64)
+        d += s * s * ((np.uint64(3) * rx) ^ ry)
+        low = s - np.uint64(1)
+        flip = (ry == 0) & (rx == 1)
+        x = np.where(flip, low - (x & low), x & low)
+        y = np.where(flip, low - (y & low), y & low)
+        swap = ry == 0
+        x, y = np.where(swap, y, x), np.where(swap, x, y)
+        s = s // np.uint64(2)
+    return d
+
+
+def partition_by_dst(dst, count: int) -> list:
+    """Cut points (count + 1) of edges sorted by (dst, src): partition i grows
+    until it holds (remaining / partitions left) edges and the destination
+    changes.  Partitions after the edges run out are empty."""
+    n = len(dst)
+    cut, pos = [0], 0
+    for i in range(count):
+        if pos < n:
+            expected = (n - pos) // (count - i) + pos
+            while True:
+                pos += 1
+                if pos == n or (pos >= expected and dst[pos] != dst[pos - 1]):
+                    break
+        cut.append(pos)
+    return cut
+
+
+def flat_files(V: int, offsets, adj, directed: bool, parts: int, src: str = "Vertex",
+               typ: str = "edge", dst: str = "Vertex") -> dict:
+    """{file name: (n, 2) uint32 pairs} of the flat edge files."""
+    off = np.asarray(offsets, dtype=np.int64)
+    a = np.asarray(adj, dtype=np.uint32)
+    s = np.repeat(np.arange(V, dtype=np.uint32), np.diff(off))
+    e = np.stack([s, a], axis=1)
+    n = 1
+    while n < V:
+        n *= 2
+    fwd = f"{src}-{typ}->{dst}" if directed else f"{src}-{typ}-{dst}"
+    names = [(fwd, False)] + ([(f"{dst}<-{typ}-{src}", True)] if directed else [])
+
+    def by_src_dst(x):
+        return x[np.lexsort((x[:, 1], x[:, 0]))]
+
+    def by_curve(x):
+        k = hilbert_key(n, x[:, 0], x[:, 1])
+        return x[np.argsort(k, kind="stable")]
+
+    out = {}
+
+    def put(name, bwd, x):
+        out[name] = np.ascontiguousarray(x[:, ::-1] if bwd else x)
+
+    for nm, bwd in names:
+        put(nm + "_src_dst", bwd, by_src_dst(e))
+        put(nm + "_hilbert", bwd, by_curve(e))
+    ds = e[np.lexsort((e[:, 0], e[:, 1]))]
+    cut = partition_by_dst(ds[:, 1].tolist(), parts)
+    for i in range(parts):
+        p = ds[cut[i]:cut[i + 1]]
+        for nm, bwd in names:
+            put(f"{nm}_src_dst_{i}", bwd, by_src_dst(p))
+            put(f"{nm}_hilbert_{i}", bwd, by_curve(p))
+    return out

@@ -1,5 +1,7 @@
 // Seeded fuzz of the host-side generators and samplers (vs_csr_from_edges,
-// vs_sample_sources, vs_sample_paths3), built for the host only with
+// vs_sample_sources, vs_sample_paths3) and of the store writer's flat edge
+// files (vs_store_write with 1..70 partitions, fewer edges than partitions
+// included, into a scratch directory), built for the host only with
 // AddressSanitizer and UndefinedBehaviorSanitizer by tests/test_store_fuzz.py.
 //
 // Edge lists with random sizes, ids (some out of range), symmetrize and
@@ -8,7 +10,7 @@
 // call must return a status; a CSR that comes back must be ascending sets
 // with offsets that match its edge count, and sampled paths must follow it.
 //
-// usage: fuzz_host <iterations>
+// usage: fuzz_host <iterations> [scratch directory]
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -29,8 +31,8 @@ static int fail(const char *what, int i) {
 }
 
 int main(int argc, char **argv) {
-  if (argc != 2) {
-    fprintf(stderr, "usage: %s <iterations>\n", argv[0]);
+  if (argc != 2 && argc != 3) {
+    fprintf(stderr, "usage: %s <iterations> [scratch directory]\n", argv[0]);
     return 2;
   }
   const int iters = atoi(argv[1]);
@@ -79,6 +81,12 @@ int main(int argc, char **argv) {
           if (off[y + 1] > off[y] && !std::binary_search(adj + off[y], adj + off[y + 1], z))
             return fail("path b-c", i);
         }
+      if (argc == 3 && built % 25 == 1) {
+        const uint32_t parts = 1 + static_cast<uint32_t>(rng() % 70);
+        if (vs_store_write(argv[2], "fuzz", "V", "e", static_cast<int32_t>(rng() % 2), off, adj, V,
+                           ce, parts) != VS_OK)
+          return fail("store write", i);
+      }
       vs_csr_free(c);
     }
     const uint32_t S = static_cast<uint32_t>(rng() % (V + 2));

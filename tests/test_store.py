@@ -303,3 +303,110 @@ def test_mutated_stores_fail_cleanly(tmp_path):
                               vs._lib.VS_ERR_UNSUPPORTED), (i, e)
             failed += 1
     assert failed > 100 and ok + failed == 300
+
+
+# ------------------------------------------------------------ flat edge files
+# The reference's curve, restated as its two tables (peano.hh:20-55): the
+# order in which each traversal visits the quadrants q = 2 * xbit + ybit, and
+# the traversal used inside the quadrant visited at each position.
+_ORDER = {"RA": (0, 1, 3, 2), "RD": (3, 2, 0, 1), "CA": (0, 2, 3, 1), "CD": (3, 1, 0, 2)}
+_SUB = {"RA": ("CA", "RA", "RA", "CD"), "RD": ("CD", "RD", "RD", "CA"),
+        "CA": ("RA", "CA", "CA", "RD"), "CD": ("RD", "CD", "CD", "RA")}
+
+
+def _curve_walk(n, x, y):
+    key, d, h = 0, "RA", n // 2
+    while h:
+        q = (2 if x & h else 0) + (1 if y & h else 0)
+        pos = _ORDER[d].index(q)
+        key += pos * h * h
+        d = _SUB[d][pos]
+        h //= 2
+    return key
+
+
+def test_hilbert_key_hand_4x4():
+    # quadrants in RA order A(0,0) B(0,1) D(1,1) C(1,0); A walked CA, B and D
+    # walked RA, C walked CD
+    hand = [[0, 3, 4, 5], [1, 2, 7, 6], [14, 13, 8, 9], [15, 12, 11, 10]]
+    x, y = np.meshgrid(np.arange(4), np.arange(4), indexing="ij")
+    assert ost.hilbert_key(4, x.ravel(), y.ravel()).reshape(4, 4).tolist() == hand
+
+
+def test_hilbert_key_equals_curve_tables():
+    for n in (1, 2, 8, 64):
+        x, y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+        k = ost.hilbert_key(n, x.ravel(), y.ravel())
+        assert k.tolist() == [_curve_walk(n, int(a), int(b))
+                              for a, b in zip(x.ravel(), y.ravel())]
+    rng = np.random.default_rng(5)
+    n = 1 << 30
+    x, y = rng.integers(0, n, 500), rng.integers(0, n, 500)
+    assert ost.hilbert_key(n, x, y).tolist() == [_curve_walk(n, int(a), int(b))
+                                                 for a, b in zip(x, y)]
+
+
+def test_hilbert_full_grid_steps_are_unit():
+    # the reference's Hilbert test property: in curve order, consecutive cells
+    # of a full grid share a row or a column (and here: are neighbours)
+    n = 32
+    x, y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    o = np.argsort(ost.hilbert_key(n, x.ravel(), y.ravel()))
+    px, py = x.ravel()[o], y.ravel()[o]
+    assert np.all(np.abs(np.diff(px)) + np.abs(np.diff(py)) == 1)
+    assert sorted(ost.hilbert_key(n, x.ravel(), y.ravel()).tolist()) == list(range(n * n))
+
+
+def test_partition_by_dst_hand():
+    dst = [0, 0, 0, 1, 2, 2, 2, 2, 3, 5]
+    # 10 edges / 3: first cut at >= 3 and a new dst -> 3; then (7 / 2) + 3 = 6,
+    # dst 2 runs to 8 -> 8; the rest -> 10
+    assert ost.partition_by_dst(dst, 3) == [0, 3, 8, 10]
+    assert ost.partition_by_dst(dst, 1) == [0, 10]
+    # one destination cannot be split: later partitions are empty
+    assert ost.partition_by_dst([4, 4, 4, 4], 3) == [0, 4, 4, 4]
+    assert ost.partition_by_dst([], 2) == [0, 0, 0]
+
+
+def _pairs(path):
+    cnt, size, d = ost.read_diskarray(path, np.uint64)
+    assert size == 8 and os.path.getsize(path) == 64 + ost.capacity(cnt, 8) * 8
+    return np.stack([d & 0xFFFFFFFF, d >> 32], axis=1).astype(np.uint32)
+
+
+@pytest.mark.parametrize("directed,parts", [(False, 4), (True, 3), (True, 1), (False, 64)])
+def test_writer_flat_files_equal_oracle(tmp_path, directed, parts):
+    g = vs.HostCSR.rmat(9, 8, seed=11)
+    off, adj = np.asarray(g.offsets), np.asarray(g.adj)
+    d = tmp_path / "flat"
+    d.mkdir()
+    vs.store_write(d, off, adj, "P", "knows", directed=directed, flat_partitions=parts)
+    exp = ost.flat_files(g.V, off, adj, directed, parts, "P", "knows", "P")
+    assert len(exp) == (2 if directed else 1) * (2 + 2 * parts)
+    for name, pairs in exp.items():
+        got = _pairs(d / name)
+        assert np.array_equal(got, pairs), name
+    si = json.loads((d / "schema.json").read_text())["edge"][0]["store_info"]
+    assert si == {"flats": [1, 0], "partitioned_flats": [1, 0],
+                  "partition_counts": [parts, parts]}
+    # every partition family holds each edge once; no destination is split
+    fwd = "P-knows->P" if directed else "P-knows-P"
+    parts_h = [_pairs(d / f"{fwd}_hilbert_{i}") for i in range(parts)]
+    allp = np.concatenate(parts_h)
+    assert len(allp) == len(adj)
+    assert np.array_equal(allp[np.lexsort((allp[:, 1], allp[:, 0]))], _pairs(d / f"{fwd}_src_dst"))
+    owners = {}
+    for i, p in enumerate(parts_h):
+        for v in np.unique(p[:, 1]).tolist():
+            assert owners.setdefault(v, i) == i
+    # the store still opens as before
+    o2, a2, _ = lib_open(d)
+    assert np.array_equal(o2, off) and np.array_equal(a2, adj)
+
+
+def test_writer_without_flats_lists_none(tmp_path):
+    g = vs.HostCSR.rmat(6, 4, seed=2)
+    vs.store_write(tmp_path, g.offsets, g.adj)
+    si = json.loads((tmp_path / "schema.json").read_text())["edge"][0]["store_info"]
+    assert si == {"flats": [], "partitioned_flats": [], "partition_counts": []}
+    assert not [p for p in os.listdir(tmp_path) if "hilbert" in p or "src_dst" in p and "csr" not in p]

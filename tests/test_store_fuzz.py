@@ -61,9 +61,10 @@ def test_store_reader_fuzz_under_sanitizers(fuzz_exe, tmp_path, which):
 
 def test_host_generators_fuzz_under_sanitizers(tmp_path):
     """tests/cpp/fuzz_host.cpp: random edge lists (some ids out of range) into
-    vs_csr_from_edges, random and damaged CSRs into vs_sample_paths3, and
-    source samples of every size, under ASan + UBSan; built CSRs must be
-    ascending sets and sampled paths must follow them."""
+    vs_csr_from_edges, random and damaged CSRs into vs_sample_paths3, source
+    samples of every size, and built CSRs written as stores with flat edge
+    files, under ASan + UBSan; built CSRs must be ascending sets and sampled
+    paths must follow them."""
     if shutil.which("g++") is None:
         pytest.skip("needs g++")
     exe = tmp_path / "fuzz_host"
@@ -74,7 +75,10 @@ def test_host_generators_fuzz_under_sanitizers(tmp_path):
     b = subprocess.run(cmd, capture_output=True, text=True)
     assert b.returncode == 0, b.stderr[-4000:]
     iters = 1500
-    r = subprocess.run([str(exe), str(iters)], capture_output=True, text=True, timeout=300,
+    scratch = tmp_path / "stores"
+    scratch.mkdir()
+    r = subprocess.run([str(exe), str(iters), str(scratch)], capture_output=True, text=True,
+                       timeout=300,
                        env={"ASAN_OPTIONS": "detect_leaks=1:abort_on_error=0",
                             "UBSAN_OPTIONS": "print_stacktrace=1", "OMP_NUM_THREADS": "2",
                             "PATH": "/usr/bin:/bin"})

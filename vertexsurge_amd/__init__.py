@@ -229,15 +229,18 @@ class GraphStore:
 
 
 def store_write(directory, offsets, adj, vertex_type: str = "Vertex", edge_type: str = "edge",
-                directed: bool = False, graph_name: str = "") -> None:
+                directed: bool = False, graph_name: str = "",
+                flat_partitions: int = 0) -> None:
     """Writes one relation of vertex_type to itself as a store the reference's
-    Graph::load reads (vs_store_write).  directory must exist."""
+    Graph::load reads (vs_store_write).  flat_partitions > 0 also writes the
+    src_dst / hilbert flat edge files, whole and in that many destination
+    partitions.  directory must exist."""
     off = np.ascontiguousarray(offsets, dtype=np.uint64)
     a = _u32(adj)
     V = len(off) - 1
     _lib.call("vs_store_write", str(directory).encode(), graph_name.encode(),
               vertex_type.encode(), edge_type.encode(), int(directed), _ptr(off, C.c_uint64),
-              _ptr(a, C.c_uint32), V, int(off[-1]) if V >= 0 else 0)
+              _ptr(a, C.c_uint32), V, int(off[-1]) if V >= 0 else 0, int(flat_partitions))
 
 
 def stats_dict(st: _lib.VexpandStats) -> dict:

@@ -137,7 +137,7 @@ SIGNATURES = {
     "vs_store_graph": (C.c_int, [vp, C.POINTER(vp)]),
     "vs_store_close": (C.c_int, [vp]),
     "vs_store_write": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int32,
-                                 u64p, u32p, C.c_uint32, C.c_uint64]),
+                                 u64p, u32p, C.c_uint32, C.c_uint64, C.c_uint32]),
     "vs_vexpand": (C.c_int, [vp, u32p, C.c_uint32, C.POINTER(VexpandParams), C.POINTER(vp),
                              C.POINTER(VexpandStats)]),
     "vs_find_shortest": (C.c_int, [vp, u32p, u32p, C.c_uint32, C.c_int32, C.c_uint32, i32p,

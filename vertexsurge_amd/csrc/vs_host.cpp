@@ -78,8 +78,9 @@ static void build_csr(uint32_t V, uint64_t npairs, bool symmetrize,
   out.adj.resize(out.E);
 #pragma omp parallel for schedule(dynamic, 1024)
   for (uint64_t i = 0; i < V; ++i) {
-    std::memcpy(out.adj.data() + out.off[i], tmp.data() + off[i],
-                deg[i] * sizeof(uint32_t));
+    if (deg[i])  // an edgeless CSR has no adj storage (data() may be null)
+      std::memcpy(out.adj.data() + out.off[i], tmp.data() + off[i],
+                  deg[i] * sizeof(uint32_t));
   }
 }
 

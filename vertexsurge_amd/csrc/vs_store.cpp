@@ -426,6 +426,125 @@ vs_status write_array(const std::string &path, const void *data, size_t elem, ui
   return VS_OK;
 }
 
+// ----------------------------------------------------------- flat edges
+#define FLAT_TRY(call)                  \
+  do {                                  \
+    const vs_status st_ = (call);       \
+    if (st_ != VS_OK) return st_;       \
+  } while (0)
+
+// Position of (x, y) on the Peano-Hilbert curve of an n x n grid (n a power
+// of two) that starts in the (0, 0) quadrant and turns clockwise (the
+// reference's XYToIndex(n, x, y, PeanoDirection::RA), peano.hh:20-89).  A
+// curve state is one of four traversals of the quadrants q = 2 * xbit + ybit;
+// kVisit[state][q] is the position at which that traversal visits q, and
+// kNext[state][position] the traversal inside the quadrant visited there.
+// States: 0 clockwise from (0,0), 1 clockwise from (1,1), 2 counter-clockwise
+// from (0,0), 3 counter-clockwise from (1,1).
+uint64_t peano_key(uint64_t n, uint64_t x, uint64_t y) {
+  static constexpr uint8_t kVisit[4][4] = {{0, 1, 3, 2}, {2, 3, 1, 0}, {0, 3, 1, 2},
+                                           {2, 1, 3, 0}};
+  static constexpr uint8_t kNext[4][4] = {{2, 0, 0, 3}, {3, 1, 1, 2}, {0, 2, 2, 1},
+                                          {1, 3, 3, 0}};
+  uint64_t key = 0;
+  int state = 0;
+  for (uint64_t h = n >> 1; h; h >>= 1) {
+    const int q = ((x & h) ? 2 : 0) | ((y & h) ? 1 : 0);
+    const int pos = kVisit[state][q];
+    key += static_cast<uint64_t>(pos) * h * h;
+    state = kNext[state][pos];
+  }
+  return key;
+}
+
+struct FlatEdge {
+  uint32_t src, dst;
+};
+
+// Edge boundaries of `count` destination partitions over edges sorted by
+// (dst, src) (partition_by_dst, graph_loader.hh:111-139): partition i takes
+// edges until it reaches an even share of what is left and the destination
+// changes, so no destination is split.  When fewer edges than partitions
+// remain, the last partitions are empty (the reference reads past the end
+// there).
+std::vector<uint64_t> dst_partitions(const std::vector<FlatEdge> &e, uint32_t count) {
+  std::vector<uint64_t> cut{0};
+  uint64_t pos = 0;
+  const uint64_t n = e.size();
+  for (uint32_t i = 0; i < count; ++i) {
+    if (pos < n) {
+      const uint64_t expected = (n - pos) / (count - i) + pos;
+      do {
+        ++pos;
+      } while (pos < n && !(pos >= expected && e[pos].dst != e[pos - 1].dst));
+    }
+    cut.push_back(pos);
+  }
+  return cut;
+}
+
+enum FlatOrder { kOrderHilbert = 0, kOrderSrcDst = 1 };  // FlatEdgeOrder values
+
+// Writes edges [b, e) as the DiskArray<VIDPair> file `path`; the backward
+// files hold every pair reversed (write_vid_pair_to_disk, graph_loader.hh:86-108).
+vs_status write_pairs(const std::string &path, const FlatEdge *b, uint64_t n, bool backward) {
+  if (!backward) return write_array(path, b, sizeof(FlatEdge), n);
+  std::vector<FlatEdge> r(n);
+  for (uint64_t i = 0; i < n; ++i) r[i] = {b[i].dst, b[i].src};
+  return write_array(path, r.data(), sizeof(FlatEdge), n);
+}
+
+void sort_src_dst(FlatEdge *b, FlatEdge *e) {
+  std::sort(b, e, [](const FlatEdge &x, const FlatEdge &y) {
+    return x.src != y.src ? x.src < y.src : x.dst < y.dst;
+  });
+}
+
+// Sorts by curve position; equal positions are equal pairs.
+void sort_hilbert(FlatEdge *b, FlatEdge *e, uint64_t n) {
+  std::vector<std::pair<uint64_t, FlatEdge>> k(static_cast<size_t>(e - b));
+  for (size_t i = 0; i < k.size(); ++i) k[i] = {peano_key(n, b[i].src, b[i].dst), b[i]};
+  std::sort(k.begin(), k.end(),
+            [](const auto &x, const auto &y) { return x.first < y.first; });
+  for (size_t i = 0; i < k.size(); ++i) b[i] = k[i].second;
+}
+
+// The flat edge files of one relation (gen_and_write_one_edge,
+// graph_loader.hh:186-276, with store_info flats {src_dst, hilbert} and
+// partitioned_flats {src_dst, hilbert} x `parts`, as the reference's importers
+// set them up, graph_loader.cpp:216-223): for each stored direction,
+// <name>_src_dst and <name>_hilbert over all edges, and <name>_src_dst_<i> /
+// <name>_hilbert_<i> over the destination partitions.
+vs_status write_flats(const std::string &d, const EdgeRel &r, const uint64_t *off,
+                      const uint32_t *adj, uint32_t V, uint64_t E, uint32_t parts) {
+  std::vector<FlatEdge> e(E);
+  for (uint32_t u = 0; u < V; ++u)
+    for (uint64_t k = off[u]; k < off[u + 1]; ++k) e[k] = {u, adj[k]};
+  uint64_t n = 1;
+  while (n < V) n <<= 1;  // round_to_2_power(max(src count, dst count))
+  std::vector<std::pair<std::string, bool>> dirs{{r.fwd_name(), false}};
+  if (r.directed) dirs.push_back({r.bwd_name(), true});
+  sort_src_dst(e.data(), e.data() + E);
+  for (const auto &[name, bwd] : dirs) FLAT_TRY(write_pairs(d + "/" + name + "_src_dst", e.data(), E, bwd));
+  sort_hilbert(e.data(), e.data() + E, n);
+  for (const auto &[name, bwd] : dirs) FLAT_TRY(write_pairs(d + "/" + name + "_hilbert", e.data(), E, bwd));
+  std::sort(e.begin(), e.end(), [](const FlatEdge &x, const FlatEdge &y) {
+    return x.dst != y.dst ? x.dst < y.dst : x.src < y.src;
+  });
+  const std::vector<uint64_t> cut = dst_partitions(e, parts);
+  for (uint32_t i = 0; i < parts; ++i) {
+    FlatEdge *b = e.data() + cut[i], *f = e.data() + cut[i + 1];
+    const std::string sfx = "_" + std::to_string(i);
+    sort_src_dst(b, f);
+    for (const auto &[name, bwd] : dirs)
+      FLAT_TRY(write_pairs(d + "/" + name + "_src_dst" + sfx, b, f - b, bwd));
+    sort_hilbert(b, f, n);
+    for (const auto &[name, bwd] : dirs)
+      FLAT_TRY(write_pairs(d + "/" + name + "_hilbert" + sfx, b, f - b, bwd));
+  }
+  return VS_OK;
+}
+
 }  // namespace
 }  // namespace vs
 
@@ -640,7 +759,8 @@ vs_status vs_store_close(vs_store_t s) try {
 
 vs_status vs_store_write(const char *dir, const char *graph_name, const char *vertex_type,
                          const char *edge_type, int32_t directed, const uint64_t *offsets,
-                         const uint32_t *adj, uint32_t V, uint64_t E) try {
+                         const uint32_t *adj, uint32_t V, uint64_t E,
+                         uint32_t flat_partitions) try {
   clear_error();
   if (!dir || !vertex_type || !edge_type || !*vertex_type || !*edge_type || !offsets ||
       (E && !adj)) {
@@ -691,8 +811,24 @@ vs_status vs_store_write(const char *dir, const char *graph_name, const char *ve
     if (s == VS_OK) s = write_array(d + "/" + r.bwd_name() + "_csr_dst_src", radj.data(), 4, E);
   }
   if (s != VS_OK) return s;
+  if (flat_partitions) {
+    try {
+      FLAT_TRY(write_flats(d, r, offsets, adj, V, E, flat_partitions));
+    } catch (const std::bad_alloc &) {
+      set_error("vs_store_write: out of host memory for the flat edge files");
+      return VS_ERR_NOMEM;
+    }
+  }
   // schema.json with the field names of the NLOHMANN_DEFINE_TYPE_INTRUSIVE
-  // lists (schema.hh:32, 121-123, 137-138); no flat edge files are written
+  // lists (schema.hh:32, 121-123, 137-138); FlatEdgeOrder values go out as
+  // their enum integers (Hilbert 0, SrcDst 1; schema.hh:50-55), the member
+  // to_json being no ADL candidate
+  std::string flats = "[]", pcounts = "[]";
+  if (flat_partitions) {
+    flats = "[\n                    1,\n                    0\n                ]";
+    const std::string p = std::to_string(flat_partitions);
+    pcounts = "[\n                    " + p + ",\n                    " + p + "\n                ]";
+  }
   std::string j = "{\n    \"edge\": [\n        {\n            \"allow_multiple_edge\": false,\n"
                   "            \"directed\": ";
   j += r.directed ? "true" : "false";
@@ -701,9 +837,9 @@ vs_status vs_store_write(const char *dir, const char *graph_name, const char *ve
   j += ",\n            \"properties\": {},\n            \"src\": ";
   json_escape(j, r.src);
   j += ",\n            \"statistics\": {\n                \"count\": " + std::to_string(E) +
-       "\n            },\n            \"store_info\": {\n                \"flats\": [],\n"
-       "                \"partition_counts\": [],\n                \"partitioned_flats\": []\n"
-       "            },\n            \"type\": ";
+       "\n            },\n            \"store_info\": {\n                \"flats\": " + flats +
+       ",\n                \"partition_counts\": " + pcounts +
+       ",\n                \"partitioned_flats\": " + flats + "\n            },\n            \"type\": ";
   json_escape(j, r.type);
   j += "\n        }\n    ],\n    \"graph_name\": ";
   json_escape(j, graph_name ? graph_name : "");
