@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <map>
 #include <memory>
 #include <vector>
@@ -318,7 +319,13 @@ class CallerOrder {
 vs_status flat_pairs_device(vs_result_s *r, uint32_t which, bool want, uint32_t **d_start,
                             uint32_t **d_target, size_t *bytes, uint64_t *n);
 // Multi-way intersection over edge relations with every array in device
-// memory (vs_mintersect_device, vs_mintersect.hip).
+// memory (vs_mintersect_device, vs_mintersect.hip): lists into d_lists
+// (capacity list_cap), or, in mintersect_run, into the buffer lists_for
+// returns for the total once it is known (nullptr: counts only).
+using ListsFor = std::function<vs_status(uint64_t, uint32_t **)>;
+vs_status mintersect_run(vs_graph_s *const *deps, uint32_t arity, const uint32_t *d_tuples,
+                         uint64_t T, uint32_t *d_counts, uint64_t *d_offsets,
+                         const ListsFor *lists_for, uint64_t *total, vs_mintersect_stats *st);
 vs_status mintersect_impl(vs_graph_s *const *deps, uint32_t arity, const uint32_t *d_tuples,
                           uint64_t T, uint32_t *d_counts, uint64_t *d_offsets,
                           uint32_t *d_lists, uint64_t list_cap, uint64_t *total,

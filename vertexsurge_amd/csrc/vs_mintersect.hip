@@ -549,11 +549,9 @@ static uint64_t stage_limit_bytes() {
   return static_cast<uint64_t>(mb) << 20;
 }
 
-vs_status mintersect_impl(const vs_graph_t *deps, uint32_t arity,
-                                 const uint32_t *d_tuples, uint64_t T,
-                                 uint32_t *d_counts, uint64_t *d_offsets,
-                                 uint32_t *d_lists, uint64_t list_cap,
-                                 uint64_t *total, vs_mintersect_stats *st) {
+vs_status mintersect_run(const vs_graph_t *deps, uint32_t arity, const uint32_t *d_tuples,
+                         uint64_t T, uint32_t *d_counts, uint64_t *d_offsets,
+                         const ListsFor *lists_for, uint64_t *total, vs_mintersect_stats *st) {
   if (!deps || arity == 0 || arity > kMaxArity) {
     set_error("vs_mintersect: arity %u not in 1..%d", arity, kMaxArity);
     return VS_ERR_INVALID;
@@ -620,7 +618,7 @@ vs_status mintersect_impl(const vs_graph_t *deps, uint32_t arity,
   for (int k = 0; k < kBuckets; ++k) a.order[k] = static_cast<const uint32_t *>(lp) + k * T;
   a.norder = static_cast<const ull *>(np);
   const uint64_t stage_max = stage_limit_bytes();
-  const bool try_stage = d_offsets && d_lists && stage_max > 0;
+  const bool try_stage = d_offsets && lists_for && stage_max > 0;
   VS_HIP(hipEventRecord(e0, s));
   // with staging in view the split also writes each tuple's bound into counts
   const uint64_t split_chunks = (T + kBlock * kSplitRounds - 1) / (kBlock * kSplitRounds);
@@ -652,15 +650,14 @@ vs_status mintersect_impl(const vs_graph_t *deps, uint32_t arity,
       VS_HIP(hipGetLastError());
       VS_TRY(scan_counts(g, s, d_counts, d_offsets, T, &tot));
       if (total) *total = tot;
-      if (tot > list_cap) {
-        set_error("vs_mintersect: list capacity %llu < total %llu",
-                  (unsigned long long)list_cap, (unsigned long long)tot);
-        return VS_ERR_CAPACITY;
+      uint32_t *d_lists = nullptr;
+      VS_TRY((*lists_for)(tot, &d_lists));
+      if (tot && d_lists) {  // a provider may decline the lists (nullptr)
+        hipLaunchKernelGGL(k_mi_compact, dim3(grid_for((T + 63) / 64, g->num_cus)),
+                           dim3(kBlock), 0, s, d_counts, soff, d_offsets,
+                           static_cast<const uint32_t *>(stg), d_lists, T);
+        VS_HIP(hipGetLastError());
       }
-      hipLaunchKernelGGL(k_mi_compact, dim3(grid_for((T + 63) / 64, g->num_cus)), dim3(kBlock),
-                         0, s, d_counts, soff, d_offsets, static_cast<const uint32_t *>(stg),
-                         d_lists, T);
-      VS_HIP(hipGetLastError());
       staged = true;
     } else {
       clear_error();  // no room to stage: two passes
@@ -672,16 +669,15 @@ vs_status mintersect_impl(const vs_graph_t *deps, uint32_t arity,
     if (d_offsets) {
       VS_TRY(scan_counts(g, s, d_counts, d_offsets, T, &tot));
       if (total) *total = tot;
-      if (d_lists) {
-        if (tot > list_cap) {
-          set_error("vs_mintersect: list capacity %llu < total %llu",
-                    (unsigned long long)list_cap, (unsigned long long)tot);
-          return VS_ERR_CAPACITY;
+      if (lists_for) {
+        uint32_t *d_lists = nullptr;
+        VS_TRY((*lists_for)(tot, &d_lists));
+        if (tot && d_lists) {
+          a.offsets = d_offsets;
+          a.lists = d_lists;
+          launch_mintersect(kMiFill, arity, grid, s, a);
+          VS_HIP(hipGetLastError());
         }
-        a.offsets = d_offsets;
-        a.lists = d_lists;
-        launch_mintersect(kMiFill, arity, grid, s, a);
-        VS_HIP(hipGetLastError());
       }
     }
   }
@@ -701,7 +697,7 @@ vs_status mintersect_impl(const vs_graph_t *deps, uint32_t arity,
     st->total = tot;
     // compulsory bytes: distinct lists touched + tuples + counts + offsets + lists
     uint64_t b = 4ull * arity * T + 4 * T + (d_offsets ? 8 * (T + 1) : 0) +
-                 (d_lists ? 4 * tot : 0);
+                 (lists_for ? 4 * tot : 0);
     for (uint32_t d = 0; d < arity; ++d) {
       bool seen = false;
       for (uint32_t q = 0; q < d; ++q) seen |= deps[q] == deps[d];
@@ -973,6 +969,23 @@ static vs_status mintersect_noalloc(const vs_graph_t *deps, uint32_t arity,
   return VS_OK;
 }
 
+vs_status mintersect_impl(const vs_graph_t *deps, uint32_t arity, const uint32_t *d_tuples,
+                          uint64_t T, uint32_t *d_counts, uint64_t *d_offsets,
+                          uint32_t *d_lists, uint64_t list_cap, uint64_t *total,
+                          vs_mintersect_stats *st) {
+  const ListsFor into = [&](uint64_t tot, uint32_t **out) -> vs_status {
+    *out = d_lists;
+    if (tot > list_cap) {
+      set_error("vs_mintersect: list capacity %llu < total %llu",
+                static_cast<unsigned long long>(list_cap), static_cast<unsigned long long>(tot));
+      return VS_ERR_CAPACITY;
+    }
+    return VS_OK;
+  };
+  return mintersect_run(deps, arity, d_tuples, T, d_counts, d_offsets, d_lists ? &into : nullptr,
+                        total, st);
+}
+
 } // namespace vs
 
 using namespace vs;
@@ -1043,26 +1056,30 @@ vs_status vs_mintersect(const vs_graph_t *deps, uint32_t arity,
   if (b.ob) VS_TRY(g->pool.get(b.ob, false, g->stream, &b.o));
   VS_HIP(hipMemcpyAsync(b.t, tuples, b.tb, hipMemcpyHostToDevice, g->stream));
   uint64_t tot = 0;
-  // pass 1: counts + offsets (lists == NULL)
-  VS_TRY(mintersect_impl(deps, arity, static_cast<uint32_t *>(b.t), T,
-                         static_cast<uint32_t *>(b.c), static_cast<uint64_t *>(b.o),
-                         nullptr, 0, &tot, lists ? nullptr : stats));
+  // one intersection pass; the device list buffer is taken once the total
+  // is known (counts and offsets are returned on a capacity error too)
+  const ListsFor into = [&](uint64_t n, uint32_t **out) -> vs_status {
+    *out = nullptr;
+    if (n > list_cap) {
+      set_error("vs_mintersect: list capacity %llu < total %llu",
+                static_cast<unsigned long long>(list_cap), static_cast<unsigned long long>(n));
+      return VS_ERR_CAPACITY;
+    }
+    b.lb = std::max<uint64_t>(n, 1) * 4;
+    VS_TRY(g->pool.get(b.lb, false, g->stream, &b.l));
+    *out = static_cast<uint32_t *>(b.l);
+    return VS_OK;
+  };
+  const vs_status rs = mintersect_run(deps, arity, static_cast<uint32_t *>(b.t), T,
+                                      static_cast<uint32_t *>(b.c),
+                                      static_cast<uint64_t *>(b.o), lists ? &into : nullptr,
+                                      &tot, stats);
+  if (rs != VS_OK && rs != VS_ERR_CAPACITY) return rs;
   if (total) *total = tot;
   VS_HIP(hipMemcpy(counts, b.c, b.cb, hipMemcpyDeviceToHost));
   if (offsets) VS_HIP(hipMemcpy(offsets, b.o, b.ob, hipMemcpyDeviceToHost));
-  if (lists) {
-    if (tot > list_cap) {
-      set_error("vs_mintersect: list capacity %llu < total %llu",
-                (unsigned long long)list_cap, (unsigned long long)tot);
-      return VS_ERR_CAPACITY;
-    }
-    b.lb = std::max<uint64_t>(tot, 1) * 4;
-    VS_TRY(g->pool.get(b.lb, false, g->stream, &b.l));
-    VS_TRY(mintersect_impl(deps, arity, static_cast<uint32_t *>(b.t), T,
-                           static_cast<uint32_t *>(b.c), static_cast<uint64_t *>(b.o),
-                           static_cast<uint32_t *>(b.l), tot, &tot, stats));
-    if (tot) VS_HIP(hipMemcpy(lists, b.l, tot * 4, hipMemcpyDeviceToHost));
-  }
+  if (rs != VS_OK) return rs;
+  if (lists && tot) VS_HIP(hipMemcpy(lists, b.l, tot * 4, hipMemcpyDeviceToHost));
   return VS_OK;
 } VS_CATCH("vs_mintersect")
 

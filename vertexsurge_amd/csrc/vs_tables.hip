@@ -805,20 +805,19 @@ static vs_status mintersect_varrow(vs_result_t base, uint32_t i, const vs_colmat
     VS_TRY(mc_device(mat_deps, nd, tup.as<uint32_t>(), P, cnt.as<uint32_t>(), off.as<uint64_t>(),
                      lists_for, &tot, &nfound, &ms));
   } else {
-    std::vector<vs_graph_s *> gd(graph_deps, graph_deps + nd);
-    vs_mintersect_stats ms1{}, ms2{};
-    VS_TRY(mintersect_impl(gd.data(), nd, tup.as<uint32_t>(), P, cnt.as<uint32_t>(),
-                           off.as<uint64_t>(), nullptr, 0, &tot, &ms1));
+    // one intersection pass; the list buffer is taken once the total is known
+    const ListsFor lists_for = [&](uint64_t total, uint32_t **out) -> vs_status {
+      *out = nullptr;
+      if (total == 0) return VS_OK;
+      if (total > cap) { over = true; return VS_OK; }
+      VS_TRY(lst.get(total * 4, false));
+      *out = d_lists = lst.as<uint32_t>();
+      return VS_OK;
+    };
+    vs_mintersect_stats ms1{};
+    VS_TRY(mintersect_run(graph_deps, nd, tup.as<uint32_t>(), P, cnt.as<uint32_t>(),
+                          off.as<uint64_t>(), want ? &lists_for : nullptr, &tot, &ms1));
     ms = static_cast<float>(ms1.ms);
-    if (want && tot && tot <= cap) {
-      VS_TRY(lst.get(tot * 4, false));
-      d_lists = lst.as<uint32_t>();
-      VS_TRY(mintersect_impl(gd.data(), nd, tup.as<uint32_t>(), P, cnt.as<uint32_t>(),
-                             off.as<uint64_t>(), d_lists, tot, &tot, &ms2));
-      ms += static_cast<float>(ms2.ms);
-    } else if (want && tot > cap) {
-      over = true;
-    }
   }
   *n = tot;
   if (st) {
