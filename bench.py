@@ -271,7 +271,8 @@ def main() -> int:
     variant_json = {k: {"launches": v[0], "avg_ms": round(v[1] / v[0], 4) if v[0] else 0.0,
                         "alg_bytes": int(v[2])} for k, v in variants.items()}
     # HBM bytes per launch measured by the rocprofv3 counter passes on this
-    # workload (committed with the profile summary); null for other workloads
+    # workload under the tuning this run used (committed with the profile
+    # summary); null for other workloads or tunings
     traffic = None
     tfile = ROOT / "profiles" / "traffic_k_bw_expand_rmat24_s4096.json"
     workload = f"rmat{args.scale}_ef{args.edge_factor}_s{args.sources}_" \
@@ -279,7 +280,9 @@ def main() -> int:
     if tfile.exists():
         tj = json.loads(tfile.read_text())
         if tj.get("workload") == workload:
-            traffic = int(tj["traffic_bytes_per_launch"])
+            for run in tj.get("runs", []):
+                if run["tuning"] == dict(tuning):
+                    traffic = int(run["traffic_bytes_per_launch"])
     p_dom = "pull" if pst["ms_pull"] >= pst["ms_push"] else "push"
     p_n = max(1, pst[f"n_{p_dom}"])
     p_ach = (pst[f"bytes_{p_dom}"] / p_n) / (pst[f"ms_{p_dom}"] / p_n / 1e3) / 1e9 \

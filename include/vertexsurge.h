@@ -170,7 +170,7 @@ typedef struct {
   uint32_t push_alpha;  /* push while frontier-edges * alpha < E */
   uint32_t sparse_pct;  /* pull reads the frontier's word masks (sparse-row
                            gathers) when its nonzero words, weighted by degree,
-                           are below this percentage of all; 0 = 50 */
+                           are below this percentage of all; 0 = 35 */
   uint32_t pull_hubs;   /* vertices whose frontier rows pull gathers first, with
                            cached loads (the most-listed ones); 0 = 131072 */
 } vs_tuning;

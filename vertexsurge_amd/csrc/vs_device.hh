@@ -95,8 +95,11 @@ struct DevCSR {
 // the 256 MB Infinity Cache can hold next to the streaming gathers
 constexpr uint32_t kPullHubs = 131072;
 // default word-mask threshold: pull gathers sparse rows through their word
-// masks when the degree-weighted share of nonzero words is below this (%)
-constexpr uint32_t kSparsePct = 50;
+// masks when the degree-weighted share of nonzero words is below this (%).
+// On R-MAT 24 / 4096 sources the hop-3 frontier sits between 35 and 50: its
+// dense pull took 41.4 ms and fetched 195 GB, the word-mask pull 47.1 ms and
+// 244 GB; hop 2 (far below 35) stays on the word masks.
+constexpr uint32_t kSparsePct = 35;
 
 // Builds the pull items of a CSR once (vertex | chunk << 32 per kPullChunk
 // in-edges, vertices without edges omitted).
