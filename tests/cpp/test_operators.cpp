@@ -174,6 +174,27 @@ int main(int argc, char **argv) {
     print_vec("mintersect_matmat_flat", flat);
   }
 
+  // triangles through each edge: VExpand(1..1) as the VarRow base, both edge
+  // endpoints bound to the same relation (mintersect.cpp:244-333)
+  {
+    VExpand v1("Person", "Person", "knows", Direction::Forward, 1, 1,
+               VExpandPathParameters::Any(), ExpandStrategy::Dense, "a.id", std::nullopt);
+    const std::vector<VID> few(starts.begin(), starts.begin() + 40);
+    v1.run(few);
+    auto edges = v1.results();
+    Mintersect tv("c", "Person", false);
+    tv.add_dependency(rel);
+    tv.add_dependency(rel);
+    tv.run(edges, {0, 1});
+    std::vector<uint64_t> flat(tv.flat_tuples().begin(), tv.flat_tuples().end());
+    print_vec("varrow_triangles_flat", flat);
+    Mintersect tc("c", "Person", true);
+    tc.add_dependency(rel);
+    tc.add_dependency(rel);
+    tc.run(edges, {0, 1});
+    std::printf("\"varrow_triangles_count\": %llu,\n", (unsigned long long)tc.count());
+  }
+
   // a -[1..2]-> c <-[1..2]- b: VExpand, TableConvert (transposed), Mintersect
   {
     VExpand vp("Person", "Person", "knows", Direction::Forward, 1, 2,

@@ -52,6 +52,15 @@ def test_cpp_operators_match_oracle(orc):
     assert r["mintersect_matmat_flat"] == [1, 1, 1, 2, 1, 1, 3, 1, 1]
     # the VExpand -> TableConvert -> Mintersect pattern against the oracle
     from oracle import tables as ot
+    # triangles through the edges of 40 sources: VarRow-base Mintersect
+    few = starts[:40]
+    m1, _, _, _ = orc.vexpand_rows(g.offsets, g.adj, few, 1, 1, "any")
+    M1 = np.unpackbits(m1[0].view(np.uint8), axis=1, bitorder="little")[:, :g.V].astype(bool)
+    es, et, ew = ot.mintersect_varrow(M1, few, None, [0, 1],
+                                      adjacency=[(g.offsets, g.adj)] * 2)
+    assert len(ew) > 0
+    assert r["varrow_triangles_count"] == len(ew)
+    assert r["varrow_triangles_flat"] == np.stack([es, et, ew], axis=1).ravel().tolist()
     m, _, _, _ = orc.vexpand_rows(g.offsets, g.adj, starts, 1, 2, "any")
     M = np.unpackbits(m[0].view(np.uint8), axis=1, bitorder="little")[:, :g.V].astype(bool)
     E, rm, cm = ot.table_convert(M, starts, None, True)

@@ -549,6 +549,45 @@ class Mintersect {
         tuples_.push_back(lists[k]);
       }
   }
+  // VarRowMatrixTable base (mintersect.cpp:244-333): every set bit of matrix
+  // `which` is a base tuple (start = the row's vertex, target = the column's
+  // vertex); dependency d intersects the column bound to base variable
+  // var_idx[d] (0 = start, 1 = target; the reference's var_base_idx).  Output
+  // tuples (start, target, new vertex), by base tuple then new vertex.
+  void run(const VarRowMatrixTable &base, const std::vector<uint32_t> &var_idx,
+           uint32_t which = 0) {
+    if (!deps_.empty() && !mat_deps_.empty())
+      throw std::runtime_error("Mintersect: mixed edge-relation and ColMain dependencies");
+    const bool mats = !mat_deps_.empty();
+    const uint32_t nd = static_cast<uint32_t>(mats ? mat_deps_.size() : deps_.size());
+    if (nd == 0 || var_idx.size() != nd)
+      throw std::runtime_error("Mintersect: one base variable per dependency");
+    std::vector<vs_graph_t> hs;
+    std::vector<vs_colmat_t> ms;
+    for (auto &d : deps_) hs.push_back(d->handle());
+    for (auto &d : mat_deps_) ms.push_back(d->handle());
+    const vs_colmat_t *mp = mats ? ms.data() : nullptr;
+    const vs_graph_t *gp = mats ? nullptr : hs.data();
+    uint64_t total = 0;
+    check(vs_mintersect_varrow(base.handle(), which, mp, gp, nd, var_idx.data(), nullptr,
+                               nullptr, nullptr, 0, &total, nullptr),
+          "vs_mintersect_varrow");
+    total_num_ = total;
+    tuples_.clear();
+    counts_.clear();
+    offsets_.clear();
+    if (only_count_ || total == 0) return;
+    std::vector<uint32_t> st(total), tg(total), nv(total);
+    check(vs_mintersect_varrow(base.handle(), which, mp, gp, nd, var_idx.data(), st.data(),
+                               tg.data(), nv.data(), total, &total, nullptr),
+          "vs_mintersect_varrow");
+    tuples_.reserve(3 * total);
+    for (uint64_t k = 0; k < total; ++k) {
+      tuples_.push_back(st[k]);
+      tuples_.push_back(tg[k]);
+      tuples_.push_back(nv[k]);
+    }
+  }
   // only_count result: the "count" Int64 column (mintersect.cpp:336-340)
   uint64_t count() const { return total_num_; }
   const std::vector<VID> &flat_tuples() const { return tuples_; }
