@@ -1660,6 +1660,12 @@ static vs_status autotune_batch(vs_graph_t g, const vs_query *queries, uint32_t 
     return s;
   }
   g->tuning = cur;
+  // the last candidates rebuilt the hubs-first pull lists for their hub
+  // counts: rebuild them for the kept one now, so the caller's next call
+  // does not pay for it
+  const DevCSR *push = nullptr, *pull = nullptr;
+  VS_TRY(graph_csrs(g, p.direction, &push, &pull));
+  VS_TRY(ensure_pull_items(*const_cast<DevCSR *>(pull), V, cur.pull_hubs));
   if (best) *best = cur;
   if (best_ms) *best_ms = bms;
   return VS_OK;
