@@ -69,7 +69,7 @@ def parse():
     ap.add_argument("--tile-words", type=int, default=0, help="plain leg tile width")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", default="",
-                    help="tile_words,push_alpha,sparse_pct,pull_hubs for the graph (no autotune)")
+                    help="tile_words,push_alpha,sparse_pct,pull_hubs[,wpush_pml] for the graph (no autotune)")
     ap.add_argument("--weights", default=str(ROOT / "vertexsurge_amd" / "sched_weights.npz"))
     ap.add_argument("--cpu-sources", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
@@ -201,8 +201,10 @@ def main() -> int:
 
     queries = [vs.BatchQuery(src, args.k_min, args.k_max, args.path)]
     if args.tuning:
-        tw, alpha, sparse, hubs = (int(x) for x in args.tuning.split(","))
-        G.set_tuning(tile_words=tw, push_alpha=alpha, sparse_pct=sparse, pull_hubs=hubs)
+        vals = [int(x) for x in args.tuning.split(",")]
+        tw, alpha, sparse, hubs = vals[:4]
+        G.set_tuning(tile_words=tw, push_alpha=alpha, sparse_pct=sparse, pull_hubs=hubs,
+                     wpush_pml=vals[4] if len(vals) > 4 else 0)
     tuning = G.get_tuning()
     tune_ms = None
     if not args.no_autotune and not args.tuning:

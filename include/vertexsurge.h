@@ -173,6 +173,12 @@ typedef struct {
                            are below this percentage of all; 0 = 35 */
   uint32_t pull_hubs;   /* vertices whose frontier rows pull gathers first, with
                            cached loads (the most-listed ones); 0 = 131072 */
+  uint32_t wpush_pml;   /* word push on pull hops that gather whole rows: the
+                           frontier words cheapest to scatter (fewest out-edges of
+                           the rows holding bits there), taken cheapest first
+                           while their out-edges sum to at most this many per
+                           mille of E, are scattered instead of gathered;
+                           0 = 40; above 1000 = off */
 } vs_tuning;
 vs_status vs_graph_set_tuning(vs_graph_t g, const vs_tuning *t);
 vs_status vs_graph_get_tuning(vs_graph_t g, vs_tuning *t);
@@ -287,6 +293,8 @@ typedef struct {
   /* 1 when the learned scheduler picked tile_words */
   uint32_t tile_words_learned;
   uint32_t reserved2;
+  /* words scattered by the word push at hop i (index i-1), summed over tiles */
+  uint32_t wpush_words[VS_MAX_HOPS];
 } vs_vexpand_stats;
 
 typedef struct vs_result_s *vs_result_t;
@@ -386,6 +394,8 @@ typedef struct {
   double ms_expand_sparse;
   uint64_t n_expand_sparse;
   uint64_t bytes_sparse;
+  uint64_t wpush_words; /* words scattered by the word push, summed over hops
+                           and tiles */
 } vs_batch_stats;
 
 /* Runs nq independent VExpand queries packed into shared bit-matrix tiles.

@@ -233,3 +233,33 @@ def test_autotune_batch_exact_and_no_slower(g14):
         assert np.array_equal(c0, c1) and np.array_equal(d0, d1)
     finally:
         G.set_tuning(**saved)
+
+
+@pytest.mark.parametrize("pml", [1000, 200])
+def test_batch_word_push(orc, g14, pml):
+    """Batched engine with the word push (decided on the device per slot and
+    hop): the result matrices equal the plain engine's without it, the counts
+    and digests the oracle's; hops that are a tile's last never push."""
+    g, G = g14
+    saved = G.get_tuning()
+    qs = make_queries(g.V, 10, seed=17, smax=3000, kmax=(2, 5), kmins=(0, 1, 2))
+    try:
+        G.set_tuning(**{**saved, "wpush_pml": 2000})
+        plain = [vs.vexpand(G, q.sources, q.k_min, q.k_max, q.path, sched="pull")[0]
+                 for q in qs]
+        G.set_tuning(**{**saved, "wpush_pml": pml})
+        for use_graph in (True, False):
+            rs, st = vs.vexpand_batch_results(G, qs, sched="pull", use_graph=use_graph,
+                                              tile_words=16)
+            if pml == 1000:
+                assert st["wpush_words"] > 0
+            for a, b in zip(rs, plain):
+                _same_result(a, b, g.V)
+                a.close()
+        c, d, _ = vs.vexpand_batch(G, qs, sched="pull", tile_words=16)
+        ec, ed = expected(orc, g, qs)
+        assert np.array_equal(c, ec) and np.array_equal(d, ed)
+    finally:
+        G.set_tuning(**saved)
+        for b in plain:
+            b.close()

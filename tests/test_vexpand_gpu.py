@@ -191,3 +191,50 @@ def test_repeat_calls_reuse_pool(orc, rmat16):
         res.close()
     for o in outs[1:]:
         assert np.array_equal(o, outs[0])
+
+
+@pytest.mark.parametrize("pml", [1000, 200, 2000])
+@pytest.mark.parametrize("path", ["shortest", "any"])
+@pytest.mark.parametrize("tw", [16, 64])
+def test_word_push(orc, rmat16, pml, path, tw):
+    """Word push on the pull hops that gather whole rows (the cheapest words
+    scattered into N, the rest pulled, a word-push epilogue after the
+    chunked-vertex one): bit-exact against the oracle with budgets that push
+    most words (1000 per mille of E), some (200) and none (2000 = off), over
+    R-MAT 16 with chunked in-lists, a ragged last word, k_min 0..2."""
+    g, G = rmat16
+    saved = G.get_tuning()
+    try:
+        G.set_tuning(**{**saved, "wpush_pml": pml})
+        src = vs.sample_sources(g.V, 1000, seed=11)
+        pushed = 0
+        for k_min, k_max in [(1, 4), (2, 4), (0, 3)]:
+            res, st, _ = check(orc, G, g.offsets, g.adj, src, k_min, k_max, path,
+                               sched="pull", tile_words=tw)
+            pushed += sum(st["wpush_words"])
+            res.close()
+        if pml > 1000:
+            assert pushed == 0
+        if pml == 1000:
+            assert pushed > 0
+    finally:
+        G.set_tuning(**saved)
+
+
+def test_word_push_layers_and_find_shortest(orc, rmat16):
+    """with_distance layers and FindShortest distances with the word push on."""
+    g, G = rmat16
+    saved = G.get_tuning()
+    try:
+        G.set_tuning(**{**saved, "wpush_pml": 1000})
+        src = vs.sample_sources(g.V, 300, seed=78)
+        for path in ("shortest", "any"):
+            check(orc, G, g.offsets, g.adj, src, 1, 4, path, with_distance=True, sched="pull")
+        rng = np.random.default_rng(3)
+        tgt = rng.integers(0, g.V, size=300).astype(np.uint32)
+        dist = vs.find_shortest(G, src, tgt, k_max=50)
+        for r in range(0, 300, 7):
+            d = orc.bfs_dist(g.offsets, g.adj, int(src[r]))
+            assert dist[r] == d[tgt[r]], r
+    finally:
+        G.set_tuning(**saved)

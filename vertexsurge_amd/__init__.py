@@ -152,15 +152,16 @@ class Graph:
     def handle(self):
         return self._h
 
-    def set_tuning(self, tile_words=0, push_alpha=16, pull_hubs=0, sparse_pct=0):
-        t = _lib.Tuning(tile_words, push_alpha, sparse_pct, pull_hubs)
+    def set_tuning(self, tile_words=0, push_alpha=16, pull_hubs=0, sparse_pct=0, wpush_pml=0):
+        t = _lib.Tuning(tile_words, push_alpha, sparse_pct, pull_hubs, wpush_pml)
         _lib.call("vs_graph_set_tuning", self._h, C.byref(t))
 
     def get_tuning(self) -> dict:
         t = _lib.Tuning()
         _lib.call("vs_graph_get_tuning", self._h, C.byref(t))
         return {"tile_words": t.tile_words, "push_alpha": t.push_alpha,
-                "pull_hubs": t.pull_hubs, "sparse_pct": t.sparse_pct}
+                "pull_hubs": t.pull_hubs, "sparse_pct": t.sparse_pct,
+                "wpush_pml": t.wpush_pml}
 
     def set_scheduler(self, sched: "Scheduler | None"):
         # the library keeps its own reference to an attached scheduler
@@ -258,6 +259,7 @@ def stats_dict(st: _lib.VexpandStats) -> dict:
         "vis_frac": list(st.vis_frac[:k]),
         "src_degsum": st.src_degsum, "src_vol2": st.src_vol2,
         "tile_words_learned": bool(st.tile_words_learned),
+        "wpush_words": list(st.wpush_words[:k]),
     }
 
 
@@ -709,7 +711,8 @@ def _batch_stats(st) -> dict:
             "push_decisions": st.push_decisions, "pull_decisions": st.pull_decisions,
             "ms_expand": st.ms_expand, "n_expand": st.n_expand, "bytes_alg": st.bytes_alg,
             "ms_wall": st.ms_wall, "ms_expand_sparse": st.ms_expand_sparse,
-            "n_expand_sparse": st.n_expand_sparse, "bytes_sparse": st.bytes_sparse}
+            "n_expand_sparse": st.n_expand_sparse, "bytes_sparse": st.bytes_sparse,
+            "wpush_words": st.wpush_words}
 
 
 def autotune_batch(graph: Graph, queries: list, tile_words: int = 0, sched: str = "heuristic",
@@ -722,7 +725,8 @@ def autotune_batch(graph: Graph, queries: list, tile_words: int = 0, sched: str 
     _lib.call("vs_autotune_batch", graph.handle, arr, len(queries), C.byref(p), C.byref(t),
               C.byref(best), C.byref(start))
     return ({"tile_words": t.tile_words, "push_alpha": t.push_alpha,
-             "sparse_pct": t.sparse_pct, "pull_hubs": t.pull_hubs}, best.value, start.value)
+             "sparse_pct": t.sparse_pct, "pull_hubs": t.pull_hubs,
+             "wpush_pml": t.wpush_pml}, best.value, start.value)
 
 
 def vexpand_batch_results(graph: Graph, queries: list, tile_words: int = 0,
@@ -773,4 +777,5 @@ def vexpand_batch(graph: Graph, queries: list, tile_words: int = 0, sched: str =
         "push_decisions": st.push_decisions, "pull_decisions": st.pull_decisions,
         "ms_expand": st.ms_expand, "n_expand": st.n_expand, "bytes_alg": st.bytes_alg,
         "ms_wall": st.ms_wall, "ms_expand_sparse": st.ms_expand_sparse,
-        "n_expand_sparse": st.n_expand_sparse, "bytes_sparse": st.bytes_sparse}
+        "n_expand_sparse": st.n_expand_sparse, "bytes_sparse": st.bytes_sparse,
+            "wpush_words": st.wpush_words}

@@ -62,12 +62,14 @@ class VexpandStats(C.Structure):
                 ("frontier_act", C.c_uint64 * MAX_HOPS), ("frontier_deg", C.c_uint64 * MAX_HOPS),
                 ("vis_frac", C.c_double * MAX_HOPS),
                 ("src_degsum", C.c_uint64), ("src_vol2", C.c_uint64),
-                ("tile_words_learned", C.c_uint32), ("reserved2", C.c_uint32)]
+                ("tile_words_learned", C.c_uint32), ("reserved2", C.c_uint32),
+                ("wpush_words", C.c_uint32 * MAX_HOPS)]
 
 
 class Tuning(C.Structure):
     _fields_ = [("tile_words", C.c_uint32), ("push_alpha", C.c_uint32),
-                ("sparse_pct", C.c_uint32), ("pull_hubs", C.c_uint32)]
+                ("sparse_pct", C.c_uint32), ("pull_hubs", C.c_uint32),
+                ("wpush_pml", C.c_uint32)]
 
 
 class Query(C.Structure):
@@ -88,7 +90,7 @@ class BatchStats(C.Structure):
                 ("pull_decisions", C.c_uint64), ("ms_expand", C.c_double),
                 ("n_expand", C.c_uint64), ("bytes_alg", C.c_uint64), ("ms_wall", C.c_double),
                 ("ms_expand_sparse", C.c_double), ("n_expand_sparse", C.c_uint64),
-                ("bytes_sparse", C.c_uint64)]
+                ("bytes_sparse", C.c_uint64), ("wpush_words", C.c_uint64)]
 
 
 class StoreInfo(C.Structure):

@@ -55,6 +55,9 @@ struct SlotDesc {
   uint64_t *front;   // [H + 1][64]: per-word OR of the rows of L_h
   char *part;        // [H + 1][kParts] partial HopCounters, kPartStride apart
   uint64_t *fpart;   // [H + 1][kParts][64] partial front words
+  ull *wdeg;         // [H + 1][64]: per-word push degree of L_h
+  ull *wpart;        // [H + 1][kWParts][64] partial per-word push degrees
+  uint64_t pw;       // this hop: the words the word push scatters (pull slots)
   uint32_t tile;     // global tile of the current wave, UINT32_MAX if idle
   int32_t mode;      // this hop: 0 skip, 1 push, 2 pull
   uint32_t dense;    // this hop: pull without the active-bitmap lookup
@@ -73,11 +76,13 @@ struct BatchCtl {
   uint32_t pad;
   ull unit_prefix[kMaxSlots + 1];    // units of push slots and dense pull slots
   ull unit_prefix_s[kMaxSlots + 1];  // units of sparse pull slots (word masks)
+  ull unit_prefix_w[kMaxSlots + 1];  // word-push items of pull slots
   ull te;
   ull push_dec;
   ull pull_dec;
   ull bytes;
   ull bytes_sparse;  // of the hops run by the sparse (word-mask) expansion
+  ull wpush_words;   // words scattered by the word push
 };
 
 struct BatchConst {
@@ -93,6 +98,7 @@ struct BatchConst {
   uint32_t sparse_pct;  // word-mask threshold (percent of words, by degree)
   uint32_t use_r;
   uint64_t E;
+  uint64_t wpush_budget;  // word push: push-degree budget per tile and hop (0: off)
   const uint64_t *poff;
   const uint32_t *padj;
   const uint64_t *xoff;
@@ -171,6 +177,9 @@ __device__ __forceinline__ HopArgs slot_args(const BatchConst &c, const SlotDesc
   a.newords = 0;
   a.front = sd.front + static_cast<size_t>(hop - 1) * 64;
   a.front_next = sd.fpart + part * 64;
+  a.wdeg = sd.wpart + static_cast<size_t>(hop) * kWParts * 64;
+  a.pw = sd.mode == 2 ? sd.pw : 0;
+  a.Fwm = sd.Fm[(hop - 1) & 1];
   return a;
 }
 
@@ -191,9 +200,10 @@ __device__ __forceinline__ HopCounters *part_ctr(const SlotDesc &sd, uint32_t h,
                                                        kPartStride);
 }
 
-// One wave: folds hop h's partial counters and front words into the primary
-// copy, zeroes the partials, and returns the folded counters to every lane.
-__device__ HopCounters fold_parts(const SlotDesc &sd, uint32_t h) {
+// One wave: folds hop h's partial counters, front words and per-word push
+// degrees into the primary copy, zeroes the partials, and returns the folded
+// counters to every lane (and lane j word j's push degree in wd).
+__device__ HopCounters fold_parts(const SlotDesc &sd, uint32_t h, uint64_t &wd) {
   const int lane = lane_id();
   HopCounters q{};
   if (lane < static_cast<int>(kParts)) {
@@ -222,6 +232,17 @@ __device__ HopCounters fold_parts(const SlotDesc &sd, uint32_t h) {
     }
   }
   if (f) sd.front[static_cast<size_t>(h) * 64 + lane] |= f;
+  uint64_t d = 0;
+  ull *wp = sd.wpart + static_cast<size_t>(h) * kWParts * 64 + lane;
+  for (uint32_t p = 0; p < kWParts; ++p) {
+    const uint64_t x = wp[p * 64];
+    if (x) {
+      d += x;
+      wp[p * 64] = 0;
+    }
+  }
+  wd = sd.wdeg[static_cast<size_t>(h) * 64 + lane] + d;
+  if (d) sd.wdeg[static_cast<size_t>(h) * 64 + lane] = wd;
   if (lane == 0) add_ctr(sd.ctr[h], w);
   return t;
 }
@@ -243,8 +264,10 @@ __global__ void k_bw_assign(BatchConst c) {
     HopCounters *h = &c.slots[i / (c.H + 1)].ctr[i % (c.H + 1)];
     *h = HopCounters{};
   }
-  for (uint32_t i = threadIdx.x; i < c.nslots * (c.H + 1) * 64; i += blockDim.x)
+  for (uint32_t i = threadIdx.x; i < c.nslots * (c.H + 1) * 64; i += blockDim.x) {
     c.slots[i / ((c.H + 1) * 64)].front[i % ((c.H + 1) * 64)] = 0;
+    c.slots[i / ((c.H + 1) * 64)].wdeg[i % ((c.H + 1) * 64)] = 0;
+  }
 }
 
 // ---- zero visited / result matrices of active slots (fill kernel: no
@@ -292,6 +315,7 @@ __global__ void k_bw_l0(BatchConst c) {
     if (oldF == 0) {
       atomicAdd(&h->nz, 1ull);
       atomicAdd(&h->nzdeg, static_cast<ull>(deg));
+      atomicAdd(&sd.wdeg[j], static_cast<ull>(deg));
     }
     atomicOr(reinterpret_cast<ull *>(&sd.Fm[0][v]), 1ull << j);
     if (sd.Vis) {
@@ -322,9 +346,12 @@ __global__ __launch_bounds__(64) void k_bw_decide(BatchConst c, uint32_t hop) {
   SlotDesc &sd = c.slots[s];
   const int lane = lane_id();
   int mode = 0;
-  const HopCounters pc = fold_parts(sd, hop - 1);
+  uint64_t wd = 0;  // lane j: push degree of word j of L_{hop-1}
+  const HopCounters pc = fold_parts(sd, hop - 1, wd);
+  bool wpush_ok = false;  // the tile writes a frontier this hop
   if (sd.tile != UINT32_MAX) {
     const TileInfo ti = c.tiles[sd.tile];
+    wpush_ok = hop < ti.hmax;
     if (pc.pop > 0 && hop <= ti.hmax) {
       if (c.sched == VS_SCHED_PUSH) mode = 1;
       else if (c.sched == VS_SCHED_PULL) mode = 2;
@@ -358,12 +385,27 @@ __global__ __launch_bounds__(64) void k_bw_decide(BatchConst c, uint32_t hop) {
       }
     }
   }
+  const bool masked = pc.nzdeg * 100 < pc.actdeg * c.TW * c.sparse_pct;
+  // word push (pull over whole rows, frontier written this hop): the words
+  // with push degree > 0, cheapest first (ties by word), within the budget
+  uint64_t pw = 0;
+  if (mode == 2 && !masked && wpush_ok && c.wpush_budget) {
+    const uint64_t w = lane < static_cast<int>(c.TW) ? wd : 0;
+    uint64_t cum = 0;
+    for (int k = 0; k < 64; ++k) {
+      const uint64_t wk = shfl64(w, k);
+      if (wk && (wk < w || (wk == w && k <= lane))) cum += wk;
+    }
+    pw = __ballot(w != 0 && cum <= c.wpush_budget);
+  }
   if (lane == 0) {
     sd.mode = mode;
+    sd.pw = pw;
+    if (pw) atomicAdd(&c.ctl->wpush_words, static_cast<ull>(__popcll(pw)));
     // most in-edges come from active vertices: pull skips the bitmap lookup
     sd.dense = (mode == 2 && pc.actdeg * 4 > 3 * c.E) ? 1u : 0u;
     // rows sparse in words: gather only the words their masks name
-    sd.masked = (pc.nzdeg * 100 < pc.actdeg * c.TW * c.sparse_pct) ? 1u : 0u;
+    sd.masked = masked ? 1u : 0u;
     sd.ctr[hop].how = (mode == 2 && sd.masked) ? 3u : static_cast<ull>(mode);
     if (mode == 1) atomicAdd(&c.ctl->push_dec, 1ull);
     if (mode == 2) atomicAdd(&c.ctl->pull_dec, 1ull);
@@ -379,9 +421,13 @@ __global__ __launch_bounds__(kBlock) void k_bw_items(BatchConst c, uint32_t hop)
   for (uint32_t u = blockIdx.x; u < c.nslots * nslice; u += gridDim.x) {
     const uint32_t s = u / nslice;
     const SlotDesc &sd = c.slots[s];
-    if (sd.mode != 1) continue;  // uniform across the block
+    // uniform across the block: push slots, and pull slots with a word push
+    // (only the vertices holding bits in a pushed word)
+    const bool wp = sd.mode == 2 && sd.pw;
+    if (sd.mode != 1 && !wp) continue;
     items_slice(sd.bm[(hop - 1) & 1], nvb, (u % nslice) * kBlock, c.xoff, sd.items,
-                &sd.ctr[hop - 1].items, s_wsum, &s_base);
+                &sd.ctr[hop - 1].items, s_wsum, &s_base, wp ? sd.Fm[(hop - 1) & 1] : nullptr,
+                wp ? sd.pw : 0);
   }
 }
 
@@ -389,17 +435,20 @@ __global__ __launch_bounds__(kBlock) void k_bw_items(BatchConst c, uint32_t hop)
 __global__ void k_bw_plan(BatchConst c, uint32_t hop) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const uint32_t nvb = (c.V + 63) / 64;
-  ull acc = 0, acc_s = 0;
+  ull acc = 0, acc_s = 0, acc_w = 0;
   for (uint32_t s = 0; s < c.nslots; ++s) {
     c.ctl->unit_prefix[s] = acc;
     c.ctl->unit_prefix_s[s] = acc_s;
+    c.ctl->unit_prefix_w[s] = acc_w;
     const SlotDesc &sd = c.slots[s];
     if (sd.mode == 2 && sd.masked) acc_s += c.npitems;
     else if (sd.mode == 2) acc += c.npitems;
     else if (sd.mode == 1) acc += sd.ctr[hop - 1].items;
+    if (sd.mode == 2 && sd.pw) acc_w += sd.ctr[hop - 1].items;
   }
   c.ctl->unit_prefix[c.nslots] = acc;
   c.ctl->unit_prefix_s[c.nslots] = acc_s;
+  c.ctl->unit_prefix_w[c.nslots] = acc_w;
 }
 
 // ---- the persistent expansion: pull vertex-blocks and push items of all
@@ -439,7 +488,7 @@ __global__ __launch_bounds__(kBlock) void k_bw_expand(BatchConst c, uint32_t hop
       se = prefix[s + 1];
     }
     if (s != cur) {
-      if (cur != UINT32_MAX && mode == 2) flush_ctr(lc, a.ctr, a.front_next, j);
+      if (cur != UINT32_MAX && mode == 2) flush_ctr(lc, a.ctr, a.front_next, a.wdeg, j);
       cur = s;
       a = slot_args(c, c.slots[s], hop);
       mode = c.slots[s].mode;
@@ -460,7 +509,31 @@ __global__ __launch_bounds__(kBlock) void k_bw_expand(BatchConst c, uint32_t hop
       push_item<TW>(a, a.items[local]);
     }
   }
-  if (cur != UINT32_MAX && mode == 2) flush_ctr(lc, a.ctr, a.front_next, j);
+  if (cur != UINT32_MAX && mode == 2) flush_ctr(lc, a.ctr, a.front_next, a.wdeg, j);
+}
+
+// ---- word push of the pull slots that have one: persistent over their
+// items, before the pull
+template <int TW>
+__global__ __launch_bounds__(kBlock) void k_bw_wpush(BatchConst c, uint32_t hop) {
+  const ull *prefix = c.ctl->unit_prefix_w;
+  const ull total = prefix[c.nslots];
+  const ull nwaves = static_cast<ull>(gridDim.x) * kWavesPerBlock;
+  uint32_t cur = UINT32_MAX;
+  HopArgs a;
+  ull sb = 0, se = 0;
+  for (ull u = blockIdx.x * kWavesPerBlock + wave_id(); u < total; u += nwaves) {
+    if (cur == UINT32_MAX || u < sb || u >= se) {
+      uint32_t s = cur == UINT32_MAX ? 0 : cur;
+      while (s + 1 <= c.nslots && prefix[s + 1] <= u) ++s;
+      while (prefix[s] > u) --s;
+      sb = prefix[s];
+      se = prefix[s + 1];
+      cur = s;
+      a = slot_args(c, c.slots[s], hop);
+    }
+    wpush_item<TW>(a, a.items[u - sb]);
+  }
 }
 
 // ---- push epilogue and frontier clear, grid-stride over (slot, vertex block)
@@ -483,14 +556,14 @@ __global__ __launch_bounds__(kBlock) void k_bw_epi(BatchConst c, uint32_t hop) {
     const uint32_t i = static_cast<uint32_t>(u % nvb);
     if (md == 2 && i >= c.nhwords) continue;
     if (s != cur) {
-      if (cur != UINT32_MAX) flush_ctr(lc, a.ctr, a.front_next, lane_id() % TW);
+      if (cur != UINT32_MAX) flush_ctr(lc, a.ctr, a.front_next, a.wdeg, lane_id() % TW);
       cur = s;
       a = slot_args(c, c.slots[s], hop);
       mk = lane_masks(a, lane_id() % TW);
     }
     epi_block<TW, SHORTEST>(a, md == 2 ? c.hwords[i] : i, mk, lc);
   }
-  if (cur != UINT32_MAX) flush_ctr(lc, a.ctr, a.front_next, lane_id() % TW);
+  if (cur != UINT32_MAX) flush_ctr(lc, a.ctr, a.front_next, a.wdeg, lane_id() % TW);
 }
 
 template <int TW>
@@ -666,6 +739,8 @@ __global__ void k_bw_finish(BatchConst c) {
         fp[p * 64] = 0;
       }
       sd.front[static_cast<size_t>(c.H) * 64 + i % 64] |= f;
+      ull *wp = sd.wpart + static_cast<size_t>(c.H) * kWParts * 64 + i % 64;
+      for (uint32_t p = 0; p < kWParts; ++p) wp[p * 64] = 0;
     }
     if (threadIdx.x == 0)
       for (uint32_t s = 0; s < c.nslots; ++s)
@@ -824,6 +899,7 @@ struct ExpandEvents {
 template <int TW>
 static void enqueue_hop_tw(const BatchConst &c, bool shortest, uint32_t hop,
                            uint32_t grid, hipStream_t s, ExpandEvents *ev) {
+  hipLaunchKernelGGL((k_bw_wpush<TW>), dim3(grid), dim3(kBlock), 0, s, c, hop);
   if (ev) ev->record(s, true, false);
   if (shortest)
     hipLaunchKernelGGL((k_bw_expand<TW, true, false>), dim3(grid), dim3(kBlock), 0, s, c, hop);
@@ -871,7 +947,7 @@ static uint32_t enqueue_wave(const BatchConst &c, bool shortest, uint32_t grid,
     hipLaunchKernelGGL(k_bw_items, dim3(std::max<uint32_t>(1, item_grid)), dim3(kBlock), 0, s, c, hop);
     hipLaunchKernelGGL(k_bw_plan, dim3(1), dim3(64), 0, s, c, hop);
     VS_TW_BATCH(c.TW, enqueue_hop_tw, c, shortest, hop, grid, s, ev);
-    n += 7;
+    n += 8;
   }
   VS_TW_BATCH(c.TW, enqueue_final_tw, c, grid, s);
   hipLaunchKernelGGL(k_bw_finish, dim3(std::max<uint32_t>(1, (c.nslots * c.TW * 64 + 255) / 256)),
@@ -1194,7 +1270,8 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   const size_t per_slot = mat * (shortest ? 4 : 3) + (use_r ? mat : 0) + 3 * nvb * 8 +
                           2 * static_cast<size_t>(V) * 8 +
                           items_cap * 8 + (H + 1) * sizeof(HopCounters) +
-                          (H + 1) * kParts * (kPartStride + 64 * 8);
+                          (H + 1) * kParts * (kPartStride + 64 * 8) +
+                          (H + 1) * (kWParts + 1) * 64 * 8;
   size_t freeb = 0, totalb = 0;
   VS_HIP(hipMemGetInfo(&freeb, &totalb));
   freeb += g->pool.cached();  // the slot buffers come from the graph's pool
@@ -1291,6 +1368,11 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
     sd[i].part = static_cast<char *>(q);
     VS_TRY(get((H + 1) * kParts * 64 * 8ull, true, &q));
     sd[i].fpart = static_cast<uint64_t *>(q);
+    VS_TRY(get((H + 1) * 64 * 8ull, true, &q));
+    sd[i].wdeg = static_cast<ull *>(q);
+    VS_TRY(get((H + 1) * kWParts * 64 * 8ull, true, &q));
+    sd[i].wpart = static_cast<ull *>(q);
+    sd[i].pw = 0;
     sd[i].tile = UINT32_MAX;
     sd[i].mode = 0;
   }
@@ -1313,6 +1395,7 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   c.sparse_pct = g->tuning.sparse_pct ? g->tuning.sparse_pct : kSparsePct;
   c.use_r = use_r ? 1 : 0;
   c.E = push->E;
+  c.wpush_budget = word_push_budget(g->tuning, push->E);
   c.poff = pull->off;
   c.padj = pull->adj;
   c.xoff = push->off;
@@ -1480,6 +1563,7 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   st.pull_decisions += ctl.pull_dec;
   st.bytes_alg += ctl.bytes;
   st.bytes_sparse += ctl.bytes_sparse;
+  st.wpush_words += ctl.wpush_words;
   return VS_OK;
 }
 

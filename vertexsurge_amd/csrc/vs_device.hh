@@ -100,6 +100,11 @@ constexpr uint32_t kPullHubs = 131072;
 // dense pull took 41.4 ms and fetched 195 GB, the word-mask pull 47.1 ms and
 // 244 GB; hop 2 (far below 35) stays on the word masks.
 constexpr uint32_t kSparsePct = 35;
+// default word-push budget, per mille of E: on R-MAT 24 / 4096 sources the
+// hop-3 frontier's sparsest word reaches 2.6 % of E by push degree (and its
+// lane keeps every destination's pull going through all of its in-edges),
+// the next one 14 %
+constexpr uint32_t kWPushPml = 40;
 
 // Builds the pull items of a CSR once (vertex | chunk << 32 per kPullChunk
 // in-edges, vertices without edges omitted).

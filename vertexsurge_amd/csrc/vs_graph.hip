@@ -174,14 +174,23 @@ vs_status ensure_pull_items(DevCSR &c, uint32_t V, uint32_t hubs) {
   it.reserve(static_cast<size_t>(V) + c.E / kPullChunk + 1);
   std::vector<uint32_t> hw;
   bool heavy = false;
+  // Every first chunk before any later one: items are strided over the
+  // waves in list order, so a chunked vertex's first chunk (its hub
+  // neighbours) has usually finished before its later chunks start, and
+  // they skip the rows it already found.
   for (uint32_t v = 0; v < V; ++v) {
     const uint64_t d = c.hoff[v + 1] - c.hoff[v];
-    const uint64_t nch = (d + kPullChunk - 1) / kPullChunk;
-    if (nch > 1) {
+    if (d == 0) continue;
+    it.push_back(v);
+    if (d > kPullChunk) {
       heavy = true;
       if (hw.empty() || hw.back() != v / 64) hw.push_back(v / 64);
     }
-    for (uint64_t q = 0; q < nch; ++q) it.push_back(v | (q << 32));
+  }
+  for (uint32_t v = 0; v < V; ++v) {
+    const uint64_t d = c.hoff[v + 1] - c.hoff[v];
+    const uint64_t nch = (d + kPullChunk - 1) / kPullChunk;
+    for (uint64_t q = 1; q < nch; ++q) it.push_back(v | (q << 32));
   }
   if (!hw.empty()) {
     void *ph = nullptr;

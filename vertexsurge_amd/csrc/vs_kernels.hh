@@ -29,6 +29,17 @@
 // sparse frontier rows (hop 2 gathers rows of the sources' neighbours, a few
 // words each) then cost a few cache lines instead of TW words.
 //
+// Word push: on a dense pull hop, the frontier words that are cheapest to
+// scatter (the fewest out-edges of rows holding bits there, summed over the
+// words from the cheapest up to a budget) are pushed instead of pulled.  A
+// pull lane stops gathering once its word holds every row it still needs;
+// words whose rows mostly lie farther away never get there and keep the
+// wave reading every in-edge, so taking those few sparse words out of the
+// pull lets the dense ones finish it early.  The scatter ORs them into N
+// before the pull; the pull never gathers them: the first item of each
+// destination takes them from N (a light vertex finishes them with its
+// pulled words, a chunked one leaves them to its epilogue).
+//
 // Row masks: `live` bits are the rows that still expand at this hop (rows
 // < S, and in a batch the rows whose query has k_max >= hop); `racc` bits are
 // the rows whose result takes this hop's layer (k_min <= hop); `live_next`
@@ -94,22 +105,59 @@ struct HopArgs {
   // keep every destination's early exit from firing.
   const uint64_t *front;
   uint64_t *front_next;      // the same OR for L_i, accumulated here
+  // Per-word push degree of L_i: [kWParts][64] partial sums (a block adds
+  // into partial blockIdx % kWParts), word j summing the push-degree of
+  // every vertex whose row holds bits in word j; or nullptr.
+  ull *wdeg;
+  // Word push of this hop: the pushed words of the tile (0: none; N set
+  // otherwise) and the frontier's word masks (read by the scatter).
+  uint64_t pw;
+  const uint64_t *Fwm;
 };
+
+// Partial copies of the per-word push-degree sums.
+constexpr uint32_t kWParts = 16;
+
+// Word-push budget (push-degree sum of the pushed words) of a graph tuning.
+inline uint64_t word_push_budget(const vs_tuning &t, uint64_t E) {
+  const uint64_t pml = t.wpush_pml ? t.wpush_pml : kWPushPml;
+  return pml > 1000 ? 0 : E / 1000 * pml + E % 1000 * pml / 1000;
+}
+
+// Word push choice of one tile from the push degrees of its frontier words:
+// the words with degree > 0, cheapest first (ties by word index), while the
+// degrees taken sum to at most budget.
+inline uint64_t word_push_pick(const uint64_t *wdeg, uint32_t TW, uint64_t budget) {
+  uint64_t pw = 0;
+  for (uint32_t j = 0; j < TW; ++j) {
+    const uint64_t w = wdeg[j];
+    if (!w) continue;
+    uint64_t cum = 0;
+    for (uint32_t k = 0; k < TW; ++k)
+      if (wdeg[k] && (wdeg[k] < w || (wdeg[k] == w && k <= j))) cum += wdeg[k];
+    if (cum <= budget) pw |= 1ull << j;
+  }
+  return pw;
+}
 
 // Per-lane partial counters, summed over the wave at the end.
 struct LocalCtr {
-  uint64_t pop = 0, nz = 0, act = 0, actdeg = 0, te = 0, visnew = 0, rnew = 0, nzdeg = 0;
+  uint64_t pop = 0, nz = 0, act = 0, actdeg = 0, te = 0, visnew = 0, rnew = 0;
+  uint64_t wdeg = 0;   // push-degree of the rows whose word j (this lane's) became
+                       // nonzero: summed over the lanes it is nzdeg
   uint64_t rowor = 0;  // OR of the words this lane emitted (its word index j)
 };
 
 __device__ __forceinline__ void flush_ctr(LocalCtr &c, HopCounters *ctr,
-                                          uint64_t *front_next, int j) {
+                                          uint64_t *front_next, ull *wdeg, int j) {
   if (front_next && c.rowor)
     atomicOr(reinterpret_cast<ull *>(&front_next[j]), static_cast<ull>(c.rowor));
+  if (wdeg && c.wdeg)
+    atomicAdd(&wdeg[(blockIdx.x % kWParts) * 64 + j], static_cast<ull>(c.wdeg));
   const uint64_t pop = wave_sum(c.pop), nz = wave_sum(c.nz), act = wave_sum(c.act),
                  actdeg = wave_sum(c.actdeg), te = wave_sum(c.te),
                  visnew = wave_sum(c.visnew), rnew = wave_sum(c.rnew),
-                 nzdeg = wave_sum(c.nzdeg);
+                 nzdeg = wave_sum(c.wdeg);
   if (lane_id() == 0) {
     if (pop) atomicAdd(&ctr->pop, static_cast<ull>(pop));
     if (nz) atomicAdd(&ctr->nz, static_cast<ull>(nz));
@@ -348,16 +396,25 @@ __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, u
   const uint32_t v = h.v;
   const size_t wi = static_cast<size_t>(v) * TW + j;
   const uint64_t vis = h.vis;
-  const uint64_t need = mk.live & mk.front & ~vis;
-  if (__ballot(need != 0) == 0) return;
   const uint64_t lb = h.lb, le = h.le;
   const bool heavy = le - lb > kPullChunk;
+  // Pushed words (word push) are never gathered: the scatter left their
+  // bits in N, and the vertex's first item takes them -- a light vertex
+  // finishes them with its pulled words, a chunked one leaves them in N for
+  // its epilogue.
+  const bool pwl = !SPARSE && ((a.pw >> j) & 1);
+  uint64_t acc = (pwl && h.ch == 0) ? a.N[wi] : 0;
+  uint64_t need = mk.live & mk.front & ~vis;
+  // A later chunk of a chunked vertex does not look again for what the
+  // vertex's other items already put into N: the items list every first
+  // chunk before any later one, so by now the first chunk (the hub
+  // neighbours, the densest rows) has usually found most of it.
+  if (heavy && h.ch > 0) need &= ~a.N[wi];
+  bool todo = need != 0 && !pwl;
   const uint64_t b = lb + static_cast<uint64_t>(h.ch) * kPullChunk;
-  const uint64_t e = min(b + kPullChunk, le);
+  const uint64_t e = __ballot(todo) ? min(b + kPullChunk, le) : b;
   const bool dense = a.dense != 0;
   const uint64_t hub_end = h.hubs == 0xffffffffu ? le : lb + h.hubs;
-  uint64_t acc = 0;
-  bool todo = need != 0;
   if (lane < TW) accw[lane] = 0;
   for (uint64_t base = b; base < e; base += 64) {
     const uint64_t idx = base + lane;
@@ -400,7 +457,7 @@ __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, u
     }
     wave_sync();
     acc = group_or<TW>(acc);
-    todo = (acc & need) != need;
+    todo = !pwl && (acc & need) != need;
     if (__ballot(todo) == 0) break;  // nothing left to find
   }
   acc = group_or<TW>(acc);
@@ -409,12 +466,16 @@ __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, u
   const uint64_t wmask = __ballot(g == 0 && nb != 0);
   const bool row_new = wmask != 0;
   if (heavy) {
-    if (g == 0 && nb)
+    // the epilogue finishes v when it is touched: new bits, or pushed bits
+    // in N (which it also clears)
+    const bool take = __ballot(g == 0 && (nb != 0 || (pwl && acc != 0))) != 0;
+    if (g == 0 && nb && !pwl)
       atomicOr(reinterpret_cast<ull *>(&a.N[wi]), static_cast<ull>(nb));
-    if (row_new && lane == 0)
+    if (take && lane == 0)
       atomicOr(reinterpret_cast<ull *>(&a.touched[v >> 6]), 1ull << (v & 63));
     return;
   }
+  if (g == 0 && pwl && acc) a.N[wi] = 0;  // the pushed bits are taken
   if (g == 0 && nb) emit_word<SHORTEST>(a, wi, nb, vis, mk.racc, c);
   if (row_new) {
     const uint64_t deg = a.xoff == a.poff ? le - lb : a.xoff[v + 1] - a.xoff[v];
@@ -423,9 +484,11 @@ __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, u
       c.actdeg += deg;
       if (a.actbm_n) atomicOr(reinterpret_cast<ull *>(&a.actbm_n[v >> 6]), 1ull << (v & 63));
       if (a.Fnm) a.Fnm[v] = wmask;
-      c.nzdeg += deg * __popcll(wmask);
     }
-    if (g == 0) c.te += deg * __popcll(nb & mk.live_next);
+    if (g == 0) {
+      c.te += deg * __popcll(nb & mk.live_next);
+      if (nb) c.wdeg += deg;
+    }
   }
 }
 
@@ -454,6 +517,33 @@ __device__ __forceinline__ void push_item(const HopArgs &a, ull item) {
                         static_cast<ull>(f));
         if (j == 0)
           atomicOr(reinterpret_cast<ull *>(&a.touched[v >> 6]), 1ull << (v & 63));
+      }
+    }
+  }
+}
+
+// Word push, one item = (vertex u, chunk of kPushChunk out-neighbours), one
+// lane per neighbour: ORs u's pushed words that hold bits into N.  Items are
+// built only for vertices holding bits in a pushed word.
+template <int TW>
+__device__ __forceinline__ void wpush_item(const HopArgs &a, ull item) {
+  const int lane = lane_id();
+  const uint32_t u = static_cast<uint32_t>(item);
+  const uint32_t ch = static_cast<uint32_t>(item >> 32);
+  const uint64_t ue = a.xoff[u + 1];
+  const uint64_t b = a.xoff[u] + static_cast<uint64_t>(ch) * kPushChunk;
+  const uint64_t e = min(b + kPushChunk, ue);
+  const uint64_t wm = a.Fwm[u] & a.pw;
+  const uint64_t *fu = a.F + static_cast<size_t>(u) * TW;
+  for (uint64_t base = b; base < e; base += 64) {
+    const uint64_t idx = base + lane;
+    if (idx < e) {
+      const uint32_t v = a.xadj[idx];
+      for (uint64_t m = wm; m; m &= m - 1) {
+        const int jj = __ffsll(static_cast<long long>(m)) - 1;
+        const uint64_t f = fu[jj];
+        if (f) atomicOr(reinterpret_cast<ull *>(&a.N[static_cast<size_t>(v) * TW + jj]),
+                        static_cast<ull>(f));
       }
     }
   }
@@ -509,10 +599,10 @@ __device__ __forceinline__ void epi_block(const HopArgs &a, uint32_t vb,
           actmask |= 1ull << i;
           c.act += 1;
           c.actdeg += deg;
-          c.nzdeg += deg * __popcll(gb);
           if (a.Fnm) a.Fnm[v] = gb;
         }
         c.te += deg * __popcll(nb & mk.live_next);
+        if (nb) c.wdeg += deg;
       }
     }
   }
@@ -575,13 +665,23 @@ __device__ __forceinline__ void clear_block_masked(uint64_t *F, uint64_t *Fm, ui
 // of kPushChunk-neighbour chunks, block-level exclusive scan, one atomicAdd
 // per block.  Must be called by the whole block (uses __syncthreads).
 
+// With fm set, only the vertices whose word mask meets pw (word push).
 __device__ __forceinline__ void items_slice(const uint64_t *actbm, uint32_t nvb,
                                             uint32_t base, const uint64_t *xoff,
                                             ull *items, ull *nitems, ull *s_wsum,
-                                            ull *s_base) {
+                                            ull *s_base, const uint64_t *fm = nullptr,
+                                            uint64_t pw = 0) {
   const int lane = lane_id(), wid = wave_id();
   const uint32_t wb = base + threadIdx.x;
-  const uint64_t m = wb < nvb ? actbm[wb] : 0;
+  uint64_t m = wb < nvb ? actbm[wb] : 0;
+  if (fm) {
+    uint64_t keep = 0;
+    for (uint64_t mm = m; mm; mm &= mm - 1) {
+      const int i = __ffsll(static_cast<long long>(mm)) - 1;
+      if (fm[wb * 64 + i] & pw) keep |= 1ull << i;
+    }
+    m = keep;
+  }
   ull cnt = 0;
   for (uint64_t mm = m; mm; mm &= mm - 1) {
     const uint32_t v = wb * 64 + (__ffsll(static_cast<long long>(mm)) - 1);
@@ -642,7 +742,7 @@ __global__ __launch_bounds__(kBlock) void k_pull(HopArgs a) {
     h = hn;
     it = nx;
   }
-  flush_ctr(c, a.ctr, a.front_next, j);
+  flush_ctr(c, a.ctr, a.front_next, a.wdeg, j);
 }
 
 template <int TW>
@@ -663,7 +763,16 @@ __global__ __launch_bounds__(kBlock) void k_push_epi(HopArgs a) {
   LocalCtr c;
   for (uint32_t i = blockIdx.x * kWavesPerBlock + wid; i < n; i += nwaves)
     epi_block<TW, SHORTEST>(a, a.ewords ? a.ewords[i] : i, mk, c);
-  flush_ctr(c, a.ctr, a.front_next, lane_id() % TW);
+  flush_ctr(c, a.ctr, a.front_next, a.wdeg, lane_id() % TW);
+}
+
+template <int TW>
+__global__ __launch_bounds__(kBlock) void k_wpush(HopArgs a) {
+  const int wid = wave_id();
+  const uint64_t nitems = *a.nitems;
+  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
+  for (uint64_t it = blockIdx.x * kWavesPerBlock + wid; it < nitems; it += nwaves)
+    wpush_item<TW>(a, a.items[it]);
 }
 
 template <int TW>
@@ -678,7 +787,8 @@ __global__ __launch_bounds__(kBlock) void k_clear(uint64_t *F, uint64_t *Fm, uin
 
 __global__ __launch_bounds__(kBlock) void k_items(const uint64_t *actbm, uint32_t V,
                                                  const uint64_t *xoff, ull *items,
-                                                 ull *nitems);
+                                                 ull *nitems, const uint64_t *fm,
+                                                 uint64_t pw);
 
 // Launchers (vs_vexpand.hip).
 void launch_pull(uint32_t tw, bool shortest, uint32_t grid, hipStream_t s,
@@ -690,7 +800,10 @@ void launch_epi(uint32_t tw, bool shortest, uint32_t grid, hipStream_t s,
 void launch_clear(uint32_t tw, uint32_t grid, hipStream_t s, uint64_t *F, uint64_t *Fm,
                   uint64_t *bm, uint32_t V);
 void launch_items(uint32_t grid, hipStream_t s, const uint64_t *bm, uint32_t V,
-                  const uint64_t *xoff, ull *items, ull *nitems);
+                  const uint64_t *xoff, ull *items, ull *nitems,
+                  const uint64_t *fm = nullptr, uint64_t pw = 0);
+void launch_wpush(uint32_t tw, bool shortest, uint32_t grid, hipStream_t s,
+                  const HopArgs &a);
 uint32_t choose_tile_words(uint32_t requested, uint32_t W);
 
 } // namespace vs

@@ -23,12 +23,13 @@ namespace vs {
 
 __global__ __launch_bounds__(kBlock) void k_items(const uint64_t *actbm, uint32_t V,
                                                  const uint64_t *xoff, ull *items,
-                                                 ull *nitems) {
+                                                 ull *nitems, const uint64_t *fm,
+                                                 uint64_t pw) {
   __shared__ ull s_wsum[kWavesPerBlock];
   __shared__ ull s_base;
   const uint32_t nvb = (V + 63) / 64;
   for (uint32_t base = blockIdx.x * kBlock; base < nvb; base += gridDim.x * kBlock)
-    items_slice(actbm, nvb, base, xoff, items, nitems, s_wsum, &s_base);
+    items_slice(actbm, nvb, base, xoff, items, nitems, s_wsum, &s_base, fm, pw);
 }
 
 // L0: row r has only its source vertex (ExpandStatus::init_first_m_layer,
@@ -150,9 +151,21 @@ void launch_clear(uint32_t tw, uint32_t grid, hipStream_t s, uint64_t *F, uint64
 
 
 void launch_items(uint32_t grid, hipStream_t s, const uint64_t *bm, uint32_t V,
-                  const uint64_t *xoff, ull *items, ull *nitems) {
+                  const uint64_t *xoff, ull *items, ull *nitems, const uint64_t *fm,
+                  uint64_t pw) {
   hipLaunchKernelGGL(k_items, dim3(grid), dim3(kBlock), 0, s, bm, V, xoff, items,
-                     nitems);
+                     nitems, fm, pw);
+}
+
+template <int TW>
+static void launch_wpush_tw(bool shortest, uint32_t grid, hipStream_t s, const HopArgs &a) {
+  (void)shortest;
+  hipLaunchKernelGGL((k_wpush<TW>), dim3(grid), dim3(kBlock), 0, s, a);
+}
+
+void launch_wpush(uint32_t tw, bool shortest, uint32_t grid, hipStream_t s,
+                  const HopArgs &a) {
+  VS_TW_CALL(tw, launch_wpush_tw, shortest, grid, s, a);
 }
 
 uint32_t choose_tile_words(uint32_t requested, uint32_t W) {
@@ -219,8 +232,9 @@ static double ev_ms(hipEvent_t a, hipEvent_t b) {
 // Host-side counters of L0 for tile t (sources are on the host anyway).
 static void l0_counters(const uint32_t *src, uint32_t S, uint32_t TW, uint32_t t,
                         const std::vector<uint64_t> &xoff, bool vis, bool r,
-                        HopCounters &c) {
+                        HopCounters &c, uint64_t *wdeg) {
   std::memset(&c, 0, sizeof(c));
+  std::fill(wdeg, wdeg + 64, 0ull);
   const uint64_t r0 = static_cast<uint64_t>(t) * TW * 64;
   const uint64_t r1 = std::min<uint64_t>(S, r0 + static_cast<uint64_t>(TW) * 64);
   std::unordered_set<uint64_t> words;
@@ -233,6 +247,7 @@ static void l0_counters(const uint32_t *src, uint32_t S, uint32_t TW, uint32_t t
     if (words.insert((row / 64) << 32 | s).second) {
       c.nz += 1;
       c.nzdeg += d;
+      wdeg[(row / 64) % TW] += d;
     }
     if (verts.insert(s).second) {
       c.act += 1;
@@ -457,6 +472,17 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
   }
   std::vector<HopCounters> hctr(nctr);
   auto C = [&](uint32_t hop, uint32_t t) -> HopCounters & { return hctr[static_cast<size_t>(hop) * T + t]; };
+  // per (hop, tile) per-word push degrees of L_hop: kWParts partial copies
+  // on the device, folded here after each hop
+  const size_t wparts = static_cast<size_t>(kWParts) * 64;
+  void *d_wdeg_v = nullptr;
+  VS_TRY(lease.get(nctr * wparts * 8, true, &d_wdeg_v));
+  ull *d_wdeg = static_cast<ull *>(d_wdeg_v);
+  std::vector<uint64_t> hwdeg(nctr * 64, 0), wpart(static_cast<size_t>(T) * wparts);
+  auto Wd = [&](uint32_t hop, uint32_t t) -> uint64_t * {
+    return &hwdeg[(static_cast<size_t>(hop) * T + t) * 64];
+  };
+  const uint64_t wpush_budget = word_push_budget(g->tuning, push->E);
 
   // ---- L0
   {
@@ -466,7 +492,7 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
                        BM[0], Vis, (R && k_min == 0) ? R : nullptr);
     VS_HIP(hipGetLastError());
     for (uint32_t t = 0; t < T; ++t)
-      l0_counters(sources, S, TW, t, push->hoff, shortest, R && k_min == 0, C(0, t));
+      l0_counters(sources, S, TW, t, push->hoff, shortest, R && k_min == 0, C(0, t), Wd(0, t));
   }
   uint32_t found = 0;
   if (find_shortest) {
@@ -562,6 +588,9 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
       a.racc = nullptr;
       a.front = d_front + (static_cast<size_t>(hop - 1) * T + t) * 64;
       a.front_next = d_front + (static_cast<size_t>(hop) * T + t) * 64;
+      a.wdeg = d_wdeg + (static_cast<size_t>(hop) * T + t) * wparts;
+      a.pw = 0;
+      a.Fwm = nullptr;
       hipEvent_t k0 = timing ? evs.make() : nullptr, k1 = timing ? evs.make() : nullptr;
       if (mode == 1) {
         VS_TRY(ensure_push());
@@ -584,12 +613,25 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
         a.nitems = nullptr;
         a.N = nullptr;
         a.touched = nullptr;
-        if (pull->has_heavy) {  // hub partials meet in N, finished by the epilogue
+        // word push: only when whole rows are gathered (the word-mask pull
+        // reads only the words that hold bits anyway)
+        const uint64_t pw = a.Fm ? 0 : word_push_pick(Wd(hop - 1, t), TW, wpush_budget);
+        if (pull->has_heavy || pw) {  // hub partials / pushed words meet in N
           VS_TRY(ensure_push());
           a.N = N + t * tile_words;
           a.touched = touched + static_cast<size_t>(t) * nvb;
         }
         if (timing) VS_HIP(hipEventRecord(k0, s));
+        if (pw) {
+          a.pw = pw;
+          a.Fwm = FM[cur] + static_cast<size_t>(t) * V;
+          a.items = items + static_cast<size_t>(t) * items_cap;
+          a.nitems = &d_ctr[static_cast<size_t>(hop - 1) * T + t].items;
+          launch_items(item_grid, s, a.actbm, V, push->off, const_cast<ull *>(a.items),
+                       const_cast<ull *>(a.nitems), a.Fwm, pw);
+          launch_wpush(TW, shortest, push_grid, s, a);
+          if (st) st->wpush_words[hop - 1] += __builtin_popcountll(pw);
+        }
         launch_pull(TW, shortest, pull_grid, s, a);
         if (timing) VS_HIP(hipEventRecord(k1, s));
         if (pull->has_heavy) {  // only the words that hold chunked vertices
@@ -609,6 +651,8 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
     if (timing) VS_HIP(hipEventRecord(eh1, s));
     VS_HIP(hipMemcpyAsync(&C(hop, 0), d_ctr + static_cast<size_t>(hop) * T,
                           sizeof(HopCounters) * T, hipMemcpyDeviceToHost, s));
+    VS_HIP(hipMemcpyAsync(wpart.data(), d_wdeg + static_cast<size_t>(hop) * T * wparts,
+                          wpart.size() * 8, hipMemcpyDeviceToHost, s));
     if (find_shortest) {
       const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((S + 255) / 256, 1024));
       hipLaunchKernelGGL(k_fs_check, dim3(grid), dim3(256), 0, s, Fn,
@@ -622,6 +666,8 @@ static vs_status vexpand_run(vs_graph_s *g, const uint32_t *sources, uint32_t S,
       VS_HIP(hipMemcpyAsync(&found_now, d_found, 8, hipMemcpyDeviceToHost, s));
     VS_HIP(hipStreamSynchronize(s));
     k_actual = hop;
+    for (uint32_t t = 0; t < T; ++t)
+      for (size_t q = 0; q < wparts; ++q) Wd(hop, t)[q % 64] += wpart[t * wparts + q];
     uint64_t newbits = 0, prev_vis = vis_nz, prev_r = r_nz, te = 0, bytes = 0;
     uint64_t frontier_bits = 0, frontier_act = 0, frontier_deg = 0;
     for (uint32_t t = 0; t < T; ++t) {
