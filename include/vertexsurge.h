@@ -624,7 +624,8 @@ vs_status vs_mintersect_varrow(vs_result_t base, uint32_t i, const vs_colmat_t *
 /* Tunes the batched engine on the caller's queries and stores the result in
  * the graph's tuning: coordinate descent over tile_words (the widest the
  * rows allow and two narrower), push_alpha (4, 16, 64), sparse_pct (35, 50,
- * 65) and pull_hubs (32768, 131072, 524288, at most V), each candidate timed
+ * 65), pull_hubs (32768, 131072, 524288, at most V) and wpush_pml (40, 200,
+ * 2000 = off), each candidate timed
  * by vs_vexpand_batch (device time, the mean of two replays of the captured
  * graph after a warm one).  The starting tuning (the graph's, or
  * p->tile_words when set) is measured first (*start_ms) and a candidate is

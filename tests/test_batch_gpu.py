@@ -229,6 +229,7 @@ def test_autotune_batch_exact_and_no_slower(g14):
         assert 0 < best <= start
         assert G.get_tuning() == tuning
         assert tuning["sparse_pct"] in (35, 50, 65) and tuning["push_alpha"] in (4, 16, 64)
+        assert tuning["wpush_pml"] in (40, 200, 2000)
         c1, d1, _ = vs.vexpand_batch(G, qs)
         assert np.array_equal(c0, c1) and np.array_equal(d0, d1)
     finally:

@@ -111,6 +111,7 @@ struct BatchConst {
   const TileInfo *tiles;
   BatchCtl *ctl;
   ull *out_counts;        // [S] (group order), zeroed before the first wave
+  uint32_t want_rows;     // per-row counts / digests wanted: the row pass runs
   ull *out_digests;       // [S] or nullptr
   const float *mlp;       // scheduler parameters or nullptr
   const uint32_t *hubcnt; // hub neighbours at the head of each pull list
@@ -455,8 +456,61 @@ __global__ void k_bw_plan(BatchConst c, uint32_t hop) {
 // slots; work unit u belongs to the slot whose prefix range holds it.  Two
 // launches per hop: SPARSE takes the pull slots whose frontier rows are read
 // through their word masks, the other one the push and dense pull slots.
+__device__ __forceinline__ uint32_t rfl32(uint32_t x) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(x)));
+}
+__device__ __forceinline__ uint64_t rfl64(uint64_t x) {
+  return static_cast<uint64_t>(rfl32(static_cast<uint32_t>(x))) |
+         (static_cast<uint64_t>(rfl32(static_cast<uint32_t>(x >> 32))) << 32);
+}
+template <class T>
+__device__ __forceinline__ T *rflp(T *p) {
+  return reinterpret_cast<T *>(rfl64(reinterpret_cast<uint64_t>(p)));
+}
+
+// A slot's arguments are the same in every lane; saying so keeps them in
+// scalar registers (the compiler cannot tell on its own: they are loaded
+// through a slot index found by a search).
+__device__ __forceinline__ void uniform_args(HopArgs &a) {
+  a.V = rfl32(a.V);
+  a.S = rfl32(a.S);
+  a.word0 = rfl32(a.word0);
+  a.dense = rfl32(a.dense);
+  a.poff = rflp(a.poff);
+  a.padj = rflp(a.padj);
+  a.xoff = rflp(a.xoff);
+  a.xadj = rflp(a.xadj);
+  a.F = rflp(a.F);
+  a.actbm = rflp(a.actbm);
+  a.Fm = rflp(a.Fm);
+  a.Fn = rflp(a.Fn);
+  a.Fnm = rflp(a.Fnm);
+  a.actbm_n = rflp(a.actbm_n);
+  a.N = rflp(a.N);
+  a.touched = rflp(a.touched);
+  a.Vis = rflp(a.Vis);
+  a.R = rflp(a.R);
+  a.items = rflp(a.items);
+  a.nitems = rflp(a.nitems);
+  a.pitems = rflp(a.pitems);
+  a.npitems = rfl64(a.npitems);
+  a.ewords = rflp(a.ewords);
+  a.newords = rfl32(a.newords);
+  a.hubcnt = rflp(a.hubcnt);
+  a.ctr = rflp(a.ctr);
+  a.live = rflp(a.live);
+  a.live_next = rflp(a.live_next);
+  a.racc = rflp(a.racc);
+  a.front = rflp(a.front);
+  a.front_next = rflp(a.front_next);
+  a.wdeg = rflp(a.wdeg);
+  a.pw = rfl64(a.pw);
+  a.Fwm = rflp(a.Fwm);
+}
+
 template <int TW, bool SHORTEST, bool SPARSE>
-__global__ __launch_bounds__(kBlock) void k_bw_expand(BatchConst c, uint32_t hop) {
+__global__ __launch_bounds__(kBlock)
+void k_bw_expand(BatchConst c, uint32_t hop) {
   __shared__ uint32_t s_nbr[kWavesPerBlock][64];
   __shared__ uint64_t s_nm[kWavesPerBlock][SPARSE ? 64 : 1];
   __shared__ uint16_t s_pairs[kWavesPerBlock][SPARSE ? kPairCap : 1];
@@ -491,7 +545,8 @@ __global__ __launch_bounds__(kBlock) void k_bw_expand(BatchConst c, uint32_t hop
       if (cur != UINT32_MAX && mode == 2) flush_ctr(lc, a.ctr, a.front_next, a.wdeg, j);
       cur = s;
       a = slot_args(c, c.slots[s], hop);
-      mode = c.slots[s].mode;
+      uniform_args(a);
+      mode = __builtin_amdgcn_readfirstlane(c.slots[s].mode);
       mk = lane_masks(a, j);
     }
     const ull slot_end = se;
@@ -924,8 +979,9 @@ static void enqueue_final_tw(const BatchConst &c, uint32_t grid, hipStream_t s) 
     hipLaunchKernelGGL(k_bw_subsrc, dim3(std::max<uint32_t>(1, (c.nslots * TW * 64 + 255) / 256)),
                        dim3(256), 0, s, c);
   // half the usual grid: each block accumulates two chunk groups in LDS
-  hipLaunchKernelGGL((k_bw_rows<TW>), dim3(std::max<uint32_t>(1, grid / 2)), dim3(kBlock), 0, s,
-                     c);
+  if (c.want_rows)
+    hipLaunchKernelGGL((k_bw_rows<TW>), dim3(std::max<uint32_t>(1, grid / 2)), dim3(kBlock), 0,
+                       s, c);
   if (c.wdest) hipLaunchKernelGGL((k_bw_export<TW>), dim3(grid), dim3(kBlock), 0, s, c);
 }
 
@@ -953,7 +1009,7 @@ static uint32_t enqueue_wave(const BatchConst &c, bool shortest, uint32_t grid,
   hipLaunchKernelGGL(k_bw_finish, dim3(std::max<uint32_t>(1, (c.nslots * c.TW * 64 + 255) / 256)),
                      dim3(256), 0, s, c);
   hipLaunchKernelGGL(k_bw_advance, dim3(1), dim3(64), 0, s, c.ctl);
-  n += 4 + (c.res_sub ? 1 : 0) + (c.wdest ? 1 : 0);
+  n += 3 + (c.want_rows ? 1 : 0) + (c.res_sub ? 1 : 0) + (c.wdest ? 1 : 0);
   return n;
 }
 
@@ -1050,7 +1106,8 @@ struct GroupResults {
 static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &qs,
                            const vs_batch_params &p, bool shortest,
                            std::vector<uint64_t> &counts, std::vector<uint64_t> &digests,
-                           bool want_digests, vs_batch_stats &st, GroupResults *ro = nullptr) {
+                           bool want_digests, vs_batch_stats &st, GroupResults *ro = nullptr,
+                           bool want_counts = true) {
   const DevCSR *push = nullptr, *pull = nullptr;
   VS_TRY(graph_csrs(g, p.direction, &push, &pull));
   const uint32_t V = g->V, nvb = (V + 63) / 64;
@@ -1408,6 +1465,7 @@ static vs_status run_group(vs_graph_s *g, const std::vector<const vs_query *> &q
   c.tiles = static_cast<const TileInfo *>(d_tiles.p);
   c.ctl = static_cast<BatchCtl *>(d_ctl.p);
   c.out_counts = static_cast<ull *>(d_counts.p);
+  c.want_rows = (want_counts || want_digests) ? 1u : 0u;
   c.out_digests = want_digests ? static_cast<ull *>(d_dig.p) : nullptr;
   c.mlp = (p.sched == VS_SCHED_LEARNED && g->sched) ? sched_params(g->sched) : nullptr;
   VS_TRY(ensure_pull_items(*const_cast<DevCSR *>(pull), V, g->tuning.pull_hubs));
@@ -1623,7 +1681,8 @@ static vs_status batch_impl(const char *fn, vs_graph_t g, const vs_query *querie
       held_ids.push_back(ids);
       ro = held.back().get();
     }
-    VS_TRY(run_group(g, qs, *p, path == VS_PATH_SHORTEST, out_c, out_d, want_d, st, ro));
+    VS_TRY(run_group(g, qs, *p, path == VS_PATH_SHORTEST, out_c, out_d, want_d, st, ro,
+                     row_counts != nullptr));
     uint64_t r = 0;
     for (uint32_t id : ids) {
       for (uint32_t i = 0; i < queries[id].S; ++i, ++r) {
@@ -1701,6 +1760,7 @@ static vs_status autotune_batch(vs_graph_t g, const vs_query *queries, uint32_t 
   if (!cur.push_alpha) cur.push_alpha = 16;
   if (!cur.sparse_pct) cur.sparse_pct = kSparsePct;
   if (!cur.pull_hubs) cur.pull_hubs = kPullHubs;
+  if (!cur.wpush_pml) cur.wpush_pml = kWPushPml;
   auto measure = [&](const vs_tuning &t, double *ms) -> vs_status {
     g->tuning = t;
     vs_batch_stats st{};
@@ -1739,6 +1799,8 @@ static vs_status autotune_batch(vs_graph_t g, const vs_query *queries, uint32_t 
   if (s == VS_OK) s = try_knob(&vs_tuning::push_alpha, {4u, 16u, 64u});
   if (s == VS_OK) s = try_knob(&vs_tuning::sparse_pct, {35u, 50u, 65u});
   if (s == VS_OK) s = try_knob(&vs_tuning::pull_hubs, hubs);
+  // word push budget: default, larger (two or three sparse words), off
+  if (s == VS_OK) s = try_knob(&vs_tuning::wpush_pml, {kWPushPml, 200u, 2000u});
   if (s != VS_OK) {
     g->tuning = saved;
     return s;
