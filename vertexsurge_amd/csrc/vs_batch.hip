@@ -139,48 +139,76 @@ struct BatchConst {
 
 constexpr uint32_t kNoRow = 0xFFFFFFFFu;  // src of a padding row (no source)
 
+// The slot table is read at a wave-uniform slot index, but the compiler
+// cannot tell (the index comes from a search) and would keep every field in
+// vector registers; readfirstlane moves them to scalar registers.
+__device__ __forceinline__ uint32_t rfl32(uint32_t x) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(x)));
+}
+__device__ __forceinline__ uint64_t rfl64(uint64_t x) {
+  return static_cast<uint64_t>(rfl32(static_cast<uint32_t>(x))) |
+         (static_cast<uint64_t>(rfl32(static_cast<uint32_t>(x >> 32))) << 32);
+}
+
+// A pointer field of the slot table, read as a wave-uniform global-memory
+// pointer: every slot buffer is a device allocation, and saying so lets the
+// compiler use global (not flat) loads, stores and atomics through it.
+template <class T>
+__device__ __forceinline__ T *gp(T *const &field) {
+  using G = __attribute__((address_space(1))) T *;
+  const G g = *reinterpret_cast<const G *>(reinterpret_cast<const void *>(&field));
+  return (T *)(G)rfl64((uint64_t)g);
+}
+// The same for a field read at a per-thread index (no readfirstlane).
+template <class T>
+__device__ __forceinline__ T *gpn(T *const &field) {
+  using G = __attribute__((address_space(1))) T *;
+  return (T *)*reinterpret_cast<const G *>(reinterpret_cast<const void *>(&field));
+}
+
 __device__ __forceinline__ HopArgs slot_args(const BatchConst &c, const SlotDesc &sd,
                                              uint32_t hop) {
   HopArgs a;
-  const TileInfo ti = c.tiles[sd.tile];
+  const uint32_t tile = rfl32(sd.tile);
+  const uint32_t word0 = rfl32(c.tiles[tile].word0), hmax = rfl32(c.tiles[tile].hmax);
   a.V = c.V;
   a.S = c.S;
-  a.word0 = ti.word0;
-  a.dense = sd.dense;
+  a.word0 = word0;
+  a.dense = rfl32(sd.dense);
   a.poff = c.poff;
   a.padj = c.padj;
   a.xoff = c.xoff;
   a.xadj = c.xadj;
-  a.F = sd.F[(hop - 1) & 1];
-  a.Fm = sd.masked ? sd.Fm[(hop - 1) & 1] : nullptr;
-  a.actbm = sd.bm[(hop - 1) & 1];
+  a.F = gp(sd.F[(hop - 1) & 1]);
+  a.Fm = rfl32(sd.masked) ? gp(sd.Fm[(hop - 1) & 1]) : nullptr;
+  a.actbm = gp(sd.bm[(hop - 1) & 1]);
   // the tile's last hop: its frontier is never read, so it is not written
   // (and needs no clearing)
-  const bool last = hop >= ti.hmax;
-  a.Fn = last ? nullptr : sd.F[hop & 1];
-  a.Fnm = last ? nullptr : sd.Fm[hop & 1];
-  a.actbm_n = last ? nullptr : sd.bm[hop & 1];
-  a.N = sd.N;
-  a.touched = sd.touched;
-  a.Vis = sd.Vis;
-  a.R = c.use_r ? sd.R : nullptr;
-  a.items = sd.items;
-  a.nitems = &sd.ctr[hop - 1].items;
+  const bool last = hop >= hmax;
+  a.Fn = last ? nullptr : gp(sd.F[hop & 1]);
+  a.Fnm = last ? nullptr : gp(sd.Fm[hop & 1]);
+  a.actbm_n = last ? nullptr : gp(sd.bm[hop & 1]);
+  a.N = gp(sd.N);
+  a.touched = gp(sd.touched);
+  a.Vis = gp(sd.Vis);
+  a.R = c.use_r ? gp(sd.R) : nullptr;
+  a.items = gp(sd.items);
+  a.nitems = &gp(sd.ctr)[hop - 1].items;
   const size_t part = static_cast<size_t>(hop) * kParts + blockIdx.x % kParts;
-  a.ctr = reinterpret_cast<HopCounters *>(sd.part + part * kPartStride);
-  a.live = c.live + static_cast<size_t>(hop) * c.Wpad + ti.word0;
-  a.live_next = c.live + static_cast<size_t>(hop + 1) * c.Wpad + ti.word0;
-  a.racc = c.racc ? c.racc + static_cast<size_t>(hop) * c.Wpad + ti.word0 : nullptr;
+  a.ctr = reinterpret_cast<HopCounters *>(gp(sd.part) + part * kPartStride);
+  a.live = c.live + static_cast<size_t>(hop) * c.Wpad + word0;
+  a.live_next = c.live + static_cast<size_t>(hop + 1) * c.Wpad + word0;
+  a.racc = c.racc ? c.racc + static_cast<size_t>(hop) * c.Wpad + word0 : nullptr;
   a.pitems = c.pitems;
   a.hubcnt = c.hubcnt;
   a.npitems = c.npitems;
   a.ewords = nullptr;
   a.newords = 0;
-  a.front = sd.front + static_cast<size_t>(hop - 1) * 64;
-  a.front_next = sd.fpart + part * 64;
-  a.wdeg = sd.wpart + static_cast<size_t>(hop) * kWParts * 64;
-  a.pw = sd.mode == 2 ? sd.pw : 0;
-  a.Fwm = sd.Fm[(hop - 1) & 1];
+  a.front = gp(sd.front) + static_cast<size_t>(hop - 1) * 64;
+  a.front_next = gp(sd.fpart) + part * 64;
+  a.wdeg = gp(sd.wpart) + static_cast<size_t>(hop) * kWParts * 64;
+  a.pw = rfl32(sd.mode) == 2 ? rfl64(sd.pw) : 0;
+  a.Fwm = gp(sd.Fm[(hop - 1) & 1]);
   return a;
 }
 
@@ -281,7 +309,7 @@ __global__ __launch_bounds__(kBlock) void k_bw_zero(BatchConst c) {
     const SlotDesc &sd = c.slots[s];
     if (sd.tile == UINT32_MAX) continue;
     for (int m = 0; m < 2; ++m) {
-      uint64_t *M = m == 0 ? sd.Vis : (c.use_r ? sd.R : nullptr);
+      uint64_t *M = m == 0 ? gpn(sd.Vis) : (c.use_r ? gpn(sd.R) : nullptr);
       if (!M) continue;
       // 16-byte stores (the matrices are hipMalloc'd, 256-byte aligned)
       uint4 *M4 = reinterpret_cast<uint4 *>(M);
@@ -312,22 +340,22 @@ __global__ void k_bw_l0(BatchConst c) {
     const uint64_t w_live1 = c.live[static_cast<size_t>(1) * c.Wpad + ti.word0 + j];
     const uint64_t deg = c.xoff[v + 1] - c.xoff[v];
     HopCounters *h = &sd.ctr[0];
-    const ull oldF = atomicOr(reinterpret_cast<ull *>(&sd.F[0][wi]), bit);
+    const ull oldF = atomicOr(reinterpret_cast<ull *>(&gpn(sd.F[0])[wi]), bit);
     if (oldF == 0) {
       atomicAdd(&h->nz, 1ull);
       atomicAdd(&h->nzdeg, static_cast<ull>(deg));
       atomicAdd(&sd.wdeg[j], static_cast<ull>(deg));
     }
-    atomicOr(reinterpret_cast<ull *>(&sd.Fm[0][v]), 1ull << j);
+    atomicOr(reinterpret_cast<ull *>(&gpn(sd.Fm[0])[v]), 1ull << j);
     if (sd.Vis) {
-      const ull oldV = atomicOr(reinterpret_cast<ull *>(&sd.Vis[wi]), bit);
+      const ull oldV = atomicOr(reinterpret_cast<ull *>(&gpn(sd.Vis)[wi]), bit);
       if (oldV == 0) atomicAdd(&h->visnew, 1ull);
     }
     if (c.use_r && c.racc && (c.racc[ti.word0 + j] & bit)) {
-      const ull oldR = atomicOr(reinterpret_cast<ull *>(&sd.R[wi]), bit);
+      const ull oldR = atomicOr(reinterpret_cast<ull *>(&gpn(sd.R)[wi]), bit);
       if (oldR == 0) atomicAdd(&h->rnew, 1ull);
     }
-    const ull oldB = atomicOr(reinterpret_cast<ull *>(&sd.bm[0][v / 64]), 1ull << (v % 64));
+    const ull oldB = atomicOr(reinterpret_cast<ull *>(&gpn(sd.bm[0])[v / 64]), 1ull << (v % 64));
     if (!(oldB & (1ull << (v % 64)))) {
       atomicAdd(&h->act, 1ull);
       atomicAdd(&h->actdeg, static_cast<ull>(deg));
@@ -426,8 +454,8 @@ __global__ __launch_bounds__(kBlock) void k_bw_items(BatchConst c, uint32_t hop)
     // (only the vertices holding bits in a pushed word)
     const bool wp = sd.mode == 2 && sd.pw;
     if (sd.mode != 1 && !wp) continue;
-    items_slice(sd.bm[(hop - 1) & 1], nvb, (u % nslice) * kBlock, c.xoff, sd.items,
-                &sd.ctr[hop - 1].items, s_wsum, &s_base, wp ? sd.Fm[(hop - 1) & 1] : nullptr,
+    items_slice(gpn(sd.bm[(hop - 1) & 1]), nvb, (u % nslice) * kBlock, c.xoff, gpn(sd.items),
+                &gpn(sd.ctr)[hop - 1].items, s_wsum, &s_base, wp ? gpn(sd.Fm[(hop - 1) & 1]) : nullptr,
                 wp ? sd.pw : 0);
   }
 }
@@ -456,58 +484,6 @@ __global__ void k_bw_plan(BatchConst c, uint32_t hop) {
 // slots; work unit u belongs to the slot whose prefix range holds it.  Two
 // launches per hop: SPARSE takes the pull slots whose frontier rows are read
 // through their word masks, the other one the push and dense pull slots.
-__device__ __forceinline__ uint32_t rfl32(uint32_t x) {
-  return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(x)));
-}
-__device__ __forceinline__ uint64_t rfl64(uint64_t x) {
-  return static_cast<uint64_t>(rfl32(static_cast<uint32_t>(x))) |
-         (static_cast<uint64_t>(rfl32(static_cast<uint32_t>(x >> 32))) << 32);
-}
-template <class T>
-__device__ __forceinline__ T *rflp(T *p) {
-  return reinterpret_cast<T *>(rfl64(reinterpret_cast<uint64_t>(p)));
-}
-
-// A slot's arguments are the same in every lane; saying so keeps them in
-// scalar registers (the compiler cannot tell on its own: they are loaded
-// through a slot index found by a search).
-__device__ __forceinline__ void uniform_args(HopArgs &a) {
-  a.V = rfl32(a.V);
-  a.S = rfl32(a.S);
-  a.word0 = rfl32(a.word0);
-  a.dense = rfl32(a.dense);
-  a.poff = rflp(a.poff);
-  a.padj = rflp(a.padj);
-  a.xoff = rflp(a.xoff);
-  a.xadj = rflp(a.xadj);
-  a.F = rflp(a.F);
-  a.actbm = rflp(a.actbm);
-  a.Fm = rflp(a.Fm);
-  a.Fn = rflp(a.Fn);
-  a.Fnm = rflp(a.Fnm);
-  a.actbm_n = rflp(a.actbm_n);
-  a.N = rflp(a.N);
-  a.touched = rflp(a.touched);
-  a.Vis = rflp(a.Vis);
-  a.R = rflp(a.R);
-  a.items = rflp(a.items);
-  a.nitems = rflp(a.nitems);
-  a.pitems = rflp(a.pitems);
-  a.npitems = rfl64(a.npitems);
-  a.ewords = rflp(a.ewords);
-  a.newords = rfl32(a.newords);
-  a.hubcnt = rflp(a.hubcnt);
-  a.ctr = rflp(a.ctr);
-  a.live = rflp(a.live);
-  a.live_next = rflp(a.live_next);
-  a.racc = rflp(a.racc);
-  a.front = rflp(a.front);
-  a.front_next = rflp(a.front_next);
-  a.wdeg = rflp(a.wdeg);
-  a.pw = rfl64(a.pw);
-  a.Fwm = rflp(a.Fwm);
-}
-
 template <int TW, bool SHORTEST, bool SPARSE>
 __global__ __launch_bounds__(kBlock)
 void k_bw_expand(BatchConst c, uint32_t hop) {
@@ -520,51 +496,34 @@ void k_bw_expand(BatchConst c, uint32_t hop) {
   const int wid = wave_id();
   const ull total = prefix[c.nslots];
   const ull nwaves = static_cast<ull>(gridDim.x) * kWavesPerBlock;
-  uint32_t cur = UINT32_MAX;
-  HopArgs a;
-  LaneMasks mk;
-  int mode = 0;
-  LocalCtr lc;
   const int j = lane_id() % TW;
-  PullHead pre{};
-  ull pre_u = ~0ull;  // unit whose head is prefetched in `pre`
-  // the current slot's unit range [sb, se): u only grows and the prefixes are
-  // fixed during the launch, so the search (dependent loads of the prefix
-  // table) runs only when u leaves the range
-  ull sb = 0, se = 0;
-  for (ull u = blockIdx.x * kWavesPerBlock + wid; u < total; u += nwaves) {
-    uint32_t s = cur;
-    if (cur == UINT32_MAX || u < sb || u >= se) {
-      s = cur == UINT32_MAX ? 0 : cur;
-      while (s + 1 <= c.nslots && prefix[s + 1] <= u) ++s;
-      while (prefix[s] > u) --s;
-      sb = prefix[s];
-      se = prefix[s + 1];
-    }
-    if (s != cur) {
-      if (cur != UINT32_MAX && mode == 2) flush_ctr(lc, a.ctr, a.front_next, a.wdeg, j);
-      cur = s;
-      a = slot_args(c, c.slots[s], hop);
-      uniform_args(a);
-      mode = __builtin_amdgcn_readfirstlane(c.slots[s].mode);
-      mk = lane_masks(a, j);
-    }
-    const ull slot_end = se;
-    const ull local = u - sb;
+  // Slots in order: a slot's units are a contiguous range [sb, se) of the
+  // unit space and this wave's units (stride nwaves) only grow, so each slot
+  // is visited once, with its arguments fixed for the inner loop.
+  ull u = blockIdx.x * kWavesPerBlock + wid;
+  for (uint32_t s = 0; s < c.nslots && u < total; ++s) {
+    const ull sb = prefix[s], se = prefix[s + 1];
+    if (u >= se) continue;
+    const HopArgs a = slot_args(c, c.slots[s], hop);
+    const int mode = __builtin_amdgcn_readfirstlane(c.slots[s].mode);
     if (mode == 2) {
-      const PullHead h = pre_u == u ? pre : pull_head<TW, SHORTEST>(a, a.pitems[local], j);
-      const ull nu = u + nwaves;  // prefetch the next unit of the same slot
-      if (nu < slot_end) {
-        pre = pull_head<TW, SHORTEST>(a, a.pitems[nu - sb], j);
-        pre_u = nu;
+      const LaneMasks mk = lane_masks(a, j);
+      LocalCtr lc;
+      PullHead h = pull_head<TW, SHORTEST>(a, a.pitems[u - sb], j);
+      for (; u < se; u += nwaves) {
+        const ull nu = u + nwaves;  // the next unit's head, one item ahead
+        PullHead hn{};
+        if (nu < se) hn = pull_head<TW, SHORTEST>(a, a.pitems[nu - sb], j);
+        pull_body<TW, SHORTEST, SPARSE>(a, h, s_nbr[wid], s_nm[wid], s_pairs[wid], s_acc[wid],
+                                        s_wl[wid], mk, lc);
+        h = hn;
       }
-      pull_body<TW, SHORTEST, SPARSE>(a, h, s_nbr[wid], s_nm[wid], s_pairs[wid], s_acc[wid],
-                                      s_wl[wid], mk, lc);
-    } else if (!SPARSE) {
-      push_item<TW>(a, a.items[local]);
+      flush_ctr(lc, a.ctr, a.front_next, a.wdeg, j);
+    } else {
+      for (; u < se; u += nwaves)
+        if (!SPARSE) push_item<TW>(a, a.items[u - sb]);
     }
   }
-  if (cur != UINT32_MAX && mode == 2) flush_ctr(lc, a.ctr, a.front_next, a.wdeg, j);
 }
 
 // ---- word push of the pull slots that have one: persistent over their
@@ -630,7 +589,7 @@ __global__ __launch_bounds__(kBlock) void k_bw_clear(BatchConst c, uint32_t whic
   for (ull u = blockIdx.x * kWavesPerBlock + wid; u < total; u += nwaves) {
     const SlotDesc &sd = c.slots[u / nvb];
     if (sd.tile == UINT32_MAX) continue;
-    clear_block_masked<TW>(sd.F[which & 1], sd.Fm[which & 1], sd.bm[which & 1],
+    clear_block_masked<TW>(gpn(sd.F[which & 1]), gpn(sd.Fm[which & 1]), gpn(sd.bm[which & 1]),
                            static_cast<uint32_t>(u % nvb));
   }
 }
@@ -704,7 +663,7 @@ __global__ __launch_bounds__(kBlock) void k_bw_rows(BatchConst c) {
     }
     const uint32_t ch = static_cast<uint32_t>((q + wid) % nchp);
     if (ch >= nch) continue;
-    const uint64_t *M = c.use_r ? sd.R : sd.Vis;
+    const uint64_t *M = c.use_r ? gpn(sd.R) : gpn(sd.Vis);
     const uint32_t v0 = ch * CV, v1 = min(c.V, v0 + CV);
     uint64_t cnt[NB];
 #pragma unroll
@@ -763,7 +722,7 @@ __global__ __launch_bounds__(kBlock) void k_bw_rows(BatchConst c) {
     const TileInfo ti = c.tiles[sd.tile];
     const uint32_t jj = sj % TW;
     if (jj * 64 >= ti.rows) continue;
-    const uint64_t *M = (c.use_r ? sd.R : sd.Vis) + jj;
+    const uint64_t *M = (c.use_r ? gpn(sd.R) : gpn(sd.Vis)) + jj;
     uint64_t acc = 0;
     for (uint32_t vb = iv * BPW; vb < min(nvb, (iv + 1) * BPW); ++vb) {
       const uint32_t v = vb * 64 + lane;
@@ -877,7 +836,7 @@ __global__ void k_bw_subsrc(BatchConst c) {
     const uint32_t lr = i % rows_per_tile;
     if (lr >= ti.rows || !c.sub_src[ti.row0 + lr]) continue;
     const size_t wi = static_cast<size_t>(c.src[ti.row0 + lr]) * c.TW + lr / 64;
-    atomicAnd(reinterpret_cast<ull *>(&sd.Vis[wi]), ~(1ull << (lr % 64)));
+    atomicAnd(reinterpret_cast<ull *>(&gpn(sd.Vis)[wi]), ~(1ull << (lr % 64)));
   }
 }
 
@@ -895,8 +854,8 @@ __global__ __launch_bounds__(kBlock) void k_bw_export(BatchConst c) {
     const uint32_t v = static_cast<uint32_t>(k / TW), j = static_cast<uint32_t>(k % TW);
     const uint2 d = c.wdest[static_cast<size_t>(sd.tile) * TW + j];
     if (d.x == UINT32_MAX) continue;
-    const uint64_t *M = c.use_r ? sd.R : sd.Vis;
-    c.qbase[d.x][(static_cast<size_t>(d.y / TW) * c.V + v) * TW + d.y % TW] = M[k];
+    const uint64_t *M = c.use_r ? gpn(sd.R) : gpn(sd.Vis);
+    gpn(c.qbase[d.x])[(static_cast<size_t>(d.y / TW) * c.V + v) * TW + d.y % TW] = M[k];
   }
 }
 
