@@ -10,11 +10,12 @@
 // Here each dependency is a CSR in HBM with sorted lists, so the same set is
 // the intersection of arity sorted lists.  One wave owns one tuple: the
 // shortest list drives, 64 elements per step (one per lane); each lane looks
-// its element up in every other list by binary search, the searches of all
-// lists in lockstep (their loads are independent, so they overlap instead of
-// queueing one list after the other), each starting where the previous
-// step's largest element landed; ballot + mbcnt give the ascending output
-// positions.  Each search first bisects the list's fence -- every 64th entry
+// its element up in the other lists by binary search, shortest list first and
+// each further list only for the lanes still in every list so far (few
+// survive the first: a random element is in a list about as often as the
+// list is long relative to V), each search starting where the previous
+// step's largest searched element landed; ballot + mbcnt give the ascending
+// output positions.  Each search first bisects the list's fence -- every 64th entry
 // of the adjacency array, a copy 64 times smaller that stays cached for the
 // long (hub) lists -- and then the at most 64 entries between two fence
 // posts, one or two cache lines: a lookup in a list of a million entries
@@ -90,74 +91,77 @@ __device__ __forceinline__ bool tuple_lists(const MiArgs &a, uint64_t t, uint64_
   return ok;
 }
 
-// Per lane: is x in every list but the driver's?  lo[d] starts at from[d]
-// (positions relative to each list's start; everything before it is < x)
-// and ends at x's lower bound.  Two phases, all lists in lockstep: the
-// fence (every 64th adjacency entry), then at most 64 entries between two
-// fence posts.
+// lower_bound of x in list d (positions relative to the list's start, the
+// search starting at `from`: everything before it is < x) for the lanes with
+// `on`.  Two phases: the fence (every 64th adjacency entry), then at most 64
+// entries between two fence posts.
+__device__ __forceinline__ uint32_t search_list(const MiArgs &a, int d, uint64_t b, uint32_t n,
+                                                uint32_t from, uint32_t x, bool on) {
+  uint32_t lo = from, hi = on ? n : 0;
+  // fence posts of the list sit at the multiples of 64 in [b + lo, b + hi),
+  // fence indices fa + [0, fhi)
+  const uint64_t fa = (b + lo + 63) >> 6;
+  const uint32_t fnr = on ? static_cast<uint32_t>(((b + hi + 63) >> 6) - fa) : 0;
+  uint32_t flo = 0, fhi = fnr;
+  const uint32_t *fence = a.fence[d];
+  while (flo < fhi) {  // first fence post >= x
+    const uint32_t mid = (flo + fhi) >> 1;
+    if (fence[fa + mid] < x) flo = mid + 1;
+    else fhi = mid;
+  }
+  if (on && fnr > 0) {
+    // post flo-1 is < x: the answer lies after it
+    if (flo > 0) lo = max(lo, static_cast<uint32_t>(64 * (fa + flo - 1) + 1 - b));
+    // post flo is >= x: the answer is at or before it
+    if (flo < fnr) hi = static_cast<uint32_t>(64 * (fa + flo) - b);
+  }
+  const uint32_t *adj = a.adj[d] + b;
+  while (lo < hi) {  // lower_bound within at most 64 entries
+    const uint32_t mid = (lo + hi) >> 1;
+    if (adj[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Per lane: is x in every list but the driver's?  The other lists are
+// searched one after the other, shortest first, each only for the lanes
+// whose x is in every list searched so far: a random x is in a list about
+// as often as the list is long relative to V, so the shortest list keeps
+// few lanes (on the 3-path workload 5 % of the driver elements), and the
+// longer lists are searched for those only.  lo[d] returns x's lower bound
+// in list d (from[d] for a lane that did not search it); searched[d] is the
+// ballot of the lanes that searched list d.
 template <int N>
 __device__ __forceinline__ bool member_all(const MiArgs &a, const uint64_t (&b)[N],
                                            const uint32_t (&n)[N], int drv,
                                            const uint32_t (&from)[N], uint32_t x, bool have,
-                                           uint32_t (&lo)[N]) {
-  // fence posts of list d sit at the multiples of 64 in [b + lo, b + hi),
-  // fence indices fa + [0, fhi)
-  uint32_t hi[N], flo[N], fhi[N];
-  uint64_t fa[N];
+                                           uint32_t (&lo)[N], uint64_t (&searched)[N]) {
+  // rank of each list by length (ties by index); the driver is skipped
+  int rank[N];
 #pragma unroll
   for (int d = 0; d < N; ++d) {
-    const bool on = have && d != drv;
-    lo[d] = on ? from[d] : 0;
-    hi[d] = on ? n[d] : 0;
-    fa[d] = (b[d] + lo[d] + 63) >> 6;
-    flo[d] = 0;
-    fhi[d] = on ? static_cast<uint32_t>(((b[d] + hi[d] + 63) >> 6) - fa[d]) : 0;
-  }
-  // phase 1: first fence post >= x
-  bool more = have;
-  while (more) {
-    more = false;
+    rank[d] = 0;
 #pragma unroll
-    for (int d = 0; d < N; ++d) {
-      if (flo[d] < fhi[d]) {
-        const uint32_t mid = (flo[d] + fhi[d]) >> 1;
-        if (a.fence[d][fa[d] + mid] < x) flo[d] = mid + 1;
-        else fhi[d] = mid;
-        more |= flo[d] < fhi[d];
-      }
-    }
-  }
-#pragma unroll
-  for (int d = 0; d < N; ++d) {
-    const bool on = have && d != drv;
-    const uint32_t fnr = on ? static_cast<uint32_t>(((b[d] + hi[d] + 63) >> 6) - fa[d]) : 0;
-    if (on && fnr > 0) {
-      const uint32_t f = flo[d];
-      // post f-1 is < x: the answer lies after it
-      if (f > 0) lo[d] = max(lo[d], static_cast<uint32_t>(64 * (fa[d] + f - 1) + 1 - b[d]));
-      // post f is >= x: the answer is at or before it
-      if (f < fnr) hi[d] = static_cast<uint32_t>(64 * (fa[d] + f) - b[d]);
-    }
-  }
-  // phase 2: lower_bound within at most 64 entries
-  more = have;
-  while (more) {
-    more = false;
-#pragma unroll
-    for (int d = 0; d < N; ++d) {
-      if (lo[d] < hi[d]) {
-        const uint32_t mid = (lo[d] + hi[d]) >> 1;
-        if (a.adj[d][b[d] + mid] < x) lo[d] = mid + 1;
-        else hi[d] = mid;
-        more |= lo[d] < hi[d];
-      }
-    }
+    for (int e = 0; e < N; ++e) rank[d] += (n[e] < n[d]) || (n[e] == n[d] && e < d);
   }
   bool keep = have;
 #pragma unroll
   for (int d = 0; d < N; ++d) {
-    if (d == drv) continue;
-    keep = keep && lo[d] < n[d] && a.adj[d][b[d] + lo[d]] == x;
+    lo[d] = from[d];
+    searched[d] = 0;
+  }
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+#pragma unroll
+    for (int d = 0; d < N; ++d) {
+      if (rank[d] != r || d == drv) continue;
+      const uint64_t on = __ballot(keep);
+      searched[d] = on;
+      if (on == 0) continue;
+      lo[d] = search_list(a, d, b[d], n[d], from[d], x, keep);
+      keep = keep && lo[d] < n[d] && a.adj[d][b[d] + lo[d]] == x;
+    }
   }
   return keep;
 }
@@ -193,10 +197,13 @@ __global__ __launch_bounds__(kBlock) void k_mintersect(MiArgs a) {
         const bool have = idx < b[drv] + best;
         const uint32_t x = have ? dl[idx] : 0xFFFFFFFFu;
         uint32_t lo[N];
-        const bool keep = member_all<N>(a, b, n, drv, from, x, have, lo);
-        // the next step's elements are all larger than lane 63's
+        uint64_t searched[N];
+        const bool keep = member_all<N>(a, b, n, drv, from, x, have, lo, searched);
+        // the next step's elements are all larger than this step's: a list
+        // resumes where its highest searching lane landed
 #pragma unroll
-        for (int d = 0; d < N; ++d) from[d] = __shfl(lo[d], 63);
+        for (int d = 0; d < N; ++d)
+          if (searched[d]) from[d] = __shfl(lo[d], 63 - __clzll(searched[d]));
         const uint64_t m = __ballot(keep);
         if (WRITE && keep) a.lists[out + lane_rank(m)] = x;
         out += __popcll(m);
@@ -229,9 +236,10 @@ __global__ __launch_bounds__(kBlock) void k_mintersect_short(MiArgs a) {
     const bool have = ok && static_cast<uint64_t>(gl) < best;
     const uint32_t x = have ? a.adj[drv][b[drv] + gl] : 0xFFFFFFFFu;
     uint32_t from[N], lo[N];
+    uint64_t searched[N];
 #pragma unroll
     for (int d = 0; d < N; ++d) from[d] = 0;
-    const bool keep = member_all<N>(a, b, n, drv, from, x, have, lo);
+    const bool keep = member_all<N>(a, b, n, drv, from, x, have, lo, searched);
     const uint64_t m = __ballot(keep);
     const uint64_t gm = (m >> (g * GS)) & ((1ull << GS) - 1);
     if (WRITE && keep)
@@ -310,6 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_mi_split(MiArgs a, uint32_t *lists, 
     for (int r = 0; r < kSplitRounds; ++r) {
       const uint32_t t = static_cast<uint32_t>(c0 + static_cast<uint64_t>(r) * kBlock +
                                                threadIdx.x);  // T < 2^32 - 1
+      if (kk[r] < 0) continue;  // past T
 #pragma unroll
       for (int q = 0; q < kBuckets; ++q)
         if (kk[r] == q) lists[q * a.T + pos[q]++] = t;
