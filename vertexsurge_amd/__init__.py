@@ -414,14 +414,27 @@ def mintersect(deps: list, tuples, with_lists: bool = True):
     offsets = np.zeros(T + 1, dtype=np.uint64)
     total = C.c_uint64()
     st = _lib.MintersectStats()
-    _lib.call("vs_mintersect", arr, arity, _ptr(tup, C.c_uint32), T, _ptr(counts, C.c_uint32),
-              _ptr(offsets, C.c_uint64), None, 0, C.byref(total), C.byref(st))
     lists = None
-    if with_lists:
-        lists = np.zeros(max(total.value, 1), dtype=np.uint32)
+    if not with_lists:
         _lib.call("vs_mintersect", arr, arity, _ptr(tup, C.c_uint32), T,
-                  _ptr(counts, C.c_uint32), _ptr(offsets, C.c_uint64),
-                  _ptr(lists, C.c_uint32), len(lists), C.byref(total), C.byref(st))
+                  _ptr(counts, C.c_uint32), _ptr(offsets, C.c_uint64), None, 0,
+                  C.byref(total), C.byref(st))
+    else:
+        # one intersection pass when the lists fit a first guess of one entry per
+        # tuple; a capacity error still returns counts, offsets and the total,
+        # and the call runs again with room for exactly that
+        cap = max(T, 1)
+        while True:
+            lists = np.zeros(cap, dtype=np.uint32)
+            try:
+                _lib.call("vs_mintersect", arr, arity, _ptr(tup, C.c_uint32), T,
+                          _ptr(counts, C.c_uint32), _ptr(offsets, C.c_uint64),
+                          _ptr(lists, C.c_uint32), cap, C.byref(total), C.byref(st))
+                break
+            except VSError as e:
+                if e.code != _lib.VS_ERR_CAPACITY or total.value <= cap:
+                    raise
+                cap = total.value
         lists = lists[:total.value]
     return counts, offsets, lists, {"ms": st.ms, "total": st.total, "bytes_alg": st.bytes_alg}
 
