@@ -238,3 +238,40 @@ def test_word_push_layers_and_find_shortest(orc, rmat16):
             assert dist[r] == d[tgt[r]], r
     finally:
         G.set_tuning(**saved)
+
+
+def test_word_push_directed_and_multi_tile(orc, rmat16):
+    """Word push where the push CSR is not the pull CSR (a directed graph, all
+    three directions) and over several tiles of one call (3000 rows at tile
+    width 16), with a budget that pushes most words."""
+    rng = np.random.default_rng(21)
+    V = 3000
+    e = np.unique(rng.integers(0, V, size=(60000, 2)).astype(np.uint64) @
+                  np.array([V, 1], dtype=np.uint64))
+    e = e[e // V != e % V]
+    h = vs.HostCSR.from_edges(V, (e // V).astype(np.uint32), (e % V).astype(np.uint32),
+                              symmetrize=False)
+    off, adj = h.offsets.copy(), h.adj.copy()
+    G = vs.Graph(off, adj, symmetric=False)
+    roff, radj = reverse_csr(off, adj)
+    uoff, uadj = union_csr(off, adj, roff, radj)
+    src = vs.sample_sources(V, 700, seed=4)
+    G.set_tuning(wpush_pml=1000)
+    for direction, (o, a) in {"forward": (off, adj), "backward": (roff, radj),
+                              "both": (uoff, uadj)}.items():
+        res, st = vs.vexpand(G, src, 1, 5, "shortest", direction=direction, sched="pull")
+        mats, ka, te, _ = orc.vexpand_rows(o, a, src, 1, 5, "shortest")
+        assert np.array_equal(res.export(0, vs.LAYOUT_ROWMAJOR), mats[0]), direction
+        assert st["te"] == te and res.k_actual_max == ka
+    g, G16 = rmat16
+    saved = G16.get_tuning()
+    try:
+        G16.set_tuning(**{**saved, "wpush_pml": 1000})
+        src = vs.sample_sources(g.V, 3000, seed=12)
+        for path in ("shortest", "any"):
+            res, st, _ = check(orc, G16, g.offsets, g.adj, src, 1, 4, path, sched="pull",
+                               tile_words=16)
+            assert st["num_tiles"] == 3
+            res.close()
+    finally:
+        G16.set_tuning(**saved)
