@@ -509,14 +509,25 @@ void k_bw_expand(BatchConst c, uint32_t hop) {
     if (mode == 2) {
       const LaneMasks mk = lane_masks(a, j);
       LocalCtr lc;
+      // three items in flight: the head (visited word, list bounds) two
+      // items ahead, the first list ids one item ahead, the body of this one
       PullHead h = pull_head<TW, SHORTEST>(a, a.pitems[u - sb], j);
+      PullHead hn{};
+      if (u + nwaves < se) hn = pull_head<TW, SHORTEST>(a, a.pitems[u + nwaves - sb], j);
+      uint32_t u0 = 0;
+      bool p0 = pull_prefetch<TW, SPARSE>(a, h, mk, j, u0);
       for (; u < se; u += nwaves) {
-        const ull nu = u + nwaves;  // the next unit's head, one item ahead
-        PullHead hn{};
-        if (nu < se) hn = pull_head<TW, SHORTEST>(a, a.pitems[nu - sb], j);
+        const ull n2 = u + 2 * nwaves;
+        PullHead h2{};
+        if (n2 < se) h2 = pull_head<TW, SHORTEST>(a, a.pitems[n2 - sb], j);
+        uint32_t u1 = 0;
+        const bool p1 = u + nwaves < se && pull_prefetch<TW, SPARSE>(a, hn, mk, j, u1);
         pull_body<TW, SHORTEST, SPARSE>(a, h, s_nbr[wid], s_nm[wid], s_pairs[wid], s_acc[wid],
-                                        s_wl[wid], mk, lc);
+                                        s_wl[wid], mk, lc, u0, p0);
         h = hn;
+        hn = h2;
+        u0 = u1;
+        p0 = p1;
       }
       flush_ctr(lc, a.ctr, a.front_next, a.wdeg, j);
     } else {

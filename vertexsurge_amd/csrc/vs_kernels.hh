@@ -239,6 +239,22 @@ __device__ __forceinline__ PullHead pull_head(const HopArgs &a, ull item, int j)
   return h;
 }
 
+// The first 64 list ids of a pull item whose first group will be scanned:
+// loaded one item ahead so that the scan does not start with a dependent
+// load.  Returns whether they were loaded (wave-uniform); the item's body
+// then takes them instead of loading its first group itself.  Later chunks
+// of a chunked vertex are not prefetched (their needed rows depend on N).
+template <int TW, bool SPARSE>
+__device__ __forceinline__ bool pull_prefetch(const HopArgs &a, const PullHead &h,
+                                              const LaneMasks &mk, int j, uint32_t &u0) {
+  const bool pwl = !SPARSE && ((a.pw >> j) & 1);
+  const bool want = h.ch == 0 && !pwl && (mk.live & mk.front & ~h.vis) != 0;
+  if (__ballot(want) == 0) return false;
+  const uint64_t idx = h.lb + lane_id();
+  if (idx < h.le) u0 = a.padj[idx];
+  return true;
+}
+
 // Frontier word of neighbour u: hub rows stay cached, other rows stream.
 template <int TW, bool CACHED>
 __device__ __forceinline__ uint64_t gather(const HopArgs &a, uint32_t u, int j) {
@@ -389,7 +405,8 @@ __device__ __forceinline__ void gather_cross(const HopArgs &a, const uint32_t *n
 template <int TW, bool SHORTEST, bool SPARSE>
 __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, uint32_t *nbr,
                                           uint64_t *nm, uint16_t *pairs, uint64_t *accw,
-                                          uint8_t *wl, const LaneMasks &mk, LocalCtr &c) {
+                                          uint8_t *wl, const LaneMasks &mk, LocalCtr &c,
+                                          uint32_t u0 = 0, bool pre = false) {
   constexpr int G = 64 / TW;
   const int lane = lane_id();
   const int g = lane / TW, j = lane % TW;
@@ -421,7 +438,8 @@ __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, u
     uint32_t u = 0;
     bool act = false;
     if (idx < e) {
-      u = a.padj[idx];
+      // the first group's ids may have been loaded while the previous item ran
+      u = pre && base == b ? u0 : a.padj[idx];
       // a dense frontier skips the bitmap lookup: inactive rows are zero
       act = dense || ((a.actbm[u >> 6] >> (u & 63)) & 1);
     }
