@@ -509,25 +509,52 @@ void k_bw_expand(BatchConst c, uint32_t hop) {
     if (mode == 2) {
       const LaneMasks mk = lane_masks(a, j);
       LocalCtr lc;
-      // three items in flight: the head (visited word, list bounds) two
-      // items ahead, the first list ids one item ahead, the body of this one
-      PullHead h = pull_head<TW, SHORTEST>(a, a.pitems[u - sb], j);
-      PullHead hn{};
-      if (u + nwaves < se) hn = pull_head<TW, SHORTEST>(a, a.pitems[u + nwaves - sb], j);
-      uint32_t u0 = 0;
-      bool p0 = pull_prefetch<TW, SPARSE>(a, h, mk, j, u0);
+      // Items in flight per wave: the head (visited word, list bounds) of a
+      // later item, the first list ids of the next one -- and at tile width
+      // 64 their active-bitmap words (word-mask frontiers and bitmap-tested
+      // dense ones) one item ahead, the ids two ahead, the head three ahead.
+      // Narrower tiles keep the shallower pipeline: their wider lane groups
+      // leave no registers for it at four waves per SIMD.
+      constexpr bool kDeep = TW == 64;
+      const bool bmp = kDeep && (SPARSE || a.dense == 0);
+      constexpr ull D = kDeep ? 3 : 2;  // items from a head load to its body
+      PullHead h0 = pull_head<TW, SHORTEST>(a, a.pitems[u - sb], j);
+      PullHead h1{}, h2{};
+      if (u + nwaves < se) h1 = pull_head<TW, SHORTEST>(a, a.pitems[u + nwaves - sb], j);
+      if (kDeep && u + 2 * nwaves < se)
+        h2 = pull_head<TW, SHORTEST>(a, a.pitems[u + 2 * nwaves - sb], j);
+      uint32_t u0 = 0, u1 = 0, bw0 = 0;
+      bool p0 = pull_prefetch<TW, SPARSE>(a, h0, mk, j, u0);
+      if (p0 && bmp) bw0 = pull_prefetch_bm(a, h0, u0);
+      bool p1 = kDeep && u + nwaves < se && pull_prefetch<TW, SPARSE>(a, h1, mk, j, u1);
       for (; u < se; u += nwaves) {
-        const ull n2 = u + 2 * nwaves;
-        PullHead h2{};
-        if (n2 < se) h2 = pull_head<TW, SHORTEST>(a, a.pitems[n2 - sb], j);
-        uint32_t u1 = 0;
-        const bool p1 = u + nwaves < se && pull_prefetch<TW, SPARSE>(a, hn, mk, j, u1);
-        pull_body<TW, SHORTEST, SPARSE>(a, h, s_nbr[wid], s_nm[wid], s_pairs[wid], s_acc[wid],
-                                        s_wl[wid], mk, lc, u0, p0);
-        h = hn;
-        hn = h2;
-        u0 = u1;
-        p0 = p1;
+        const ull nh = u + D * nwaves;
+        PullHead hl{};
+        if (nh < se) hl = pull_head<TW, SHORTEST>(a, a.pitems[nh - sb], j);
+        uint32_t un = 0, bw1 = 0;
+        bool pn;
+        if constexpr (kDeep) {
+          pn = u + 2 * nwaves < se && pull_prefetch<TW, SPARSE>(a, h2, mk, j, un);
+          if (p1 && bmp) bw1 = pull_prefetch_bm(a, h1, u1);
+        } else {
+          pn = u + nwaves < se && pull_prefetch<TW, SPARSE>(a, h1, mk, j, un);
+        }
+        pull_body<TW, SHORTEST, SPARSE>(a, h0, s_nbr[wid], s_nm[wid], s_pairs[wid], s_acc[wid],
+                                        s_wl[wid], mk, lc, u0, p0, bw0, p0 && bmp);
+        h0 = h1;
+        if constexpr (kDeep) {
+          h1 = h2;
+          h2 = hl;
+          u0 = u1;
+          u1 = un;
+          p0 = p1;
+          p1 = pn;
+          bw0 = bw1;
+        } else {
+          h1 = hl;
+          u0 = un;
+          p0 = pn;
+        }
       }
       flush_ctr(lc, a.ctr, a.front_next, a.wdeg, j);
     } else {

@@ -255,6 +255,14 @@ __device__ __forceinline__ bool pull_prefetch(const HopArgs &a, const PullHead &
   return true;
 }
 
+// The bitmap halves of the ids pull_prefetch loaded (ahead of the body,
+// like the ids; a dense frontier does not read the bitmap).
+__device__ __forceinline__ uint32_t pull_prefetch_bm(const HopArgs &a, const PullHead &h,
+                                                     uint32_t u0) {
+  const uint32_t *bm = reinterpret_cast<const uint32_t *>(a.actbm);
+  return h.lb + lane_id() < h.le ? bm[u0 >> 5] : 0u;
+}
+
 // Frontier word of neighbour u: hub rows stay cached, other rows stream.
 template <int TW, bool CACHED>
 __device__ __forceinline__ uint64_t gather(const HopArgs &a, uint32_t u, int j) {
@@ -406,7 +414,8 @@ template <int TW, bool SHORTEST, bool SPARSE>
 __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, uint32_t *nbr,
                                           uint64_t *nm, uint16_t *pairs, uint64_t *accw,
                                           uint8_t *wl, const LaneMasks &mk, LocalCtr &c,
-                                          uint32_t u0 = 0, bool pre = false) {
+                                          uint32_t u0 = 0, bool pre = false,
+                                          uint32_t bw0 = 0, bool prebm = false) {
   constexpr int G = 64 / TW;
   const int lane = lane_id();
   const int g = lane / TW, j = lane % TW;
@@ -439,9 +448,12 @@ __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, u
     bool act = false;
     if (idx < e) {
       // the first group's ids may have been loaded while the previous item ran
-      u = pre && base == b ? u0 : a.padj[idx];
-      // a dense frontier skips the bitmap lookup: inactive rows are zero
-      act = dense || ((a.actbm[u >> 6] >> (u & 63)) & 1);
+      const bool first = base == b;
+      u = pre && first ? u0 : a.padj[idx];
+      // a dense frontier skips the bitmap lookup: inactive rows are zero;
+      // the first group's bitmap words may also have been loaded ahead
+      act = dense || (prebm && first ? ((bw0 >> (u & 31)) & 1)
+                                     : ((a.actbm[u >> 6] >> (u & 63)) & 1));
     }
     const uint64_t m = __ballot(act);
     const uint32_t n = __popcll(m);
