@@ -321,49 +321,76 @@ __global__ __launch_bounds__(kBlock) void k_bw_zero(BatchConst c) {
 
 // ---- L0 of every active slot, with its counters
 __global__ void k_bw_l0(BatchConst c) {
-  const uint32_t nvb = (c.V + 63) / 64;
   const uint32_t rows_per_tile = c.TW * 64;
   const uint32_t total = c.nslots * rows_per_tile;
+  const int lane = lane_id();
+  // The loop is wave-uniform (total and the stride are multiples of 64), and
+  // a wave's 64 rows share one slot and one word j: the counters take one
+  // atomic per wave each, not one per row (thousands of same-address atomics
+  // otherwise serialise in L2).
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += gridDim.x * blockDim.x) {
     const uint32_t s = i / rows_per_tile, lr = i % rows_per_tile;
     const SlotDesc &sd = c.slots[s];
-    if (sd.tile == UINT32_MAX) continue;
+    if (sd.tile == UINT32_MAX) continue;  // uniform: the wave's slot
     const TileInfo ti = c.tiles[sd.tile];
-    if (lr >= ti.rows) continue;
-    const uint32_t row = ti.row0 + lr;
-    const uint32_t v = c.src[row];
-    if (v == kNoRow) continue;  // padding between queries' words
     const uint32_t j = lr / 64;
     const ull bit = 1ull << (lr % 64);
-    const size_t wi = static_cast<size_t>(v) * c.TW + j;
-    const uint64_t w_live1 = c.live[static_cast<size_t>(1) * c.Wpad + ti.word0 + j];
-    const uint64_t deg = c.xoff[v + 1] - c.xoff[v];
-    HopCounters *h = &sd.ctr[0];
-    const ull oldF = atomicOr(reinterpret_cast<ull *>(&gpn(sd.F[0])[wi]), bit);
-    if (oldF == 0) {
-      atomicAdd(&h->nz, 1ull);
-      atomicAdd(&h->nzdeg, static_cast<ull>(deg));
-      atomicAdd(&sd.wdeg[j], static_cast<ull>(deg));
+    const uint32_t v = lr < ti.rows ? c.src[ti.row0 + lr] : kNoRow;  // kNoRow: padding
+    ull nz = 0, nzdeg = 0, visnew = 0, rnew = 0, act = 0, actdeg = 0, pop = 0, te = 0;
+    uint64_t fb = 0;
+    if (v != kNoRow) {
+      const size_t wi = static_cast<size_t>(v) * c.TW + j;
+      const uint64_t w_live1 = c.live[static_cast<size_t>(1) * c.Wpad + ti.word0 + j];
+      const uint64_t deg = c.xoff[v + 1] - c.xoff[v];
+      const ull oldF = atomicOr(reinterpret_cast<ull *>(&gpn(sd.F[0])[wi]), bit);
+      if (oldF == 0) {
+        nz = 1;
+        nzdeg = deg;
+      }
+      atomicOr(reinterpret_cast<ull *>(&gpn(sd.Fm[0])[v]), 1ull << j);
+      if (sd.Vis) {
+        const ull oldV = atomicOr(reinterpret_cast<ull *>(&gpn(sd.Vis)[wi]), bit);
+        visnew = oldV == 0;
+      }
+      if (c.use_r && c.racc && (c.racc[ti.word0 + j] & bit)) {
+        const ull oldR = atomicOr(reinterpret_cast<ull *>(&gpn(sd.R)[wi]), bit);
+        rnew = oldR == 0;
+      }
+      const ull oldB =
+          atomicOr(reinterpret_cast<ull *>(&gpn(sd.bm[0])[v / 64]), 1ull << (v % 64));
+      if (!(oldB & (1ull << (v % 64)))) {
+        act = 1;
+        actdeg = deg;
+      }
+      pop = 1;
+      fb = bit;
+      if (w_live1 & bit) te = deg;
     }
-    atomicOr(reinterpret_cast<ull *>(&gpn(sd.Fm[0])[v]), 1ull << j);
-    if (sd.Vis) {
-      const ull oldV = atomicOr(reinterpret_cast<ull *>(&gpn(sd.Vis)[wi]), bit);
-      if (oldV == 0) atomicAdd(&h->visnew, 1ull);
+    nz = wave_sum(nz);
+    nzdeg = wave_sum(nzdeg);
+    visnew = wave_sum(visnew);
+    rnew = wave_sum(rnew);
+    act = wave_sum(act);
+    actdeg = wave_sum(actdeg);
+    pop = wave_sum(pop);
+    te = wave_sum(te);
+    fb = wave_or(fb);
+    if (lane == 0 && pop) {
+      HopCounters *h = &sd.ctr[0];
+      if (nz) atomicAdd(&h->nz, nz);
+      if (nzdeg) {
+        atomicAdd(&h->nzdeg, nzdeg);
+        atomicAdd(&sd.wdeg[j], nzdeg);
+      }
+      if (visnew) atomicAdd(&h->visnew, visnew);
+      if (rnew) atomicAdd(&h->rnew, rnew);
+      if (act) atomicAdd(&h->act, act);
+      if (actdeg) atomicAdd(&h->actdeg, actdeg);
+      atomicAdd(&h->pop, pop);
+      atomicOr(reinterpret_cast<ull *>(&sd.front[j]), static_cast<ull>(fb));
+      if (te) atomicAdd(&h->te, te);
     }
-    if (c.use_r && c.racc && (c.racc[ti.word0 + j] & bit)) {
-      const ull oldR = atomicOr(reinterpret_cast<ull *>(&gpn(sd.R)[wi]), bit);
-      if (oldR == 0) atomicAdd(&h->rnew, 1ull);
-    }
-    const ull oldB = atomicOr(reinterpret_cast<ull *>(&gpn(sd.bm[0])[v / 64]), 1ull << (v % 64));
-    if (!(oldB & (1ull << (v % 64)))) {
-      atomicAdd(&h->act, 1ull);
-      atomicAdd(&h->actdeg, static_cast<ull>(deg));
-    }
-    atomicAdd(&h->pop, 1ull);
-    atomicOr(reinterpret_cast<ull *>(&sd.front[j]), bit);
-    if (w_live1 & bit) atomicAdd(&h->te, static_cast<ull>(deg));
-    (void)nvb;
   }
 }
 
@@ -599,7 +626,11 @@ __global__ __launch_bounds__(kBlock) void k_bw_epi(BatchConst c, uint32_t hop) {
   HopArgs a;
   LaneMasks mk;
   LocalCtr lc;
-  for (ull u = blockIdx.x * kWavesPerBlock + wid; u < total; u += nwaves) {
+  const ull w = blockIdx.x * kWavesPerBlock + wid;
+  // units only grow along a wave's rounds, so slots are still visited in order
+  for (ull k = 0; k * nwaves < total; ++k) {
+    const ull u = skewed_unit(k, w, nwaves);
+    if (u >= total) continue;
     const uint32_t s = static_cast<uint32_t>(u / nvb);
     // push slots (every bitmap word), and pull slots whose chunked vertices
     // left partial words in N (only the words that hold such vertices)
@@ -624,7 +655,10 @@ __global__ __launch_bounds__(kBlock) void k_bw_clear(BatchConst c, uint32_t whic
   const uint32_t nvb = (c.V + 63) / 64;
   const ull total = static_cast<ull>(c.nslots) * nvb;
   const ull nwaves = static_cast<ull>(gridDim.x) * kWavesPerBlock;
-  for (ull u = blockIdx.x * kWavesPerBlock + wid; u < total; u += nwaves) {
+  const ull w = blockIdx.x * kWavesPerBlock + wid;
+  for (ull k = 0; k * nwaves < total; ++k) {
+    const ull u = skewed_unit(k, w, nwaves);
+    if (u >= total) continue;
     const SlotDesc &sd = c.slots[u / nvb];
     if (sd.tile == UINT32_MAX) continue;
     clear_block_masked<TW>(gpn(sd.F[which & 1]), gpn(sd.Fm[which & 1]), gpn(sd.bm[which & 1]),

@@ -115,6 +115,16 @@ struct HopArgs {
   const uint64_t *Fwm;
 };
 
+// Grid-stride loops over bitmap words (epilogues, clears): the unit that
+// wave w takes in round k (units [k * nwaves, (k + 1) * nwaves)), each round
+// rotated by k * kSkew waves.  R-MAT puts its densest words at indices that
+// agree in their low bits (multiples of large powers of two); a plain stride
+// hands all of them to one wave, which then runs ~50x the mean.
+constexpr uint64_t kSkew = 97;
+__device__ __forceinline__ uint64_t skewed_unit(uint64_t k, uint64_t w, uint64_t nwaves) {
+  return k * nwaves + (w + nwaves - (k * kSkew) % nwaves) % nwaves;
+}
+
 // Partial copies of the per-word push-degree sums.
 constexpr uint32_t kWParts = 16;
 
@@ -790,9 +800,12 @@ __global__ __launch_bounds__(kBlock) void k_push_epi(HopArgs a) {
   const LaneMasks mk = lane_masks(a, lane_id() % TW);
   const uint32_t n = a.ewords ? a.newords : (a.V + 63) / 64;
   const uint32_t nwaves = gridDim.x * kWavesPerBlock;
+  const uint32_t w = blockIdx.x * kWavesPerBlock + wid;
   LocalCtr c;
-  for (uint32_t i = blockIdx.x * kWavesPerBlock + wid; i < n; i += nwaves)
-    epi_block<TW, SHORTEST>(a, a.ewords ? a.ewords[i] : i, mk, c);
+  for (uint64_t k = 0; k * nwaves < n; ++k) {
+    const uint64_t i = skewed_unit(k, w, nwaves);
+    if (i < n) epi_block<TW, SHORTEST>(a, a.ewords ? a.ewords[i] : static_cast<uint32_t>(i), mk, c);
+  }
   flush_ctr(c, a.ctr, a.front_next, a.wdeg, lane_id() % TW);
 }
 
@@ -811,8 +824,11 @@ __global__ __launch_bounds__(kBlock) void k_clear(uint64_t *F, uint64_t *Fm, uin
   const int wid = wave_id();
   const uint32_t nvb = (V + 63) / 64;
   const uint32_t nwaves = gridDim.x * kWavesPerBlock;
-  for (uint32_t vb = blockIdx.x * kWavesPerBlock + wid; vb < nvb; vb += nwaves)
-    clear_block<TW>(F, Fm, actbm, vb);
+  const uint32_t w = blockIdx.x * kWavesPerBlock + wid;
+  for (uint64_t k = 0; k * nwaves < nvb; ++k) {
+    const uint64_t vb = skewed_unit(k, w, nwaves);
+    if (vb < nvb) clear_block<TW>(F, Fm, actbm, static_cast<uint32_t>(vb));
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_items(const uint64_t *actbm, uint32_t V,
