@@ -715,10 +715,22 @@ __device__ __forceinline__ void items_slice(const uint64_t *actbm, uint32_t nvb,
   const uint32_t wb = base + threadIdx.x;
   uint64_t m = wb < nvb ? actbm[wb] : 0;
   if (fm) {
+    // eight mask loads in flight per round (a dense bitmap word would
+    // otherwise cost 64 dependent round trips)
     uint64_t keep = 0;
-    for (uint64_t mm = m; mm; mm &= mm - 1) {
-      const int i = __ffsll(static_cast<long long>(mm)) - 1;
-      if (fm[wb * 64 + i] & pw) keep |= 1ull << i;
+    for (uint64_t mm = m; mm;) {
+      int ii[8];
+      uint64_t wm[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        ii[t] = mm ? __ffsll(static_cast<long long>(mm)) - 1 : -1;
+        mm &= mm - 1;
+      }
+#pragma unroll
+      for (int t = 0; t < 8; ++t) wm[t] = ii[t] >= 0 ? fm[wb * 64 + ii[t]] : 0;
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        if (wm[t] & pw) keep |= 1ull << ii[t];
     }
     m = keep;
   }
