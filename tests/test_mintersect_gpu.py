@@ -136,3 +136,46 @@ def test_device_capacity_error_keeps_counts(orc):
         for p in (d_t, d_c, d_o, d_l):
             lib.call("vs_dev_free", p)
         G.close()
+
+
+def giants_graph(seed=5):
+    """Three vertices whose lists are longer than the 8192 entries above which
+    a tuple's shortest list gets a wave of its own, among short lists."""
+    rng = np.random.default_rng(seed)
+    V = 40000
+    lists = []
+    for v in range(V):
+        if v < 3:
+            k = 9000 + 3000 * v
+            nb = rng.choice(np.arange(3, V), size=k, replace=False)
+        else:
+            nb = rng.choice(V, size=int(rng.integers(0, 12)), replace=False)
+        lists.append(np.sort(nb).astype(np.uint32))
+    off = np.zeros(V + 1, np.uint64)
+    off[1:] = np.cumsum([len(x) for x in lists])
+    return off, np.concatenate(lists)
+
+
+def giant_tuples(T, V, seed):
+    """min(T, 40) tuples over the three long lists, the rest random."""
+    rng = np.random.default_rng(seed)
+    ng = min(T, 40)
+    tup = np.concatenate([rng.integers(0, 3, size=(ng, 3)),
+                          rng.integers(0, V, size=(T - ng, 3))]).astype(np.uint32)
+    rng.shuffle(tup)
+    return tup
+
+
+# T = 20: as many giants as waves (they are strided); T = 30: one wave
+# each with waves to spare; T = 5000: giants among short and long tuples
+@pytest.mark.parametrize("T", [20, 30, 5000])
+def test_giant_tuples(orc, T):
+    off, adj = giants_graph()
+    G = vs.Graph(off, adj, symmetric=False)
+    tup = giant_tuples(T, len(off) - 1, seed=T)
+    counts, offsets, lists, st = vs.mintersect([G, G, G], tup)
+    c2, o2, l2 = orc.mintersect_sorted([(off, adj)] * 3, tup)
+    assert np.array_equal(counts, c2)
+    assert np.array_equal(offsets, o2)
+    assert np.array_equal(lists, l2)
+    assert int(c2.max()) > 1000  # the giants' intersections are large

@@ -103,6 +103,27 @@ def test_noalloc_equals_device_entry_point(orc, graphs):
         dev.close()
 
 
+def test_noalloc_giant_tuples(orc):
+    """Tuples whose shortest list exceeds 8192 entries (counted apart in the
+    shared array): a mixed call, then a smaller one of giants only."""
+    from test_mintersect_gpu import giant_tuples, giants_graph
+    off, adj = giants_graph()
+    G = vs.Graph(off, adj, symmetric=False)
+    dev = Dev()
+    try:
+        for T, seed in ((5000, 41), (30, 42)):
+            tup = giant_tuples(T, len(off) - 1, seed)
+            ref = run("vs_mintersect_device", [G, G, G], tup, dev)
+            got = run("vs_mintersect_device_noalloc", [G, G, G], tup, dev)
+            assert got[3] == ref[3]
+            for a, b in zip(got[:3], ref[:3]):
+                assert np.array_equal(a, b)
+            c2, o2, l2 = orc.mintersect_sorted([(off, adj)] * 3, tup)
+            assert np.array_equal(got[0], c2) and np.array_equal(got[2], l2)
+    finally:
+        dev.close()
+
+
 def test_noalloc_capacity_and_limits(graphs):
     g, G, _, _ = graphs
     dev = Dev()

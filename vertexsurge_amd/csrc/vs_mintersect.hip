@@ -57,11 +57,18 @@ struct MiArgs {
   // tuples split by the length of their shortest list: at most 8, 16, 32
   // entries (eight, four, two tuples per wave), and the rest (one per wave)
   const uint32_t *order[4];
-  const ull *norder;  // device counts of the four lists
+  const ull *norder;  // device counts: the four lists, then the giants (kCounters)
 };
 
 constexpr int kBuckets = 4;
 __host__ __device__ constexpr uint32_t bucket_len(int k) { return 8u << k; }  // 8, 16, 32
+// Tuples whose shortest list exceeds kGiantLen entries ("giants": a few
+// hundred of the 4 M random 3-paths of R-MAT 24, up to 40 K entries) are
+// counted separately and stored from the back of the long list; each gets
+// a wave of its own.  Strided with the others, the wave holding the largest
+// took 2.6x the mean work and the kernel waited for it.
+constexpr int kLong = kBuckets - 1, kGiant = kBuckets, kCounters = kBuckets + 1;
+constexpr uint64_t kGiantLen = 8192;
 
 // A tuple's lists: start and length of each, and the shortest (the driver).
 // Returns false if an id is outside its dependency (empty intersection).
@@ -173,44 +180,61 @@ constexpr int kMiCount = 0, kMiFill = 1, kMiStage = 2;
 // Tuples with a long shortest list: one wave per tuple, 64 driver elements
 // per step; each step's searches start where lane 63's landed.
 template <int MODE, int N>
-__global__ __launch_bounds__(kBlock) void k_mintersect(MiArgs a) {
+__device__ __forceinline__ void mi_long_tuple(const MiArgs &a, uint64_t t, int lane) {
   constexpr bool WRITE = MODE != kMiCount, COUNT = MODE != kMiFill;
+  uint64_t b[N];
+  uint32_t n[N];
+  int drv;
+  uint64_t best;
+  const bool ok = tuple_lists<N>(a, t, b, n, drv, best);
+  uint32_t total = 0;
+  uint64_t out = WRITE ? a.offsets[t] : 0;
+  if (ok && best > 0) {
+    const uint32_t *dl = a.adj[drv];
+    uint32_t from[N];
+#pragma unroll
+    for (int d = 0; d < N; ++d) from[d] = 0;
+    for (uint64_t base = b[drv]; base < b[drv] + best; base += 64) {
+      const uint64_t idx = base + lane;
+      const bool have = idx < b[drv] + best;
+      const uint32_t x = have ? dl[idx] : 0xFFFFFFFFu;
+      uint32_t lo[N];
+      uint64_t searched[N];
+      const bool keep = member_all<N>(a, b, n, drv, from, x, have, lo, searched);
+      // the next step's elements are all larger than this step's: a list
+      // resumes where its highest searching lane landed
+#pragma unroll
+      for (int d = 0; d < N; ++d)
+        if (searched[d]) from[d] = __shfl(lo[d], 63 - __clzll(searched[d]));
+      const uint64_t m = __ballot(keep);
+      if (WRITE && keep) a.lists[out + lane_rank(m)] = x;
+      out += __popcll(m);
+      total += __popcll(m);
+    }
+  }
+  if (COUNT && lane == 0) a.counts[t] = total;
+}
+
+// Tuples with a long shortest list: one wave per tuple, 64 driver elements
+// per step; each step's searches start where lane 63's landed.  The giants
+// (stored backwards from the end of the long list) go first, one wave each
+// while there are fewer of them than waves; the other waves share the rest.
+template <int MODE, int N>
+__global__ __launch_bounds__(kBlock) void k_mintersect(MiArgs a) {
   const int lane = lane_id(), wid = wave_id();
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
-  const uint64_t cnt = a.norder[kBuckets - 1];
-  for (uint64_t i = blockIdx.x * kWavesPerBlock + wid; i < cnt; i += nwaves) {
-    const uint64_t t = a.order[kBuckets - 1][i];
-    uint64_t b[N];
-    uint32_t n[N];
-    int drv;
-    uint64_t best;
-    const bool ok = tuple_lists<N>(a, t, b, n, drv, best);
-    uint32_t total = 0;
-    uint64_t out = WRITE ? a.offsets[t] : 0;
-    if (ok && best > 0) {
-      const uint32_t *dl = a.adj[drv];
-      uint32_t from[N];
-#pragma unroll
-      for (int d = 0; d < N; ++d) from[d] = 0;
-      for (uint64_t base = b[drv]; base < b[drv] + best; base += 64) {
-        const uint64_t idx = base + lane;
-        const bool have = idx < b[drv] + best;
-        const uint32_t x = have ? dl[idx] : 0xFFFFFFFFu;
-        uint32_t lo[N];
-        uint64_t searched[N];
-        const bool keep = member_all<N>(a, b, n, drv, from, x, have, lo, searched);
-        // the next step's elements are all larger than this step's: a list
-        // resumes where its highest searching lane landed
-#pragma unroll
-        for (int d = 0; d < N; ++d)
-          if (searched[d]) from[d] = __shfl(lo[d], 63 - __clzll(searched[d]));
-        const uint64_t m = __ballot(keep);
-        if (WRITE && keep) a.lists[out + lane_rank(m)] = x;
-        out += __popcll(m);
-        total += __popcll(m);
-      }
+  const uint64_t w = static_cast<uint64_t>(blockIdx.x) * kWavesPerBlock + wid;
+  const uint64_t cnt = a.norder[kLong], ng = a.norder[kGiant];
+  const uint32_t *lst = a.order[kLong];
+  if (ng < nwaves) {
+    if (w < ng) {
+      mi_long_tuple<MODE, N>(a, lst[a.T - 1 - w], lane);
+    } else {
+      for (uint64_t i = w - ng; i < cnt; i += nwaves - ng) mi_long_tuple<MODE, N>(a, lst[i], lane);
     }
-    if (COUNT && lane == 0) a.counts[t] = total;
+  } else {
+    for (uint64_t i = w; i < ng; i += nwaves) mi_long_tuple<MODE, N>(a, lst[a.T - 1 - i], lane);
+    for (uint64_t i = w; i < cnt; i += nwaves) mi_long_tuple<MODE, N>(a, lst[i], lane);
   }
 }
 
@@ -261,14 +285,14 @@ constexpr int kSplitRounds = 8;
 template <int N>
 __global__ __launch_bounds__(kBlock) void k_mi_split(MiArgs a, uint32_t *lists, ull *norder,
                                                     uint32_t *bound) {
-  __shared__ uint32_t s_wave[kWavesPerBlock][kBuckets];
-  __shared__ ull s_base[kBuckets];
+  __shared__ uint32_t s_wave[kWavesPerBlock][kCounters];
+  __shared__ ull s_base[kCounters];
   const int lane = lane_id(), wid = wave_id();
   constexpr uint64_t kChunk = static_cast<uint64_t>(kBlock) * kSplitRounds;
   // block-uniform loop: every thread reaches the barriers
   for (uint64_t c0 = blockIdx.x * kChunk; c0 < a.T; c0 += gridDim.x * kChunk) {
     int kk[kSplitRounds];
-    uint32_t cnt[kBuckets] = {};
+    uint32_t cnt[kCounters] = {};
 #pragma unroll
     for (int r = 0; r < kSplitRounds; ++r) {
       const uint64_t t = c0 + static_cast<uint64_t>(r) * kBlock + threadIdx.x;
@@ -279,18 +303,18 @@ __global__ __launch_bounds__(kBlock) void k_mi_split(MiArgs a, uint32_t *lists, 
         int drv;
         uint64_t best;
         if (!tuple_lists<N>(a, t, b, n, drv, best)) best = 0;
-        int k = kBuckets - 1;
+        int k = best > kGiantLen ? kGiant : kLong;
         for (int q = kBuckets - 2; q >= 0; --q)
           if (best <= bucket_len(q)) k = q;
         if (bound) bound[t] = static_cast<uint32_t>(best);  // a list holds < 2^32 ids
         kk[r] = k;
 #pragma unroll
-        for (int q = 0; q < kBuckets; ++q) cnt[q] += k == q;
+        for (int q = 0; q < kCounters; ++q) cnt[q] += k == q;
       }
     }
-    uint32_t inc[kBuckets];
+    uint32_t inc[kCounters];
 #pragma unroll
-    for (int q = 0; q < kBuckets; ++q) {
+    for (int q = 0; q < kCounters; ++q) {
       inc[q] = cnt[q];
 #pragma unroll
       for (int d = 1; d < 64; d <<= 1) {
@@ -300,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_mi_split(MiArgs a, uint32_t *lists, 
       if (lane == 63) s_wave[wid][q] = inc[q];
     }
     __syncthreads();
-    if (threadIdx.x < kBuckets) {
+    if (threadIdx.x < kCounters) {
       const int q = threadIdx.x;
       uint32_t tot = 0;
       for (int w = 0; w < kWavesPerBlock; ++w) {
@@ -311,9 +335,9 @@ __global__ __launch_bounds__(kBlock) void k_mi_split(MiArgs a, uint32_t *lists, 
       s_base[q] = tot ? atomicAdd(&norder[q], static_cast<ull>(tot)) : 0;
     }
     __syncthreads();
-    uint64_t pos[kBuckets];
+    uint64_t pos[kCounters];
 #pragma unroll
-    for (int q = 0; q < kBuckets; ++q) pos[q] = s_base[q] + s_wave[wid][q] + inc[q] - cnt[q];
+    for (int q = 0; q < kCounters; ++q) pos[q] = s_base[q] + s_wave[wid][q] + inc[q] - cnt[q];
 #pragma unroll
     for (int r = 0; r < kSplitRounds; ++r) {
       const uint32_t t = static_cast<uint32_t>(c0 + static_cast<uint64_t>(r) * kBlock +
@@ -322,6 +346,8 @@ __global__ __launch_bounds__(kBlock) void k_mi_split(MiArgs a, uint32_t *lists, 
 #pragma unroll
       for (int q = 0; q < kBuckets; ++q)
         if (kk[r] == q) lists[q * a.T + pos[q]++] = t;
+      // giants from the back of the long list (long + giants <= T)
+      if (kk[r] == kGiant) lists[kLong * a.T + (a.T - 1 - pos[kGiant]++)] = t;
     }
     __syncthreads();  // s_wave / s_base are rewritten by the next chunk
   }
@@ -618,7 +644,7 @@ vs_status mintersect_run(const vs_graph_t *deps, uint32_t arity, const uint32_t 
   } back{g->pool};
   // tuple lists by driver length: room for T ids in each bucket, counters
   void *lp = nullptr, *np = nullptr;
-  const size_t lbytes = kBuckets * T * 4, nbytes = kBuckets * 8;
+  const size_t lbytes = kBuckets * T * 4, nbytes = kCounters * 8;
   VS_TRY(g->pool.get(lbytes, false, s, &lp));
   back.add(lp, lbytes);
   VS_TRY(g->pool.get(nbytes, false, s, &np));
@@ -756,7 +782,7 @@ vs_status mintersect_run(const vs_graph_t *deps, uint32_t arity, const uint32_t 
 // vs_mintersect_device_noalloc runs the two-pass intersection (split ->
 // count -> scan -> fill) in a fixed device array instead of pool buffers,
 // one copy per GPU, held until the process ends:
-//   [0, 256)                       the four bucket counters (ull)
+//   [0, 256)                       the bucket and giant counters (ull)
 //   [256, kNaHeader)               per-block sums of the counts scan and
 //                                  their total (ull, at most 1153 entries)
 //   [kNaHeader, kNaScratchBytes)   the four bucket lists, T ids each
@@ -947,7 +973,7 @@ static vs_status mintersect_noalloc(const vs_graph_t *deps, uint32_t arity,
   a.counts = d_counts;
   for (int k = 0; k < kBuckets; ++k) a.order[k] = buckets + static_cast<size_t>(k) * T;
   a.norder = norder;
-  hipLaunchKernelGGL(k_na_zero, dim3(1), dim3(64), 0, s, norder, static_cast<uint32_t>(kBuckets));
+  hipLaunchKernelGGL(k_na_zero, dim3(1), dim3(64), 0, s, norder, static_cast<uint32_t>(kCounters));
   const uint32_t grid = grid_for(T, g->num_cus);
   const uint64_t split_chunks = (T + kBlock * kSplitRounds - 1) / (kBlock * kSplitRounds);
   launch_mi_split(arity, grid_for(split_chunks * kWavesPerBlock, g->num_cus), s, a, buckets,
