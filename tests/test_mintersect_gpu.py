@@ -140,14 +140,15 @@ def test_device_capacity_error_keeps_counts(orc):
 
 def giants_graph(seed=5):
     """Three vertices whose lists are longer than the 8192 entries above which
-    a tuple's shortest list gets a wave of its own, among short lists."""
+    a tuple's shortest list gets a wave of its own, two whose lists exceed the
+    2048 entries of the next class, among short lists."""
     rng = np.random.default_rng(seed)
     V = 40000
     lists = []
     for v in range(V):
-        if v < 3:
-            k = 9000 + 3000 * v
-            nb = rng.choice(np.arange(3, V), size=k, replace=False)
+        if v < 5:
+            k = 9000 + 3000 * v if v < 3 else 3000 + 2000 * (v - 3)
+            nb = rng.choice(np.arange(5, V), size=k, replace=False)
         else:
             nb = rng.choice(V, size=int(rng.integers(0, 12)), replace=False)
         lists.append(np.sort(nb).astype(np.uint32))
@@ -156,23 +157,25 @@ def giants_graph(seed=5):
     return off, np.concatenate(lists)
 
 
-def giant_tuples(T, V, seed):
-    """min(T, 40) tuples over the three long lists, the rest random."""
+def giant_tuples(T, V, seed, hi=5):
+    """min(T, 40) tuples over the first hi vertices (the long lists), the rest
+    random."""
     rng = np.random.default_rng(seed)
     ng = min(T, 40)
-    tup = np.concatenate([rng.integers(0, 3, size=(ng, 3)),
+    tup = np.concatenate([rng.integers(0, hi, size=(ng, 3)),
                           rng.integers(0, V, size=(T - ng, 3))]).astype(np.uint32)
     rng.shuffle(tup)
     return tup
 
 
 # T = 20: as many giants as waves (they are strided); T = 30: one wave
-# each with waves to spare; T = 5000: giants among short and long tuples
+# each with waves to spare; T = 5000: giants and the next class among short
+# and long tuples
 @pytest.mark.parametrize("T", [20, 30, 5000])
 def test_giant_tuples(orc, T):
     off, adj = giants_graph()
     G = vs.Graph(off, adj, symmetric=False)
-    tup = giant_tuples(T, len(off) - 1, seed=T)
+    tup = giant_tuples(T, len(off) - 1, seed=T, hi=3 if T <= 30 else 5)
     counts, offsets, lists, st = vs.mintersect([G, G, G], tup)
     c2, o2, l2 = orc.mintersect_sorted([(off, adj)] * 3, tup)
     assert np.array_equal(counts, c2)

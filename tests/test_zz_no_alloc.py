@@ -112,7 +112,7 @@ def test_noalloc_giant_tuples(orc):
     dev = Dev()
     try:
         for T, seed in ((5000, 41), (30, 42)):
-            tup = giant_tuples(T, len(off) - 1, seed)
+            tup = giant_tuples(T, len(off) - 1, seed, hi=5 if T > 30 else 3)
             ref = run("vs_mintersect_device", [G, G, G], tup, dev)
             got = run("vs_mintersect_device_noalloc", [G, G, G], tup, dev)
             assert got[3] == ref[3]

@@ -66,9 +66,13 @@ __host__ __device__ constexpr uint32_t bucket_len(int k) { return 8u << k; }  //
 // hundred of the 4 M random 3-paths of R-MAT 24, up to 40 K entries) are
 // counted separately and stored from the back of the long list; each gets
 // a wave of its own.  Strided with the others, the wave holding the largest
-// took 2.6x the mean work and the kernel waited for it.
-constexpr int kLong = kBuckets - 1, kGiant = kBuckets, kCounters = kBuckets + 1;
-constexpr uint64_t kGiantLen = 8192;
+// took 2.6x the mean work and the kernel waited for it.  The next class
+// ("large", over kLargeLen entries, stored from the back of the 32-entry
+// bucket's list) is dealt out first, one per wave, so that no wave draws
+// several of them on top of its share of the rest.
+constexpr int kLong = kBuckets - 1, kGiant = kBuckets, kLarge = kBuckets + 1;
+constexpr int kCounters = kBuckets + 2;
+constexpr uint64_t kGiantLen = 8192, kLargeLen = 2048;
 
 // A tuple's lists: start and length of each, and the shortest (the driver).
 // Returns false if an id is outside its dependency (empty intersection).
@@ -224,16 +228,19 @@ __global__ __launch_bounds__(kBlock) void k_mintersect(MiArgs a) {
   const int lane = lane_id(), wid = wave_id();
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * kWavesPerBlock;
   const uint64_t w = static_cast<uint64_t>(blockIdx.x) * kWavesPerBlock + wid;
-  const uint64_t cnt = a.norder[kLong], ng = a.norder[kGiant];
-  const uint32_t *lst = a.order[kLong];
+  const uint64_t cnt = a.norder[kLong], ng = a.norder[kGiant], nl = a.norder[kLarge];
+  const uint32_t *lst = a.order[kLong], *lrg = a.order[kLong - 1];
   if (ng < nwaves) {
     if (w < ng) {
       mi_long_tuple<MODE, N>(a, lst[a.T - 1 - w], lane);
     } else {
-      for (uint64_t i = w - ng; i < cnt; i += nwaves - ng) mi_long_tuple<MODE, N>(a, lst[i], lane);
+      const uint64_t rw = nwaves - ng;
+      for (uint64_t i = w - ng; i < nl; i += rw) mi_long_tuple<MODE, N>(a, lrg[a.T - 1 - i], lane);
+      for (uint64_t i = w - ng; i < cnt; i += rw) mi_long_tuple<MODE, N>(a, lst[i], lane);
     }
   } else {
     for (uint64_t i = w; i < ng; i += nwaves) mi_long_tuple<MODE, N>(a, lst[a.T - 1 - i], lane);
+    for (uint64_t i = w; i < nl; i += nwaves) mi_long_tuple<MODE, N>(a, lrg[a.T - 1 - i], lane);
     for (uint64_t i = w; i < cnt; i += nwaves) mi_long_tuple<MODE, N>(a, lst[i], lane);
   }
 }
@@ -303,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_mi_split(MiArgs a, uint32_t *lists, 
         int drv;
         uint64_t best;
         if (!tuple_lists<N>(a, t, b, n, drv, best)) best = 0;
-        int k = best > kGiantLen ? kGiant : kLong;
+        int k = best > kGiantLen ? kGiant : best > kLargeLen ? kLarge : kLong;
         for (int q = kBuckets - 2; q >= 0; --q)
           if (best <= bucket_len(q)) k = q;
         if (bound) bound[t] = static_cast<uint32_t>(best);  // a list holds < 2^32 ids
@@ -346,8 +353,10 @@ __global__ __launch_bounds__(kBlock) void k_mi_split(MiArgs a, uint32_t *lists, 
 #pragma unroll
       for (int q = 0; q < kBuckets; ++q)
         if (kk[r] == q) lists[q * a.T + pos[q]++] = t;
-      // giants from the back of the long list (long + giants <= T)
+      // giants from the back of the long list (long + giants <= T), large
+      // tuples from the back of the 32-entry bucket's (that bucket + large <= T)
       if (kk[r] == kGiant) lists[kLong * a.T + (a.T - 1 - pos[kGiant]++)] = t;
+      if (kk[r] == kLarge) lists[(kLong - 1) * a.T + (a.T - 1 - pos[kLarge]++)] = t;
     }
     __syncthreads();  // s_wave / s_base are rewritten by the next chunk
   }
@@ -782,7 +791,7 @@ vs_status mintersect_run(const vs_graph_t *deps, uint32_t arity, const uint32_t 
 // vs_mintersect_device_noalloc runs the two-pass intersection (split ->
 // count -> scan -> fill) in a fixed device array instead of pool buffers,
 // one copy per GPU, held until the process ends:
-//   [0, 256)                       the bucket and giant counters (ull)
+//   [0, 256)                       the bucket, giant and large counters (ull)
 //   [256, kNaHeader)               per-block sums of the counts scan and
 //                                  their total (ull, at most 1153 entries)
 //   [kNaHeader, kNaScratchBytes)   the four bucket lists, T ids each
