@@ -301,10 +301,13 @@ __global__ void k_bw_assign(BatchConst c) {
 
 // ---- zero visited / result matrices of active slots (fill kernel: no
 // memset node inside the captured graph)
+// Each block zeroes chunks of kZeroChunk 16-byte words (the launch covers
+// the matrix with one chunk per block where it can).
+constexpr size_t kZeroChunk = static_cast<size_t>(kBlock) * 8;
+
 __global__ __launch_bounds__(kBlock) void k_bw_zero(BatchConst c) {
   const size_t words = static_cast<size_t>(c.V) * c.TW;
-  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
-  const size_t tid = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  const size_t n4 = words / 2;
   for (uint32_t s = 0; s < c.nslots; ++s) {
     const SlotDesc &sd = c.slots[s];
     if (sd.tile == UINT32_MAX) continue;
@@ -313,8 +316,11 @@ __global__ __launch_bounds__(kBlock) void k_bw_zero(BatchConst c) {
       if (!M) continue;
       // 16-byte stores (the matrices are hipMalloc'd, 256-byte aligned)
       uint4 *M4 = reinterpret_cast<uint4 *>(M);
-      for (size_t i = tid; i < words / 2; i += stride) M4[i] = make_uint4(0, 0, 0, 0);
-      if ((words & 1) && tid == 0) M[words - 1] = 0;
+      for (size_t b0 = blockIdx.x * kZeroChunk; b0 < n4; b0 += gridDim.x * kZeroChunk) {
+        const size_t e = b0 + kZeroChunk < n4 ? b0 + kZeroChunk : n4;
+        for (size_t i = b0 + threadIdx.x; i < e; i += kBlock) M4[i] = make_uint4(0, 0, 0, 0);
+      }
+      if ((words & 1) && blockIdx.x == 0 && threadIdx.x == 0) M[words - 1] = 0;
     }
   }
 }
@@ -1022,7 +1028,12 @@ static uint32_t enqueue_wave(const BatchConst &c, bool shortest, uint32_t grid,
                              hipStream_t s, ExpandEvents *ev = nullptr) {
   uint32_t n = 0;
   hipLaunchKernelGGL(k_bw_assign, dim3(1), dim3(256), 0, s, c);
-  hipLaunchKernelGGL(k_bw_zero, dim3(grid), dim3(kBlock), 0, s, c);
+  {
+    const size_t n4 = static_cast<size_t>(c.V) * c.TW / 2;
+    const size_t zb = std::min<size_t>((n4 + kZeroChunk - 1) / kZeroChunk, size_t{1} << 20);
+    hipLaunchKernelGGL(k_bw_zero, dim3(static_cast<uint32_t>(std::max<size_t>(zb, 1))),
+                       dim3(kBlock), 0, s, c);
+  }
   hipLaunchKernelGGL(k_bw_l0, dim3(std::max<uint32_t>(1, (c.nslots * c.TW * 64 + 255) / 256)),
                      dim3(256), 0, s, c);
   n += 3;
