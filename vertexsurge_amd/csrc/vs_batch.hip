@@ -303,7 +303,7 @@ __global__ void k_bw_assign(BatchConst c) {
 // memset node inside the captured graph)
 // Each block zeroes chunks of kZeroChunk 16-byte words (the launch covers
 // the matrix with one chunk per block where it can).
-constexpr size_t kZeroChunk = static_cast<size_t>(kBlock) * 8;
+constexpr size_t kZeroChunk = static_cast<size_t>(kBlock) * 2;
 
 __global__ __launch_bounds__(kBlock) void k_bw_zero(BatchConst c) {
   const size_t words = static_cast<size_t>(c.V) * c.TW;
@@ -572,8 +572,11 @@ void k_bw_expand(BatchConst c, uint32_t hop) {
         } else {
           pn = u + nwaves < se && pull_prefetch<TW, SPARSE>(a, h1, mk, j, un);
         }
-        pull_body<TW, SHORTEST, SPARSE>(a, h0, s_nbr[wid], s_nm[wid], s_pairs[wid], s_acc[wid],
-                                        s_wl[wid], mk, lc, u0, p0, bw0, p0 && bmp);
+        // word-mask pulls scan every in-edge: their next group's ids load
+        // during the current group's gathers
+        pull_body<TW, SHORTEST, SPARSE, SPARSE && TW == 64>(
+            a, h0, s_nbr[wid], s_nm[wid], s_pairs[wid], s_acc[wid], s_wl[wid], mk, lc, u0, p0,
+            bw0, p0 && bmp);
         h0 = h1;
         if constexpr (kDeep) {
           h1 = h2;
@@ -1030,7 +1033,7 @@ static uint32_t enqueue_wave(const BatchConst &c, bool shortest, uint32_t grid,
   hipLaunchKernelGGL(k_bw_assign, dim3(1), dim3(256), 0, s, c);
   {
     const size_t n4 = static_cast<size_t>(c.V) * c.TW / 2;
-    const size_t zb = std::min<size_t>((n4 + kZeroChunk - 1) / kZeroChunk, size_t{1} << 20);
+    const size_t zb = std::min<size_t>((n4 + kZeroChunk - 1) / kZeroChunk, size_t{1} << 22);
     hipLaunchKernelGGL(k_bw_zero, dim3(static_cast<uint32_t>(std::max<size_t>(zb, 1))),
                        dim3(kBlock), 0, s, c);
   }

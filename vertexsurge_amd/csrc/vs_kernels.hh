@@ -420,7 +420,8 @@ __device__ __forceinline__ void gather_cross(const HopArgs &a, const uint32_t *n
 // otherwise every lane gathers its word of every active neighbour, or, when
 // the (neighbour, needed word) cross product fits kCross pairs, that cross
 // product is gathered flat over the lanes.
-template <int TW, bool SHORTEST, bool SPARSE>
+// GPF: a later group's ids are loaded while the group before it gathers.
+template <int TW, bool SHORTEST, bool SPARSE, bool GPF = false>
 __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, uint32_t *nbr,
                                           uint64_t *nm, uint16_t *pairs, uint64_t *accw,
                                           uint8_t *wl, const LaneMasks &mk, LocalCtr &c,
@@ -452,19 +453,24 @@ __device__ __forceinline__ void pull_body(const HopArgs &a, const PullHead &h, u
   const bool dense = a.dense != 0;
   const uint64_t hub_end = h.hubs == 0xffffffffu ? le : lb + h.hubs;
   if (lane < TW) accw[lane] = 0;
+  uint32_t un = 0;  // GPF: the next group's ids
   for (uint64_t base = b; base < e; base += 64) {
     const uint64_t idx = base + lane;
     uint32_t u = 0;
     bool act = false;
+    uint32_t unext = 0;
+    if (GPF && idx + 64 < e) unext = a.padj[idx + 64];
     if (idx < e) {
-      // the first group's ids may have been loaded while the previous item ran
+      // the first group's ids may have been loaded while the previous item
+      // ran (and with GPF a later group's during the group before it)
       const bool first = base == b;
-      u = pre && first ? u0 : a.padj[idx];
+      u = first ? (pre ? u0 : a.padj[idx]) : (GPF ? un : a.padj[idx]);
       // a dense frontier skips the bitmap lookup: inactive rows are zero;
       // the first group's bitmap words may also have been loaded ahead
       act = dense || (prebm && first ? ((bw0 >> (u & 31)) & 1)
                                      : ((a.actbm[u >> 6] >> (u & 63)) & 1));
     }
+    un = unext;
     const uint64_t m = __ballot(act);
     const uint32_t n = __popcll(m);
     // hub neighbours precede the others in the list, so after compaction the
